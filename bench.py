@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X Euler step on the 2D 4096^2 split MUSCL-Hancock
++ van Leer + HLLC workload (FP64), as cell-updates/s and HBM roofline.
+
+A step is one full time step of the simulator loop
+(simulator/simulator_impl.hpp:235-252): dt from the max wavespeed (fused into
+the previous step's last sweep, computed on the device), the x sweep and the
+y sweep.  Inputs are resident in HBM before the timed region starts.
+
+Prints ONE JSON line.  Only the cpu_baseline leg loads the CPU oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E bandwidth
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-variants", action="store_true", help="skip FORCE / FP32 lines")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--x-block", type=int, default=0)
+    ap.add_argument("--x-rows", type=int, default=0)
+    ap.add_argument("--y-block", type=int, default=0)
+    ap.add_argument("--y-rows", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU decomposition is a design, not a measured configuration.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fluidity_amd import euler as E
+    from fluidity_amd import native as N
+    from fluidity_amd import workloads as W
+
+    n = args.n
+    wl = W.riemann_2d(n=n, seed=1001)
+    dh = wl.dh
+    cfl = 0.9
+    cells = n * n
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+
+    def make_solver(flux, layout=N.FL_LAYOUT_SOA):
+        s = E.SplitSolver(E.Scheme(flux=flux), layout=layout)
+        s.set_tuning(x_block=args.x_block, x_rows=args.x_rows, y_block=args.y_block,
+                     y_rows=args.y_rows)
+        return s
+
+    U_soa = W.aos_to_soa(wl.U)
+
+    def run_case(flux, dtype, steps, warmup):
+        solver = make_solver(flux)
+        with torch.cuda.stream(stream):
+            a = torch.from_numpy(U_soa).to(device=dev, dtype=dtype)
+            w = torch.empty_like(a)
+        stream.synchronize()
+        if warmup:
+            solver.simulate(a, w, cfl, dh, max_iters=warmup, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = solver.simulate(a, w, cfl, dh, max_iters=steps, stream=stream)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        assert r.iters == steps, (r.iters, steps)
+        return solver, a, w, el, r
+
+    # ---- headline: FP64 HLLC
+    solver, a, w, elapsed, res = run_case(N.FL_FLUX_HLLC, torch.float64, args.steps, args.warmup)
+    value = args.steps * cells / elapsed
+    ms_per_step = elapsed * 1e3 / args.steps
+
+    # ---- per-sweep kernel time with HIP events on the launch stream
+    dtdh = res.last_dt / dh
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    reps = max(5, min(args.steps, 20))
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for _ in range(reps):
+            solver.sweep(a, w, 0, dtdh, stream=stream)
+        ev[1].record(stream)
+        for _ in range(reps):
+            solver.sweep(w, a, 1, dtdh, stream=stream)
+        ev[2].record(stream)
+    stream.synchronize()
+    t_x = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    t_y = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    bytes_per_sweep = cells * 64.0            # read 32 B + write 32 B per cell (FP64, 2D)
+    dom_name, dom_t = ("sweep_x_kernel<double,2,HLLC>", t_x) if t_x >= t_y else \
+        ("sweep_y_kernel<double,HLLC>", t_y)
+    achieved = bytes_per_sweep / dom_t / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom_name,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "x_sweep_us": round(t_x * 1e6, 1),
+        "y_sweep_us": round(t_y * 1e6, 1),
+        "step_bytes_per_cell": 128,
+        "step_frac": round(value * 128 / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    del a, w, solver
+    torch.cuda.empty_cache()
+
+    variants = {}
+    if not args.no_variants:
+        vsteps = max(10, args.steps // 2)
+        for name, flux, dt, bpc in [("force_f64", N.FL_FLUX_FORCE, torch.float64, 128),
+                                     ("hllc_f32", N.FL_FLUX_HLLC, torch.float32, 64)]:
+            s2, a2, w2, el2, _ = run_case(flux, dt, vsteps, 2)
+            v = vsteps * cells / el2
+            variants[name] = {"value": round(v, 1), "unit": "cell-updates/s",
+                              "ms_per_step": round(el2 * 1e3 / vsteps, 4),
+                              "hbm_frac": round(v * bpc / 1e9 / HBM_PEAK_GBS, 4),
+                              "vs_f64_hllc": round(v / value, 3)}
+            del s2, a2, w2
+            torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl, cfl, args.cpu_steps)
+
+    line = {
+        "metric": "cell-updates/s, 2D 4096^2 split MUSCL-Hancock+HLLC FP64",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded four-quadrant Riemann data, seed 1001)",
+        "config": {"workload": f"2D Euler {n}x{n}, gamma 1.4, MUSCL-Hancock + van Leer + HLLC, "
+                               "dimensionally split x then y, transmissive, CFL 0.9",
+                   "nx": n, "ny": n, "layout": "SoA", "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(wl, cfl, steps):
+    """The CPU oracle (a restatement of the reference path, OpenMP over rows /
+    columns) on the same 4096^2 grid for a bounded number of steps."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    s = O.scheme(flux=O.FLUX_HLLC)
+    t0 = time.perf_counter()
+    _, _, it, _ = O.simulate(wl.U, s, cfl, wl.dh, max_iters=steps, nthreads=threads)
+    el = time.perf_counter() - t0
+    cells = wl.U.shape[0] * wl.U.shape[1]
+    return {"value": round(it * cells / el, 1), "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{it} full step(s) of the same {wl.U.shape[1]}x{wl.U.shape[0]} FP64 "
+                      f"HLLC workload, {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,491 @@
+// fl_abi.hip -- host side of the C ABI declared in include/fluidity_amd.h.
+//
+// Validates arguments, builds the kernel argument blocks and launches the
+// gfx950 kernels of fl_sweep.hip on the caller's stream.  No allocation
+// happens outside fl_ctx_create.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/fluidity_amd.h"
+#include "fl_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+#define FL_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(FL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr int kSumBlocks = 1024;
+
+}  // namespace
+
+struct fl_context {
+  int device;
+  fl::StepCtl* ctl;            // device control block
+  unsigned long long* ws_tmp;  // device scratch for fl_max_wavespeed
+  double* partial;             // device partial sums, kSumBlocks * 4
+  fl::StepCtl* host_ctl;       // pinned host mirror of ctl
+  double* host_partial;        // pinned host mirror of partial
+  fl_tuning tuning;
+};
+
+namespace {
+
+bool valid_block(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0; }
+
+int check_field(const fl_field* f, const char* name) {
+  if (!f) return fail(FL_ERR_ARGUMENT, "%s is NULL", name);
+  if (!f->data) return fail(FL_ERR_ARGUMENT, "%s->data is NULL", name);
+  if (f->dtype != FL_F64 && f->dtype != FL_F32)
+    return fail(FL_ERR_ARGUMENT, "%s->dtype %d is not FL_F64/FL_F32", name, f->dtype);
+  if (f->layout != FL_LAYOUT_AOS && f->layout != FL_LAYOUT_SOA)
+    return fail(FL_ERR_ARGUMENT, "%s->layout %d is not AoS/SoA", name, f->layout);
+  if (f->ndim < 1 || f->ndim > 3)
+    return fail(FL_ERR_ARGUMENT, "%s->ndim %d out of range", name, f->ndim);
+  if (f->ndim == 3) return fail(FL_ERR_UNSUPPORTED, "%s: 3D grids are not supported", name);
+  for (int d = 0; d < f->ndim; ++d)
+    if (f->n[d] < 4)
+      return fail(FL_ERR_ARGUMENT, "%s->n[%d] = %lld: need at least 4 cells (ghost pad)",
+                  name, d, (long long)f->n[d]);
+  if (f->ndim == 2 && f->n[1] > (int64_t)65535 * 4096)
+    return fail(FL_ERR_ARGUMENT, "%s->n[1] too large", name);
+  return FL_OK;
+}
+
+int check_pair(const fl_field* in, const fl_field* out) {
+  int rc = check_field(in, "in");
+  if (rc) return rc;
+  rc = check_field(out, "out");
+  if (rc) return rc;
+  if (in->dtype != out->dtype || in->layout != out->layout || in->ndim != out->ndim)
+    return fail(FL_ERR_ARGUMENT, "in and out differ in dtype/layout/ndim");
+  for (int d = 0; d < in->ndim; ++d)
+    if (in->n[d] != out->n[d]) return fail(FL_ERR_ARGUMENT, "in and out differ in shape");
+  if (in->data == out->data)
+    return fail(FL_ERR_ARGUMENT, "in and out alias: a sweep cannot update in place");
+  return FL_OK;
+}
+
+int check_scheme(const fl_scheme* s, int ndim) {
+  if (!s) return fail(FL_ERR_ARGUMENT, "scheme is NULL");
+  if (s->limiter != FL_LIMITER_VAN_LEER && s->limiter != FL_LIMITER_SUPERBEE)
+    return fail(FL_ERR_ARGUMENT, "scheme->limiter %d invalid", s->limiter);
+  if (s->form != FL_FORM_CONSERVATIVE && s->form != FL_FORM_PRIMITIVE)
+    return fail(FL_ERR_ARGUMENT, "scheme->form %d invalid", s->form);
+  if (s->flux != FL_FLUX_HLLC && s->flux != FL_FLUX_FORCE)
+    return fail(FL_ERR_ARGUMENT, "scheme->flux %d invalid", s->flux);
+  if (s->eos != FL_EOS_IDEAL && s->eos != FL_EOS_STIFF)
+    return fail(FL_ERR_ARGUMENT, "scheme->eos %d invalid", s->eos);
+  if (!(s->gamma > 1.0) || !isfinite(s->gamma))
+    return fail(FL_ERR_ARGUMENT, "scheme->gamma %g must be finite and > 1", s->gamma);
+  if (!isfinite(s->p_inf)) return fail(FL_ERR_ARGUMENT, "scheme->p_inf not finite");
+  for (int d = 0; d < ndim; ++d)
+    if (s->bc[d] < FL_BC_TRANSMISSIVE || s->bc[d] > FL_BC_PERIODIC)
+      return fail(FL_ERR_ARGUMENT, "scheme->bc[%d] %d invalid", d, s->bc[d]);
+  return FL_OK;
+}
+
+template <typename T>
+fl::Material<T> make_material(const fl_scheme* s) {
+  fl::Material<T> m;
+  m.gamma = (T)s->gamma;
+  m.p_inf = (T)s->p_inf;
+  m.gm1 = m.gamma - T(1);
+  m.q = (m.gamma + T(1)) / (T(2) * m.gamma);
+  m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
+  return m;
+}
+
+void strides(const fl_field* f, int64_t& es, int64_t& cs) {
+  const int S = 2 + f->ndim;
+  const int64_t cells = f->n[0] * (f->ndim > 1 ? f->n[1] : 1);
+  if (f->layout == FL_LAYOUT_AOS) {
+    es = S;
+    cs = 1;
+  } else {
+    es = 1;
+    cs = cells;
+  }
+}
+
+int64_t cell_count(const fl_field* f) { return f->n[0] * (f->ndim > 1 ? f->n[1] : 1); }
+
+template <typename T>
+fl::SweepArgs<T> make_args(const fl_context* ctx, const fl_field* in, const fl_field* out,
+                           const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
+                           unsigned long long* ws_out, int rows) {
+  fl::SweepArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  a.in = (const T*)in->data;
+  a.out = (T*)out->data;
+  strides(in, a.es, a.cs);
+  a.nx = in->n[0];
+  a.ny = in->ndim > 1 ? in->n[1] : 1;
+  a.mat = make_material<T>(s);
+  a.mat64 = make_material<double>(s);
+  a.limiter = s->limiter;
+  a.form = s->form;
+  a.bcx = s->bc[0];
+  a.bcy = in->ndim > 1 ? s->bc[1] : 0;
+  a.rows_per_block = rows;
+  a.dtdh_host = (T)dtdh;
+  a.ctl = ctl;
+  a.ws_out = ws_out;
+  (void)ctx;
+  return a;
+}
+
+template <typename T>
+int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
+                   const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
+                   unsigned long long* ws_out, hipStream_t stream) {
+  const fl_tuning& tu = ctx->tuning;
+  if (in->ndim == 1) {
+    const int B = tu.x_block;
+    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, 1);
+    dim3 grid((unsigned)((a.nx + (B - 3)) / (B - 2)), 1, 1);
+    if (s->flux == FL_FLUX_HLLC)
+      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
+    else
+      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+  } else if (dim == 0) {
+    const int B = tu.x_block;
+    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.x_rows);
+    dim3 grid((unsigned)((a.nx + (B - 3)) / (B - 2)),
+              (unsigned)((a.ny + tu.x_rows - 1) / tu.x_rows), 1);
+    if (s->flux == FL_FLUX_HLLC)
+      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
+    else
+      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+  } else {
+    const int B = tu.y_block;
+    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.y_rows);
+    dim3 grid((unsigned)((a.nx + B - 1) / B), (unsigned)((a.ny + tu.y_rows - 1) / tu.y_rows), 1);
+    if (s->flux == FL_FLUX_HLLC)
+      hipLaunchKernelGGL((fl::sweep_y_kernel<T, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
+    else
+      hipLaunchKernelGGL((fl::sweep_y_kernel<T, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
+                 const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
+                 unsigned long long* ws_out, hipStream_t stream) {
+  if (in->dtype == FL_F64)
+    return launch_sweep_t<double>(ctx, in, out, dim, s, dtdh, ctl, ws_out, stream);
+  return launch_sweep_t<float>(ctx, in, out, dim, s, dtdh, ctl, ws_out, stream);
+}
+
+int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst,
+                  hipStream_t stream) {
+  int64_t es, cs;
+  strides(f, es, cs);
+  const int64_t n = cell_count(f);
+  const fl::Material<double> m64 = make_material<double>(s);
+  const int B = 256;
+  int64_t blocks = (n + B - 1) / B;
+  if (blocks > 4096) blocks = 4096;
+  dim3 grid((unsigned)blocks);
+  if (f->dtype == FL_F64) {
+    if (f->ndim == 1)
+      hipLaunchKernelGGL((fl::max_wavespeed_kernel<double, 1>), grid, dim3(B), 0, stream,
+                         (const double*)f->data, es, cs, n, m64, dst);
+    else
+      hipLaunchKernelGGL((fl::max_wavespeed_kernel<double, 2>), grid, dim3(B), 0, stream,
+                         (const double*)f->data, es, cs, n, m64, dst);
+  } else {
+    if (f->ndim == 1)
+      hipLaunchKernelGGL((fl::max_wavespeed_kernel<float, 1>), grid, dim3(B), 0, stream,
+                         (const float*)f->data, es, cs, n, m64, dst);
+    else
+      hipLaunchKernelGGL((fl::max_wavespeed_kernel<float, 2>), grid, dim3(B), 0, stream,
+                         (const float*)f->data, es, cs, n, m64, dst);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+double bits_to_double(unsigned long long b) {
+  double d;
+  memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fl_status_string(int status) {
+  switch (status) {
+    case FL_OK: return "ok";
+    case FL_ERR_ARGUMENT: return "invalid argument";
+    case FL_ERR_UNSUPPORTED: return "unsupported";
+    case FL_ERR_HIP: return "HIP runtime error";
+    case FL_ERR_NUMERIC: return "numeric failure";
+    default: return "unknown status";
+  }
+}
+
+const char* fl_last_error(void) { return g_last_error.c_str(); }
+
+const char* fl_version(void) { return "fluidity_amd 0.1 (gfx950)"; }
+
+int fl_ctx_create(int device, fl_context** out) {
+  if (!out) return fail(FL_ERR_ARGUMENT, "ctx out-pointer is NULL");
+  *out = nullptr;
+  int count = 0;
+  FL_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(FL_ERR_ARGUMENT, "device %d out of range (%d devices)", device, count);
+  DeviceGuard g(device);
+  fl_context* c = new fl_context();
+  memset(c, 0, sizeof(*c));
+  c->device = device;
+  c->tuning.x_block = 256;
+  c->tuning.x_rows = 32;
+  c->tuning.y_block = 256;
+  c->tuning.y_rows = 64;
+  hipError_t e = hipMalloc((void**)&c->ctl, sizeof(fl::StepCtl));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->ws_tmp, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->partial, sizeof(double) * kSumBlocks * 4);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_ctl, sizeof(fl::StepCtl), 0);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_partial, sizeof(double) * kSumBlocks * 4, 0);
+  if (e != hipSuccess) {
+    fl_ctx_destroy(c);
+    return fail(FL_ERR_HIP, "context allocation failed: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return FL_OK;
+}
+
+int fl_ctx_destroy(fl_context* c) {
+  if (!c) return FL_OK;
+  DeviceGuard g(c->device);
+  if (c->ctl) (void)hipFree(c->ctl);
+  if (c->ws_tmp) (void)hipFree(c->ws_tmp);
+  if (c->partial) (void)hipFree(c->partial);
+  if (c->host_ctl) (void)hipHostFree(c->host_ctl);
+  if (c->host_partial) (void)hipHostFree(c->host_partial);
+  delete c;
+  return FL_OK;
+}
+
+int fl_ctx_set_tuning(fl_context* c, const fl_tuning* t) {
+  if (!c || !t) return fail(FL_ERR_ARGUMENT, "NULL context or tuning");
+  fl_tuning n = c->tuning;
+  if (t->x_block) n.x_block = t->x_block;
+  if (t->x_rows) n.x_rows = t->x_rows;
+  if (t->y_block) n.y_block = t->y_block;
+  if (t->y_rows) n.y_rows = t->y_rows;
+  if (!valid_block(n.x_block) || !valid_block(n.y_block))
+    return fail(FL_ERR_ARGUMENT, "block widths must be multiples of 64 in [64, %d]", fl::kMaxBlock);
+  if (n.x_rows < 1 || n.x_rows > 4096 || n.y_rows < 1 || n.y_rows > 4096)
+    return fail(FL_ERR_ARGUMENT, "rows per workgroup must be in [1, 4096]");
+  c->tuning = n;
+  return FL_OK;
+}
+
+int fl_ctx_get_tuning(const fl_context* c, fl_tuning* t) {
+  if (!c || !t) return fail(FL_ERR_ARGUMENT, "NULL context or tuning");
+  *t = c->tuning;
+  return FL_OK;
+}
+
+int fl_split_sweep(fl_context* c, const fl_field* in, const fl_field* out, int dim,
+                   const fl_scheme* s, double dtdh, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  int rc = check_pair(in, out);
+  if (rc) return rc;
+  rc = check_scheme(s, in->ndim);
+  if (rc) return rc;
+  if (dim < 0 || dim >= in->ndim) return fail(FL_ERR_ARGUMENT, "dim %d out of range", dim);
+  if (!(dtdh > 0.0) || !isfinite(dtdh)) return fail(FL_ERR_ARGUMENT, "dtdh %g must be > 0", dtdh);
+  DeviceGuard g(c->device);
+  return launch_sweep(c, in, out, dim, s, dtdh, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int fl_split_solve(fl_context* c, fl_field* in, fl_field* out, const fl_scheme* s, double dtdh,
+                   void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  int rc = check_pair(in, out);
+  if (rc) return rc;
+  rc = check_scheme(s, in->ndim);
+  if (rc) return rc;
+  if (!(dtdh > 0.0) || !isfinite(dtdh)) return fail(FL_ERR_ARGUMENT, "dtdh %g must be > 0", dtdh);
+  DeviceGuard g(c->device);
+  for (int d = 0; d < in->ndim; ++d) {
+    rc = launch_sweep(c, in, out, d, s, dtdh, nullptr, nullptr, (hipStream_t)stream);
+    if (rc) return rc;
+    if (in->ndim != 1 && d < in->ndim - 1) {  // solver/cuda/split_solver.cuh:75-78
+      void* tmp = in->data;
+      in->data = out->data;
+      out->data = tmp;
+    }
+  }
+  return FL_OK;
+}
+
+int fl_max_wavespeed(fl_context* c, const fl_field* in, const fl_scheme* s, double* result,
+                     void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  if (!result) return fail(FL_ERR_ARGUMENT, "result is NULL");
+  int rc = check_field(in, "in");
+  if (rc) return rc;
+  rc = check_scheme(s, in->ndim);
+  if (rc) return rc;
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+  rc = launch_max_ws(in, s, c->ws_tmp, st);
+  if (rc) return rc;
+  FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, st));
+  FL_HIP(hipStreamSynchronize(st));
+  *result = bits_to_double(c->host_ctl->ws_bits[0]);
+  return FL_OK;
+}
+
+int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme* s,
+                const fl_sim_params* p, fl_sim_result* result, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  int rc = check_pair(state, work);
+  if (rc) return rc;
+  rc = check_scheme(s, state->ndim);
+  if (rc) return rc;
+  if (!p) return fail(FL_ERR_ARGUMENT, "params is NULL");
+  if (!(p->cfl > 0.0) || !isfinite(p->cfl)) return fail(FL_ERR_ARGUMENT, "cfl must be > 0");
+  if (!(p->dh > 0.0) || !isfinite(p->dh)) return fail(FL_ERR_ARGUMENT, "dh must be > 0");
+  if (p->max_iters < 0) return fail(FL_ERR_ARGUMENT, "max_iters must be >= 0");
+  if (isnan(p->sim_time)) return fail(FL_ERR_ARGUMENT, "sim_time is NaN");
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+
+  hipLaunchKernelGGL(fl::init_control_kernel, dim3(1), dim3(1), 0, st, c->ctl);
+  rc = launch_max_ws(state, s, &c->ctl->ws_bits[0], st);
+  if (rc) return rc;
+
+  // Steps are launched in batches; between batches the host reads the control
+  // block once to stop when the loop condition fails.
+  const int64_t kBatch = 256;
+  int64_t launched = 0;
+  fl_field a = *state, b = *work;
+  while (launched < p->max_iters) {
+    const int64_t nb = (p->max_iters - launched) < kBatch ? (p->max_iters - launched) : kBatch;
+    for (int64_t k = 0; k < nb; ++k) {
+      const int64_t step = launched + k;
+      const int parity = (int)(step & 1);
+      hipLaunchKernelGGL(fl::step_control_kernel, dim3(1), dim3(1), 0, st, c->ctl, parity,
+                         p->cfl, p->dh, p->sim_time, (long long)p->max_iters);
+      unsigned long long* ws_next = &c->ctl->ws_bits[parity ^ 1];
+      if (state->ndim == 1) {
+        const fl_field* src = (step & 1) ? &b : &a;
+        const fl_field* dst = (step & 1) ? &a : &b;
+        rc = launch_sweep(c, src, dst, 0, s, 1.0, c->ctl, ws_next, st);
+      } else {
+        rc = launch_sweep(c, &a, &b, 0, s, 1.0, c->ctl, nullptr, st);
+        if (!rc) rc = launch_sweep(c, &b, &a, 1, s, 1.0, c->ctl, ws_next, st);
+      }
+      if (rc) return rc;
+    }
+    launched += nb;
+    FL_HIP(hipMemcpyAsync(c->host_ctl, c->ctl, sizeof(fl::StepCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    if (!c->host_ctl->active) break;
+  }
+  const fl::StepCtl hc = *c->host_ctl;
+  if (state->ndim == 1 && (hc.iters & 1)) {
+    const int64_t bytes = cell_count(state) * 3 * (state->dtype == FL_F64 ? 8 : 4);
+    FL_HIP(hipMemcpyAsync(state->data, work->data, (size_t)bytes, hipMemcpyDeviceToDevice, st));
+    FL_HIP(hipStreamSynchronize(st));
+  }
+  const double final_ws = bits_to_double(hc.ws_bits[hc.iters & 1]);
+  if (result) {
+    result->run_time = hc.t;
+    result->iters = hc.iters;
+    result->last_dt = hc.dt;
+    result->max_wavespeed = final_ws;
+  }
+  if (!(final_ws > 0.0) || isnan(final_ws))
+    return fail(FL_ERR_NUMERIC, "max wavespeed %g after %lld steps (NaN or vacuum)", final_ws,
+                (long long)hc.iters);
+  return FL_OK;
+}
+
+int fl_conservation_sums(fl_context* c, const fl_field* in, double* sums, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  if (!sums) return fail(FL_ERR_ARGUMENT, "sums is NULL");
+  int rc = check_field(in, "in");
+  if (rc) return rc;
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  int64_t es, cs;
+  strides(in, es, cs);
+  const int64_t n = cell_count(in);
+  const int S = 2 + in->ndim;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > kSumBlocks) blocks = kSumBlocks;
+  dim3 grid((unsigned)blocks);
+  if (in->dtype == FL_F64) {
+    if (S == 3)
+      hipLaunchKernelGGL((fl::component_sums_kernel<double, 3>), grid, dim3(256), 0, st,
+                         (const double*)in->data, es, cs, n, c->partial);
+    else
+      hipLaunchKernelGGL((fl::component_sums_kernel<double, 4>), grid, dim3(256), 0, st,
+                         (const double*)in->data, es, cs, n, c->partial);
+  } else {
+    if (S == 3)
+      hipLaunchKernelGGL((fl::component_sums_kernel<float, 3>), grid, dim3(256), 0, st,
+                         (const float*)in->data, es, cs, n, c->partial);
+    else
+      hipLaunchKernelGGL((fl::component_sums_kernel<float, 4>), grid, dim3(256), 0, st,
+                         (const float*)in->data, es, cs, n, c->partial);
+  }
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(c->host_partial, c->partial, sizeof(double) * blocks * S,
+                        hipMemcpyDeviceToHost, st));
+  FL_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < S; ++k) {
+    double v = 0.0;
+    for (int64_t b = 0; b < blocks; ++b) v += c->host_partial[b * S + k];
+    sums[k] = v;
+  }
+  return FL_OK;
+}
+
+}  // extern "C"

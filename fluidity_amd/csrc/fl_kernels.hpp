@@ -1,0 +1,65 @@
+// fl_kernels.hpp -- argument blocks shared by the gfx950 kernels and the
+// host-side launch code of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fl_numerics.hpp"
+
+namespace fl {
+
+// Device-resident step control.  One instance drives a whole simulation so
+// that the time-step loop never synchronises with the host: the control
+// kernel turns the fused maximum wavespeed into dt and dt/dh exactly as
+// Parameters::update_time_delta / dt_dh do (simulator/parameters.hpp:60-99)
+// and advances run_time / iters (:67-71).
+struct StepCtl {
+  unsigned long long ws_bits[2];  // max wavespeed as bits, ping-pong by step parity
+  double dt;
+  double dtdh;
+  double t;                       // run time
+  long long iters;
+  int active;                     // run_time < sim_time && iters < max_iters
+  int pad;
+};
+
+template <typename T>
+struct SweepArgs {
+  const T* in;
+  T* out;
+  int64_t es;          // element stride (in T units): S for AoS, 1 for SoA
+  int64_t cs;          // component stride: 1 for AoS, nx*ny for SoA
+  int64_t nx, ny;      // interior cells
+  Material<T> mat;     // material in the sweep precision
+  Material<double> mat64;  // FP64 material for the fused wavespeed
+  int limiter, form;
+  int bcx, bcy;
+  int rows_per_block;  // row chunk marched by one workgroup
+  int pad;
+  T dtdh_host;         // used when ctl == nullptr
+  const StepCtl* ctl;  // nullable: provides dtdh and the active flag
+  unsigned long long* ws_out;  // nullable: fused max wavespeed (bit pattern)
+};
+
+// Maximum workgroup width of the sweep kernels (LDS arrays are sized for it).
+constexpr int kMaxBlock = 256;
+constexpr int kHalo = 3;  // extra cells each side staged in LDS for chi
+
+}  // namespace fl
+
+// Kernels defined in fl_sweep.hip (explicitly instantiated there).
+namespace fl {
+template <typename T, int ND, int FLUX> __global__ void sweep_x_kernel(SweepArgs<T> a);
+template <typename T, int FLUX> __global__ void sweep_y_kernel(SweepArgs<T> a);
+template <typename T, int ND>
+__global__ void max_wavespeed_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,
+                                     Material<double> m64, unsigned long long* dst);
+template <typename T, int S>
+__global__ void component_sums_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,
+                                      double* partial);
+__global__ void zero_u64_kernel(unsigned long long* p);
+__global__ void step_control_kernel(StepCtl* ctl, int parity, double cfl, double dh,
+                                    double sim_time, long long max_iters);
+__global__ void init_control_kernel(StepCtl* ctl);
+}  // namespace fl

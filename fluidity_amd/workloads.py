@@ -1,0 +1,261 @@
+"""Seeded synthetic initial data for the Euler-step workloads.
+
+The reference ships no input generator; these follow the workload
+definitions: Riemann data with rho log-uniform in [0.1, 10], u (and v)
+uniform in [-2, 2], p log-uniform in [0.01, 1000], drawn from a SplitMix64
+stream (u01 = (x >> 11) * 2^-53).  Draws whose neighbouring states would
+create vacuum (Toro eq. 4.40) are rejected and redrawn from the same stream.
+
+Primitive data is converted to the reference's conservative layout with the
+reference's prim->cons expressions (state/state_impl.hpp:388-406): E =
+rho * (1/2 |v|^2 + p / ((gamma-1) rho)), m = v * rho.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform01(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def uniform(self, a: float, b: float) -> float:
+        return a + self.uniform01() * (b - a)
+
+    def log_uniform(self, a: float, b: float) -> float:
+        la, lb = math.log(a), math.log(b)
+        return math.exp(la + self.uniform01() * (lb - la))
+
+
+def draw_prim(rng: SplitMix64, nd: int) -> list[float]:
+    """One primitive state [rho, p, u, (v)] in the workload ranges."""
+    rho = rng.log_uniform(0.1, 10.0)
+    vel = [rng.uniform(-2.0, 2.0) for _ in range(nd)]
+    p = rng.log_uniform(0.01, 1000.0)
+    return [rho, p, *vel]
+
+
+def sound_speed(w, gamma):
+    return math.sqrt(gamma * w[1] / w[0])
+
+
+def no_vacuum(wl, wr, gamma, dim=0) -> bool:
+    """Pressure positivity condition (Toro eq. 4.40) in the normal direction."""
+    du = wr[2 + dim] - wl[2 + dim]
+    return (2.0 / (gamma - 1.0)) * (sound_speed(wl, gamma) + sound_speed(wr, gamma)) > du
+
+
+def prim_to_cons(W: np.ndarray, gamma: float) -> np.ndarray:
+    """Vectorised prim->cons with the reference's operation order
+    (state/state_impl.hpp:135-142, 388-406); W[..., :] = [rho, p, v...]."""
+    nd = W.shape[-1] - 2
+    rho, p = W[..., 0], W[..., 1]
+    vss = np.zeros_like(rho)
+    for i in range(nd):
+        v = W[..., 2 + i]
+        vss = vss + v * v
+    eos = p / ((gamma - 1.0) * rho)
+    U = np.empty_like(W)
+    U[..., 0] = rho
+    U[..., 1] = rho * (0.5 * vss + eos)
+    for i in range(nd):
+        U[..., 2 + i] = W[..., 2 + i] * rho
+    return U
+
+
+# --------------------------------------------------------------------------
+# exact Riemann solver (Toro, Riemann Solvers and Numerical Methods, ch. 4)
+# --------------------------------------------------------------------------
+
+def _f_k(p, rho, pk, ak, gamma):
+    if p > pk:  # shock
+        A = 2.0 / ((gamma + 1.0) * rho)
+        B = (gamma - 1.0) / (gamma + 1.0) * pk
+        q = math.sqrt(A / (p + B))
+        return (p - pk) * q, q * (1.0 - 0.5 * (p - pk) / (B + p))
+    # rarefaction
+    r = p / pk
+    f = 2.0 * ak / (gamma - 1.0) * (r ** ((gamma - 1.0) / (2.0 * gamma)) - 1.0)
+    df = 1.0 / (rho * ak) * r ** (-(gamma + 1.0) / (2.0 * gamma))
+    return f, df
+
+
+def star_pressure(wl, wr, gamma, dim=0):
+    rl, pl, ul = wl[0], wl[1], wl[2 + dim]
+    rr, pr, ur = wr[0], wr[1], wr[2 + dim]
+    al, ar = sound_speed(wl, gamma), sound_speed(wr, gamma)
+    du = ur - ul
+    p = max(1e-12, 0.5 * (pl + pr) - 0.125 * du * (rl + rr) * (al + ar))
+    for _ in range(200):
+        fl, dfl = _f_k(p, rl, pl, al, gamma)
+        fr, dfr = _f_k(p, rr, pr, ar, gamma)
+        pn = p - (fl + fr + du) / (dfl + dfr)
+        pn = max(pn, 1e-12)
+        if abs(pn - p) <= 1e-14 * 0.5 * (pn + p):
+            p = pn
+            break
+        p = pn
+    fl, _ = _f_k(p, rl, pl, al, gamma)
+    fr, _ = _f_k(p, rr, pr, ar, gamma)
+    u = 0.5 * (ul + ur) + 0.5 * (fr - fl)
+    return p, u
+
+
+def max_wave_speed(wl, wr, gamma, dim=0) -> float:
+    """Largest |wave speed| of the exact solution of the Riemann problem."""
+    p, _ = star_pressure(wl, wr, gamma, dim)
+    ul, ur = wl[2 + dim], wr[2 + dim]
+    al, ar = sound_speed(wl, gamma), sound_speed(wr, gamma)
+    q = (gamma + 1.0) / (2.0 * gamma)
+    r = (gamma - 1.0) / (2.0 * gamma)
+    sl = ul - al * math.sqrt(q * p / wl[1] + r) if p > wl[1] else ul - al
+    sr = ur + ar * math.sqrt(q * p / wr[1] + r) if p > wr[1] else ur + ar
+    return max(abs(sl), abs(sr))
+
+
+def sample_exact(wl, wr, gamma, xi):
+    """Exact primitive solution [rho, p, u] at similarity coordinates xi = x/t."""
+    p_s, u_s = star_pressure(wl, wr, gamma)
+    out = np.empty((len(xi), 3))
+    rl, pl, ul = wl[0], wl[1], wl[2]
+    rr, pr, ur = wr[0], wr[1], wr[2]
+    al, ar = sound_speed(wl, gamma), sound_speed(wr, gamma)
+    g = gamma
+    for i, s in enumerate(xi):
+        if s <= u_s:  # left of contact
+            if p_s > pl:
+                sl = ul - al * math.sqrt((g + 1) / (2 * g) * p_s / pl + (g - 1) / (2 * g))
+                if s <= sl:
+                    out[i] = (rl, pl, ul)
+                else:
+                    rs = rl * ((p_s / pl + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p_s / pl + 1))
+                    out[i] = (rs, p_s, u_s)
+            else:
+                shl = ul - al
+                a_s = al * (p_s / pl) ** ((g - 1) / (2 * g))
+                stl = u_s - a_s
+                if s <= shl:
+                    out[i] = (rl, pl, ul)
+                elif s >= stl:
+                    out[i] = (rl * (p_s / pl) ** (1 / g), p_s, u_s)
+                else:
+                    c = 2 / (g + 1) + (g - 1) / ((g + 1) * al) * (ul - s)
+                    out[i] = (rl * c ** (2 / (g - 1)), pl * c ** (2 * g / (g - 1)),
+                              2 / (g + 1) * (al + (g - 1) / 2 * ul + s))
+        else:  # right of contact
+            if p_s > pr:
+                sr = ur + ar * math.sqrt((g + 1) / (2 * g) * p_s / pr + (g - 1) / (2 * g))
+                if s >= sr:
+                    out[i] = (rr, pr, ur)
+                else:
+                    rs = rr * ((p_s / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p_s / pr + 1))
+                    out[i] = (rs, p_s, u_s)
+            else:
+                shr = ur + ar
+                a_s = ar * (p_s / pr) ** ((g - 1) / (2 * g))
+                str_ = u_s + a_s
+                if s >= shr:
+                    out[i] = (rr, pr, ur)
+                elif s <= str_:
+                    out[i] = (rr * (p_s / pr) ** (1 / g), p_s, u_s)
+                else:
+                    c = 2 / (g + 1) - (g - 1) / ((g + 1) * ar) * (ur - s)
+                    out[i] = (rr * c ** (2 / (g - 1)), pr * c ** (2 * g / (g - 1)),
+                              2 / (g + 1) * (-ar + (g - 1) / 2 * ur + s))
+    return out
+
+
+# --------------------------------------------------------------------------
+# workloads
+# --------------------------------------------------------------------------
+
+@dataclass
+class Riemann1D:
+    U: np.ndarray        # (n, 3) conservative AoS, float64
+    wl: list
+    wr: list
+    gamma: float
+    dh: float
+    t_end: float         # time for the fastest wave to travel 0.4 (x = 0.5 -> 0.9)
+    rejected: int
+
+
+def riemann_1d(n: int = 4096, seed: int = 1000, gamma: float = 1.4,
+               travel: float = 0.4) -> Riemann1D:
+    """1D Riemann problem split at x = 0.5 (cell i is left if i/n < 0.5)."""
+    rng = SplitMix64(seed)
+    rejected = 0
+    while True:
+        wl = draw_prim(rng, 1)
+        wr = draw_prim(rng, 1)
+        if no_vacuum(wl, wr, gamma):
+            break
+        rejected += 1
+    x = np.arange(n) / n
+    W = np.where((x < 0.5)[:, None], np.array(wl)[None, :], np.array(wr)[None, :])
+    U = prim_to_cons(W.astype(np.float64), gamma)
+    smax = max_wave_speed(wl, wr, gamma)
+    return Riemann1D(np.ascontiguousarray(U), wl, wr, gamma, 1.0 / n, travel / smax, rejected)
+
+
+@dataclass
+class Riemann2D:
+    U: np.ndarray        # (ny, nx, 4) conservative AoS, float64
+    quadrants: list      # primitive states [Q1 (x>=x0,y>=y0), Q2, Q3, Q4] anticlockwise
+    split: tuple
+    gamma: float
+    dh: float
+    rejected: int
+
+
+def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D:
+    """Four-quadrant Riemann problem on an n x n grid of [0,1]^2 with the
+    split point uniform in [0.3, 0.7]^2."""
+    rng = SplitMix64(seed)
+    x0 = rng.uniform(0.3, 0.7)
+    y0 = rng.uniform(0.3, 0.7)
+    rejected = 0
+    while True:
+        q = [draw_prim(rng, 2) for _ in range(4)]
+        ok = (no_vacuum(q[1], q[0], gamma, 0) and no_vacuum(q[2], q[3], gamma, 0)
+              and no_vacuum(q[3], q[0], gamma, 1) and no_vacuum(q[2], q[1], gamma, 1))
+        if ok:
+            break
+        rejected += 1
+    x = np.arange(n) / n
+    right = x >= x0
+    top = x >= y0
+    W = np.empty((n, n, 4))
+    Q = [np.array(s) for s in q]
+    for j_top in (False, True):
+        for i_right in (False, True):
+            qi = {(True, True): 0, (False, True): 1, (False, False): 2, (True, False): 3}[
+                (i_right, j_top)]
+            mask = (top[:, None] == j_top) & (right[None, :] == i_right)
+            W[mask] = Q[qi]
+    U = prim_to_cons(W, gamma)
+    return Riemann2D(np.ascontiguousarray(U), q, (x0, y0), gamma, 1.0 / n, rejected)
+
+
+def aos_to_soa(U: np.ndarray) -> np.ndarray:
+    """(.., S) AoS -> (S, ..) SoA planes."""
+    return np.ascontiguousarray(np.moveaxis(U, -1, 0))
+
+
+def soa_to_aos(P: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.moveaxis(P, 0, -1))

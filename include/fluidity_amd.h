@@ -1,0 +1,150 @@
+/*
+ * fluidity_amd.h -- C ABI of the MI355X (gfx950) finite-volume Euler step.
+ *
+ * Drop-in boundary for fluidity's split-solver update call.  Each entry point
+ * names the reference interface it replaces (paths relative to the reference's
+ * include/fluidity/).  All calls:
+ *   - return an fl_status (0 = FL_OK); fl_last_error() describes a failure;
+ *   - take device pointers owned by the caller and a hipStream_t passed as
+ *     void* (NULL = the null stream) and are asynchronous on that stream
+ *     unless stated otherwise;
+ *   - allocate nothing except the context's own small control block.
+ * A context belongs to one device; calls on one context from several host
+ * threads must be serialised by the caller.
+ */
+#ifndef FLUIDITY_AMD_H
+#define FLUIDITY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fl_status {
+  FL_OK = 0,
+  FL_ERR_ARGUMENT = 1,     /* null pointer, bad size, bad enum, dtdh <= 0 ...  */
+  FL_ERR_UNSUPPORTED = 2,  /* a valid combination this build does not run      */
+  FL_ERR_HIP = 3,          /* the HIP runtime reported an error                 */
+  FL_ERR_NUMERIC = 4       /* NaN or non-positive maximum wavespeed (vacuum)    */
+} fl_status;
+
+typedef enum fl_dtype { FL_F64 = 0, FL_F32 = 1 } fl_dtype;
+
+/* AoS is the reference's State<T,conservative,D> array (state/state.hpp:78-98,
+ * x fastest: dimension/dimension_info.hpp:72-81); SoA keeps one plane per
+ * component, the native layout of the gfx950 kernels. */
+typedef enum fl_layout { FL_LAYOUT_AOS = 0, FL_LAYOUT_SOA = 1 } fl_layout;
+
+typedef enum fl_limiter { FL_LIMITER_VAN_LEER = 0, FL_LIMITER_SUPERBEE = 1 } fl_limiter;
+typedef enum fl_limit_form { FL_FORM_CONSERVATIVE = 0, FL_FORM_PRIMITIVE = 1 } fl_limit_form;
+typedef enum fl_flux { FL_FLUX_HLLC = 0, FL_FLUX_FORCE = 1 } fl_flux;
+typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1 } fl_eos;
+typedef enum fl_boundary {
+  FL_BC_TRANSMISSIVE = 0,  /* solver/boundary_loader.hpp:29-32 */
+  FL_BC_REFLECTIVE = 1,
+  FL_BC_PERIODIC = 2       /* extension: not in the reference */
+} fl_boundary;
+
+/* A grid of conservative states [rho, E, m_x, (m_y)] holding interior cells
+ * only (simulator/simulator_impl.hpp:272); ghost cells never live in memory. */
+typedef struct fl_field {
+  void*   data;      /* device pointer to component 0 of cell 0 */
+  int32_t dtype;     /* fl_dtype  */
+  int32_t layout;    /* fl_layout */
+  int32_t ndim;      /* 1 or 2    */
+  int32_t reserved;
+  int64_t n[3];      /* interior cells per dimension, x first; unused = 1 */
+} fl_field;
+
+/* The scheme a reference SplitSolver<FaceFlux<MHReconstructor<Limiter<Form>>,
+ * Flux, Eos>, BoundaryLoader<4>, Num<D>> instantiation fixes at compile time
+ * (solver/split_solver.hpp:37-126, solver/flux_solver.hpp:27-41). */
+typedef struct fl_scheme {
+  int32_t limiter;   /* fl_limiter    */
+  int32_t form;      /* fl_limit_form */
+  int32_t flux;      /* fl_flux       */
+  int32_t eos;       /* fl_eos        */
+  double  gamma;     /* adiabatic index (material/ideal_gas.hpp:118) */
+  double  p_inf;     /* stiffened-gas constant (material/stiff_fluid.hpp:132) */
+  int32_t bc[3];     /* fl_boundary per dimension, both ends (BoundarySetter) */
+  int32_t reserved;
+} fl_scheme;
+
+typedef struct fl_sim_params {
+  double  cfl;        /* Parameters::cfl (simulator/parameters.hpp) */
+  double  dh;         /* Domain::resolution (simulator/domain.hpp)   */
+  double  sim_time;   /* Parameters::sim_time                         */
+  int64_t max_iters;  /* Parameters::max_iters                        */
+} fl_sim_params;
+
+typedef struct fl_sim_result {
+  double  run_time;   /* Parameters::run_time after the loop */
+  int64_t iters;      /* Parameters::iters after the loop    */
+  double  last_dt;    /* dt of the last step                 */
+  double  max_wavespeed; /* max wavespeed of the final state */
+} fl_sim_result;
+
+/* Launch geometry; zeros select the built-in defaults. */
+typedef struct fl_tuning {
+  int32_t x_block;    /* lanes per x-sweep workgroup (64..256, multiple of 64) */
+  int32_t x_rows;     /* rows marched per x-sweep workgroup                   */
+  int32_t y_block;    /* lanes per y-sweep workgroup (64..256, multiple of 64) */
+  int32_t y_rows;     /* rows marched per y-sweep workgroup                   */
+} fl_tuning;
+
+typedef struct fl_context fl_context;
+
+const char* fl_status_string(int status);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* fl_last_error(void);
+/* Version string of this library. */
+const char* fl_version(void);
+
+/* Creates a context on HIP device `device`; replaces the state the reference
+ * keeps in SimulatorImpl (simulator/simulator_impl.hpp:43-211). */
+int fl_ctx_create(int device, fl_context** ctx);
+int fl_ctx_destroy(fl_context* ctx);
+int fl_ctx_set_tuning(fl_context* ctx, const fl_tuning* tuning);
+int fl_ctx_get_tuning(const fl_context* ctx, fl_tuning* tuning);
+
+/* One sweep in dimension `dim`: out = U + dtdh (F_{i-1/2} - F_{i+1/2}).
+ * Replaces solve_dim<S> -> SplitSolver::invoke
+ * (solver/cuda/split_solver.cuh:39-44, solver/split_solver.hpp:138-162). */
+int fl_split_sweep(fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
+                   const fl_scheme* scheme, double dtdh, void* stream);
+
+/* All sweeps of one step, x then y.  Mirrors SplitSolver::solve
+ * (solver/split_solver.hpp:105-115, solver/cuda/split_solver.cuh:57-80),
+ * including its swap of the caller's in/out between sweeps: on return
+ * out->data holds the result (for 2D, in->data and out->data are exchanged). */
+int fl_split_solve(fl_context* ctx, fl_field* in, fl_field* out, const fl_scheme* scheme,
+                   double dtdh, void* stream);
+
+/* max over cells of max_d |u_d| + a, computed in FP64 and returned on the
+ * host (synchronises `stream`).  Replaces set_wavespeeds + max_element
+ * (simulator/cuda/wavespeed_initialization.cuh:41-97,
+ * algorithm/max_element.hpp:28-38).  A NaN result is returned as NaN. */
+int fl_max_wavespeed(fl_context* ctx, const fl_field* in, const fl_scheme* scheme,
+                     double* max_wavespeed, void* stream);
+
+/* The simulator loop of SimulatorImpl::simulate
+ * (simulator/simulator_impl.hpp:222-258): while run_time < sim_time and
+ * iters < max_iters, dt = cfl*dh/max_ws, all sweeps, run_time += dt.  dt is
+ * computed on the device from the max wavespeed fused into the last sweep of
+ * the previous step; the loop runs max_iters steps' worth of launches and
+ * steps past the stop condition do nothing.  `state` holds the initial data
+ * and receives the result; `work` is a same-shaped buffer.  Synchronises
+ * `stream` once at the end.  result may be NULL. */
+int fl_simulate(fl_context* ctx, fl_field* state, fl_field* work, const fl_scheme* scheme,
+                const fl_sim_params* params, fl_sim_result* result, void* stream);
+
+/* FP64 sums of each conserved component over the grid (synchronises
+ * `stream`); sums has 2+ndim entries.  Conservation diagnostics beside the
+ * FP32 bulk path. */
+int fl_conservation_sums(fl_context* ctx, const fl_field* in, double* sums, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUIDITY_AMD_H */

@@ -1,0 +1,83 @@
+/*
+ * fl_oracle.h -- CPU restatement of fluidity's finite-volume Euler step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  No product code links it.
+ *
+ * Every function restates the reference header named beside it, keeping the
+ * reference's floating-point operation order so that an FP64 build with
+ * -ffp-contract=off reproduces the reference bit for bit.
+ *
+ * State vectors are the reference's conservative layout
+ * (state/state.hpp:88-98): [rho, E, m_0, .., m_{nd-1}, add_0, ..] and the
+ * primitive layout [rho, p, v_0, .., v_{nd-1}, add_0, ..].
+ */
+#ifndef FL_ORACLE_H
+#define FL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_LIM_VANLEER = 0, ORC_LIM_SUPERBEE = 1 };
+enum { ORC_FORM_CONS = 0, ORC_FORM_PRIM = 1 };
+enum { ORC_FLUX_HLLC = 0, ORC_FLUX_FORCE = 1, ORC_FLUX_LF = 2, ORC_FLUX_RICHTMYER = 3 };
+enum { ORC_EOS_IDEAL = 0, ORC_EOS_STIFF = 1 };
+enum { ORC_BC_TRANSMISSIVE = 0, ORC_BC_REFLECTIVE = 1, ORC_BC_PERIODIC = 2 };
+
+typedef struct orc_scheme {
+  int32_t limiter;   /* ORC_LIM_*  */
+  int32_t form;      /* ORC_FORM_* : variables the limiter ratio is taken on */
+  int32_t flux;      /* ORC_FLUX_* */
+  int32_t eos;       /* ORC_EOS_*  */
+  double  gamma;     /* adiabatic index */
+  double  p_inf;     /* stiffened-gas constant (ORC_EOS_STIFF only) */
+  int32_t bc[3];     /* ORC_BC_* per dimension, applied at both ends */
+  int32_t pad;
+} orc_scheme;
+
+/* ---- single-state numerics (state/state_impl.hpp, material/ideal_gas.hpp, stiff_fluid.hpp) ---- */
+double orc_pressure(const double* U, int nd, int na, const orc_scheme* s);
+double orc_sound_speed(const double* U, int nd, int na, const orc_scheme* s);
+double orc_max_wavespeed(const double* U, int nd, int na, const orc_scheme* s);
+void   orc_flux(const double* U, int nd, int na, const orc_scheme* s, int dim, double* F);
+void   orc_flux_prim(const double* W, int nd, int na, const orc_scheme* s, int dim, double* F);
+void   orc_cons_to_prim(const double* U, int nd, int na, const orc_scheme* s, double* W);
+void   orc_prim_to_cons(const double* W, int nd, int na, const orc_scheme* s, double* U);
+void   orc_apply_setter(double* U, int nd, int dim, int kind);
+
+/* ---- limiters (limiting/van_leer_limiter.hpp, superbee_limiter.hpp) ---- */
+double orc_limit_single(int limiter, double left, double right);
+
+/* ---- numerical fluxes (flux_method/flux_hllc.hpp, flux_force.hpp, ..) ---- */
+void orc_numerical_flux(const double* ul, const double* ur, int nd,
+                        const orc_scheme* s, double dtdh, int dim, double* F);
+
+/* ---- whole-grid operations on interior-only AoS arrays ---- */
+/* n[0] = cells in x (fastest), n[1] = cells in y (1 for 1D). */
+int    orc_sweep(const double* in, double* out, const int64_t* n, int nd,
+                 const orc_scheme* s, double dtdh, int dim, int nthreads);
+double orc_max_wavespeed_grid(const double* U, const int64_t* n, int nd,
+                              const orc_scheme* s, int nthreads);
+void   orc_flux_delta_at(const double* U, const int64_t* n, int nd,
+                         const orc_scheme* s, double dtdh, int dim,
+                         int64_t x, int64_t y, double* out);
+void   orc_face_states_at(const double* U, const int64_t* n, int nd,
+                          const orc_scheme* s, double dtdh, int dim,
+                          int64_t x, int64_t y, double* R, double* L,
+                          double* chi);
+/* Runs the simulator loop (simulator/simulator_impl.hpp:222-258).  U holds the
+ * initial state and receives the final one; work is a same-sized buffer.  When
+ * dt_hist is not NULL it receives the dt of each step (max_iters entries). */
+int    orc_simulate(double* U, double* work, const int64_t* n, int nd,
+                    const orc_scheme* s, double cfl, double dh,
+                    double sim_time, int64_t max_iters, int nthreads,
+                    double* run_time, int64_t* iters, double* dt_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

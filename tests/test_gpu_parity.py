@@ -1,0 +1,228 @@
+"""Parity of the gfx950 path (through the C ABI) with the CPU oracle.
+
+FP64 results must be bit-identical to the oracle (stricter than the 1e-12
+relative L-inf bound); FP32 results are checked against the FP64 oracle with
+the tolerance written in each test.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from tests.fields import cons_field
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def E():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    from fluidity_amd import euler
+    return euler
+
+
+@pytest.fixture(scope="module")
+def N():
+    from fluidity_amd import native
+    return native
+
+
+def _dev(U, dtype=torch.float64, layout=0):
+    if layout == 1:
+        U = np.ascontiguousarray(np.moveaxis(U, -1, 0))
+    return torch.from_numpy(np.ascontiguousarray(U)).to(device="cuda", dtype=dtype)
+
+
+def _host(T, layout=0):
+    a = T.double().cpu().numpy()
+    if layout == 1:
+        a = np.ascontiguousarray(np.moveaxis(a, 0, -1))
+    return a
+
+
+def _oscheme(O, lim, form, flux, bc, gamma=1.4, eos=0, p_inf=0.0):
+    return O.scheme(limiter=lim, form=form, flux=flux, eos=eos, gamma=gamma, p_inf=p_inf, bc=bc)
+
+
+def _escheme(E, lim, form, flux, bc, gamma=1.4, eos=0, p_inf=0.0):
+    return E.Scheme(limiter=lim, form=form, flux=flux, eos=eos, gamma=gamma, p_inf=p_inf,
+                    bc=tuple(bc))
+
+
+def _bitwise(a, b):
+    """Bit-identical, except that NaNs only need to sit at the same places
+    (x86 and gfx950 produce different NaN payloads)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+def _report(a, b):
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return f"NaN positions differ: gpu {na.sum()} NaNs, oracle {nb.sum()} NaNs"
+    d = np.where(na, 0.0, np.abs(a - b))
+    i = np.unravel_index(np.argmax(d), d.shape)
+    nbad = int((a[~na].view(np.uint64) != b[~nb].view(np.uint64)).sum())
+    return f"{nbad} values differ; max |diff| {d.max():.3e} at {i}: gpu {a[i]!r} oracle {b[i]!r}"
+
+
+CASES_1D = [
+    # kind, n, limiter, form, flux, bc, layout
+    ("rough", 1000, 0, 0, 0, (0,), 0),
+    ("rough", 1000, 1, 0, 1, (1,), 1),
+    ("fast", 777, 0, 1, 0, (2,), 0),
+    ("smooth", 300, 1, 1, 1, (0,), 1),
+    ("riemann", 4096, 0, 0, 0, (0,), 0),
+]
+
+
+@pytest.mark.parametrize("case", CASES_1D, ids=lambda c: "-".join(map(str, c[:3])))
+def test_sweep_1d_bitwise(E, oracle, case):
+    kind, n, lim, form, flux, bc, layout = case
+    U = cons_field(kind, (n,), seed=11 + n)
+    os_ = _oscheme(oracle, lim, form, flux, bc)
+    dtdh = 0.4 / oracle.max_wavespeed_grid(U, os_)
+    ref = oracle.sweep(U, os_, dtdh, 0)
+    solver = E.SplitSolver(_escheme(E, lim, form, flux, bc), layout=layout)
+    for xb in (64, 256):
+        solver.set_tuning(x_block=xb)
+        a = _dev(U, layout=layout)
+        b = torch.empty_like(a)
+        solver.sweep(a, b, 0, dtdh)
+        torch.cuda.synchronize()
+        got = _host(b, layout)
+        assert _bitwise(got, ref), _report(got, ref)
+
+
+CASES_2D = [
+    # kind, (ny, nx), limiter, form, flux, bc, layout
+    ("rough", (45, 70), 0, 0, 0, (0, 0), 0),
+    ("rough", (37, 130), 1, 1, 1, (1, 0), 1),
+    ("fast", (64, 64), 0, 0, 0, (0, 1), 1),
+    ("smooth", (33, 90), 0, 1, 1, (2, 2), 0),
+    ("rough", (20, 300), 0, 0, 0, (1, 1), 0),
+]
+
+TUNINGS = [(256, 32, 256, 64), (64, 1, 64, 1), (128, 7, 192, 5)]
+
+
+@pytest.mark.parametrize("case", CASES_2D, ids=lambda c: "-".join(map(str, c[:3])))
+@pytest.mark.parametrize("dim", [0, 1])
+def test_sweep_2d_bitwise(E, oracle, case, dim):
+    kind, shape, lim, form, flux, bc, layout = case
+    U = cons_field(kind, shape, seed=7 + shape[0] * shape[1])
+    os_ = _oscheme(oracle, lim, form, flux, bc)
+    dtdh = 0.4 / oracle.max_wavespeed_grid(U, os_)
+    ref = oracle.sweep(U, os_, dtdh, dim)
+    solver = E.SplitSolver(_escheme(E, lim, form, flux, bc), layout=layout)
+    for xb, xr, yb, yr in TUNINGS:
+        solver.set_tuning(x_block=xb, x_rows=xr, y_block=yb, y_rows=yr)
+        a = _dev(U, layout=layout)
+        b = torch.empty_like(a)
+        solver.sweep(a, b, dim, dtdh)
+        torch.cuda.synchronize()
+        got = _host(b, layout)
+        assert _bitwise(got, ref), f"tuning {(xb, xr, yb, yr)}: " + _report(got, ref)
+
+
+def test_max_wavespeed_bitwise(E, oracle):
+    for shape in [(1000,), (45, 70)]:
+        U = cons_field("rough", shape, seed=3)
+        os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
+        ref = oracle.max_wavespeed_grid(U, os_)
+        solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)))
+        got = solver.max_wavespeed(_dev(U))
+        assert got == ref
+
+
+def test_simulate_2d_bitwise(E, oracle):
+    """The device-resident step loop (fused CFL, device dt) against the
+    oracle's simulator loop: final state, run time and step count."""
+    U = cons_field("riemann", (96, 96), seed=1001)
+    for flux in (0, 1):
+        os_ = _oscheme(oracle, 0, 0, flux, (0, 0))
+        ref, rt, it, _ = oracle.simulate(U, os_, 0.9, 1.0 / 96, max_iters=40)
+        solver = E.SplitSolver(_escheme(E, 0, 0, flux, (0, 0)), layout=1)
+        a = _dev(U, layout=1)
+        w = torch.empty_like(a)
+        r = solver.simulate(a, w, 0.9, 1.0 / 96, max_iters=40)
+        got = _host(a, 1)
+        assert r.iters == it
+        assert r.run_time == rt
+        assert _bitwise(got, ref), _report(got, ref)
+
+
+def test_simulate_1d_config0_bitwise(E, oracle):
+    """The 1D Riemann workload (4096 cells, run until the fastest wave has
+    travelled 0.4) reproduces the oracle bit for bit, including the step
+    count and the accumulated run time (no final-dt clip)."""
+    from fluidity_amd import workloads as W
+    r1 = W.riemann_1d()
+    os_ = _oscheme(oracle, 0, 0, 0, (0,))
+    ref, rt, it, _ = oracle.simulate(r1.U, os_, 0.9, r1.dh, sim_time=r1.t_end, max_iters=100000)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0,)))
+    a = _dev(r1.U)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.9, r1.dh, sim_time=r1.t_end)
+    got = _host(a)
+    assert r.iters == it and r.run_time == rt
+    assert _bitwise(got, ref), _report(got, ref)
+
+
+def test_solve_swap_semantics(E, oracle):
+    """fl_split_solve leaves the result in `out` after the reference's swap."""
+    U = cons_field("rough", (40, 50), seed=5)
+    os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
+    dtdh = 0.3 / oracle.max_wavespeed_grid(U, os_)
+    ref = oracle.sweep(oracle.sweep(U, os_, dtdh, 0), os_, dtdh, 1)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)))
+    a = _dev(U)
+    b = torch.empty_like(a)
+    new_in, new_out = solver.solve(a, b, dtdh)
+    torch.cuda.synchronize()
+    assert new_out.data_ptr() == a.data_ptr()
+    assert _bitwise(_host(new_out), ref)
+
+
+def test_fp32_against_fp64_oracle(E, oracle):
+    """FP32 bulk sweeps, FP64 dt: relative L1 density error <= 1e-5 after
+    100 steps on a 2D Riemann workload."""
+    U = cons_field("riemann", (128, 128), seed=1001)
+    os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
+    ref, rt, it, _ = oracle.simulate(U, os_, 0.9, 1.0 / 128, max_iters=100)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+    a = _dev(U, dtype=torch.float32, layout=1)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.9, 1.0 / 128, max_iters=100)
+    got = _host(a, 1)
+    assert r.iters == it
+    rel_l1 = np.abs(got[..., 0] - ref[..., 0]).sum() / np.abs(ref[..., 0]).sum()
+    print(f"fp32 rel L1 density {rel_l1:.3e}", file=sys.stderr)
+    assert rel_l1 <= 1e-5
+
+
+def test_vacuum_reports_numeric_error(E, N):
+    """Colliding-away states that drive the pressure negative make the fused
+    wavespeed NaN; fl_simulate reports FL_ERR_NUMERIC instead of silently
+    continuing."""
+    n = 64
+    Wp = np.zeros((n, 3))
+    Wp[:, 0] = 1.0
+    Wp[:, 1] = 1e-6
+    Wp[: n // 2, 2] = -50.0
+    Wp[n // 2:, 2] = 50.0
+    from fluidity_amd import workloads as W
+    U = W.prim_to_cons(Wp, 1.4)
+    solver = E.SplitSolver(E.Scheme())
+    a = _dev(U)
+    w = torch.empty_like(a)
+    with pytest.raises(N.FluidityError) as ei:
+        solver.simulate(a, w, 0.9, 1.0 / n, max_iters=50)
+    assert ei.value.status == N.FL_ERR_NUMERIC
