@@ -24,12 +24,16 @@ def prim_field(kind: str, shape, seed: int, gamma: float = 1.4) -> np.ndarray:
         for d in range(nd):
             Wp[..., 2 + d] = rng.uniform(-0.5, 0.5, shape)
     elif kind == "fast":
-        # supersonic mean flow (+x in one half, -x in the other) with mild
-        # perturbations: HLLC's upwind branches without vacuum
+        # supersonic flow towards the middle (+3 in one half, -3 in the other)
+        # with mild perturbations: HLLC's upwind branches without vacuum
+        # (use with transmissive boundaries; periodic ones separate the flow)
         Wp[..., 0] = np.exp(rng.uniform(np.log(0.8), np.log(1.25), shape))
         Wp[..., 1] = np.exp(rng.uniform(np.log(0.8), np.log(1.25), shape))
-        sign = np.where(np.indices(shape)[0] < shape[0] // 2, 1.0, -1.0)
+        # velocity component d converges along its own axis (no vacuum)
+        idx = np.indices(shape)
         for d in range(nd):
+            ax = nd - 1 - d  # x is the last array axis
+            sign = np.where(idx[ax] < shape[ax] // 2, 1.0, -1.0)
             Wp[..., 2 + d] = 3.0 * sign + rng.uniform(-0.1, 0.1, shape)
     elif kind == "smooth":
         grids = np.meshgrid(*[np.linspace(0, 1, s, endpoint=False) for s in shape],

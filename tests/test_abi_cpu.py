@@ -1,0 +1,123 @@
+"""The C ABI library on the CPU: it loads, exports every entry point the
+header declares, and rejects bad arguments before touching a device."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fluidity_amd.h")
+LIB = os.path.join(ROOT, "fluidity_amd", "libfluidity_amd.so")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fl_[a-z_0-9]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def N():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} is not built (run __graft_entry__.build())")
+    from fluidity_amd import native
+    return native
+
+
+def test_library_exports_every_declared_symbol(N):
+    names = declared_functions()
+    assert len(names) >= 10
+    out = subprocess.check_output(["nm", "-D", "--defined-only", LIB], text=True)
+    exported = set(re.findall(r"\bT (fl_[a-z_0-9]+)\b", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    for n in names:
+        assert hasattr(N.lib, n)
+    assert set(N.EXPORTED) == set(names)
+
+
+def test_status_strings(N):
+    assert N.lib.fl_status_string(0) == b"ok"
+    assert N.lib.fl_status_string(N.FL_ERR_ARGUMENT) == b"invalid argument"
+    assert N.lib.fl_version().startswith(b"fluidity_amd")
+
+
+def test_null_context_rejected(N):
+    f = N.field(0x1000, 2, (8, 8))
+    g = N.field(0x2000, 2, (8, 8))
+    s = N.scheme()
+    rc = N.lib.fl_split_sweep(None, C.byref(f), C.byref(g), 0, C.byref(s), 0.1, None)
+    assert rc == N.FL_ERR_ARGUMENT
+    assert b"context" in N.lib.fl_last_error()
+
+
+def test_ctx_create_null_out(N):
+    assert N.lib.fl_ctx_create(0, None) == N.FL_ERR_ARGUMENT
+
+
+def _fake_ctx(N):
+    """A context-shaped handle is never dereferenced before argument checks;
+    only fl_ctx_create touches the HIP runtime."""
+    return C.c_void_p(0xdead0000)
+
+
+@pytest.mark.parametrize("mutate,needle", [
+    (lambda f, g, s: setattr(f, "data", None), b"in->data"),
+    (lambda f, g, s: f.n.__setitem__(0, 3), b"at least 4"),
+    (lambda f, g, s: setattr(f, "dtype", 7), b"dtype"),
+    (lambda f, g, s: setattr(f, "layout", 9), b"layout"),
+    (lambda f, g, s: setattr(g, "data", f.data), b"alias"),
+    (lambda f, g, s: setattr(s, "gamma", 1.0), b"gamma"),
+    (lambda f, g, s: setattr(s, "limiter", 5), b"limiter"),
+    (lambda f, g, s: setattr(s, "flux", 5), b"flux"),
+    (lambda f, g, s: s.bc.__setitem__(1, 7), b"bc[1]"),
+    (lambda f, g, s: setattr(f, "ndim", 3), b"3D"),
+])
+def test_sweep_argument_errors(N, mutate, needle):
+    f = N.field(0x1000, 2, (16, 16))
+    g = N.field(0x2000, 2, (16, 16))
+    s = N.scheme()
+    mutate(f, g, s)
+    rc = N.lib.fl_split_sweep(_fake_ctx(N), C.byref(f), C.byref(g), 0, C.byref(s), 0.1, None)
+    assert rc in (N.FL_ERR_ARGUMENT, N.FL_ERR_UNSUPPORTED)
+    assert needle in N.lib.fl_last_error()
+
+
+@pytest.mark.parametrize("dtdh", [0.0, -1.0, float("nan"), float("inf")])
+def test_sweep_rejects_bad_dtdh(N, dtdh):
+    f = N.field(0x1000, 2, (16, 16))
+    g = N.field(0x2000, 2, (16, 16))
+    s = N.scheme()
+    rc = N.lib.fl_split_sweep(_fake_ctx(N), C.byref(f), C.byref(g), 0, C.byref(s), dtdh, None)
+    assert rc == N.FL_ERR_ARGUMENT
+
+
+def test_sweep_rejects_bad_dim(N):
+    f = N.field(0x1000, 1, (16,))
+    g = N.field(0x2000, 1, (16,))
+    s = N.scheme()
+    rc = N.lib.fl_split_sweep(_fake_ctx(N), C.byref(f), C.byref(g), 1, C.byref(s), 0.1, None)
+    assert rc == N.FL_ERR_ARGUMENT
+
+
+def test_simulate_rejects_bad_params(N):
+    f = N.field(0x1000, 2, (16, 16))
+    g = N.field(0x2000, 2, (16, 16))
+    s = N.scheme()
+    for p in [N.SimParams(0.0, 0.1, 1.0, 10), N.SimParams(0.9, -1.0, 1.0, 10),
+              N.SimParams(0.9, 0.1, float("nan"), 10), N.SimParams(0.9, 0.1, 1.0, -1)]:
+        r = N.SimResult()
+        rc = N.lib.fl_simulate(_fake_ctx(N), C.byref(f), C.byref(g), C.byref(s), C.byref(p),
+                               C.byref(r), None)
+        assert rc == N.FL_ERR_ARGUMENT
+    rc = N.lib.fl_simulate(_fake_ctx(N), C.byref(f), C.byref(g), C.byref(s), None, None, None)
+    assert rc == N.FL_ERR_ARGUMENT
+
+
+def test_tuning_validation(N):
+    t = N.Tuning(100, 0, 0, 0)
+    assert N.lib.fl_ctx_set_tuning(None, C.byref(t)) == N.FL_ERR_ARGUMENT

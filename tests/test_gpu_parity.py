@@ -77,7 +77,8 @@ CASES_1D = [
     # kind, n, limiter, form, flux, bc, layout
     ("rough", 1000, 0, 0, 0, (0,), 0),
     ("rough", 1000, 1, 0, 1, (1,), 1),
-    ("fast", 777, 0, 1, 0, (2,), 0),
+    ("fast", 777, 0, 1, 0, (0,), 0),
+    ("rough", 640, 1, 1, 0, (2,), 1),
     ("smooth", 300, 1, 1, 1, (0,), 1),
     ("riemann", 4096, 0, 0, 0, (0,), 0),
 ]
@@ -106,6 +107,7 @@ CASES_2D = [
     ("rough", (45, 70), 0, 0, 0, (0, 0), 0),
     ("rough", (37, 130), 1, 1, 1, (1, 0), 1),
     ("fast", (64, 64), 0, 0, 0, (0, 1), 1),
+    ("fast", (50, 64), 1, 0, 0, (0, 0), 0),
     ("smooth", (33, 90), 0, 1, 1, (2, 2), 0),
     ("rough", (20, 300), 0, 0, 0, (1, 1), 0),
 ]
