@@ -25,6 +25,11 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E bandwidth
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch / algorithmic bytes per launch of the 4096^2 FP64 HLLC
+# sweeps, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs calibrated against a
+# plain 512 MiB device copy (profiles/sweeps_4096_f64_hllc.md).
+MEASURED_TRAFFIC_RATIO = {"x": 1.178, "y": 1.128}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -34,11 +39,18 @@ def parse():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip FORCE / FP32 lines")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--x-block", type=int, default=0)
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
     ap.add_argument("--y-rows", type=int, default=0)
+    ap.add_argument("--mode", choices=["bench", "sweeps"], default="bench",
+                    help="sweeps: a plain device copy of one grid (HBM calibration for "
+                         "counter runs) followed by --steps x sweeps and y sweeps")
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
+                    help="precision of --mode sweeps")
+    ap.add_argument("--flux", choices=["hllc", "force"], default="hllc",
+                    help="flux of --mode sweeps")
     return ap.parse_args()
 
 
@@ -57,6 +69,8 @@ def main():
 
     n = args.n
     wl = W.riemann_2d(n=n, seed=1001)
+    if args.mode == "sweeps":
+        return sweeps_mode(args, wl, torch, E, N, W)
     dh = wl.dh
     cfl = 0.9
     cells = n * n
@@ -108,9 +122,11 @@ def main():
     t_x = ev[0].elapsed_time(ev[1]) / reps * 1e-3
     t_y = ev[1].elapsed_time(ev[2]) / reps * 1e-3
     bytes_per_sweep = cells * 64.0            # read 32 B + write 32 B per cell (FP64, 2D)
-    dom_name, dom_t = ("sweep_x_kernel<double,2,HLLC>", t_x) if t_x >= t_y else \
-        ("sweep_y_kernel<double,HLLC>", t_y)
+    dom_name, dom_t, dom_k = ("sweep_x_kernel<double,2,HLLC>", t_x, "x") if t_x >= t_y else \
+        ("sweep_y_kernel<double,HLLC>", t_y, "y")
     achieved = bytes_per_sweep / dom_t / 1e9
+    traffic = (round(MEASURED_TRAFFIC_RATIO[dom_k] * bytes_per_sweep)
+               if n == 4096 else None)
     roofline = {
         "bound": "hbm",
         "kernel": dom_name,
@@ -118,7 +134,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes": int(bytes_per_sweep),
         "x_sweep_us": round(t_x * 1e6, 1),
         "y_sweep_us": round(t_y * 1e6, 1),
         "step_bytes_per_cell": 128,
@@ -166,6 +183,40 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def sweeps_mode(args, wl, torch, E, N, W):
+    """Profiling driver: one device copy of a grid-sized buffer (the plain copy
+    kernel whose counters calibrate FETCH_SIZE / WRITE_SIZE), then --steps x
+    sweeps and --steps y sweeps of the 4096^2 workload."""
+    n = args.n
+    dt = torch.float64 if args.dtype == "f64" else torch.float32
+    flux = N.FL_FLUX_HLLC if args.flux == "hllc" else N.FL_FLUX_FORCE
+    solver = E.SplitSolver(E.Scheme(flux=flux), layout=N.FL_LAYOUT_SOA)
+    solver.set_tuning(x_block=args.x_block, x_rows=args.x_rows, y_block=args.y_block,
+                      y_rows=args.y_rows)
+    a = torch.from_numpy(W.aos_to_soa(wl.U)).to(device="cuda:0", dtype=dt)
+    w = torch.empty_like(a)
+    dtdh = 0.9 / solver.max_wavespeed(a)
+    w.copy_(a)                        # calibration copy: reads and writes one grid
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    for _ in range(args.steps):
+        solver.sweep(a, w, 0, dtdh)
+    ev[1].record()
+    for _ in range(args.steps):
+        solver.sweep(w, a, 1, dtdh)
+    ev[2].record()
+    torch.cuda.synchronize()
+    tx = ev[0].elapsed_time(ev[1]) / args.steps
+    ty = ev[1].elapsed_time(ev[2]) / args.steps
+    grid_bytes = a.numel() * a.element_size()
+    print(json.dumps({"mode": "sweeps", "dtype": args.dtype, "flux": args.flux, "n": n,
+                      "grid_bytes": grid_bytes, "x_sweep_ms": round(tx, 4),
+                      "y_sweep_ms": round(ty, 4),
+                      "x_GBps": round(2 * grid_bytes / tx / 1e6, 1),
+                      "y_GBps": round(2 * grid_bytes / ty / 1e6, 1)}), flush=True)
 
 
 def cpu_baseline(wl, cfl, steps):

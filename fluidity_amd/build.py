@@ -32,25 +32,52 @@ HIP_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, out: str | None = None, defines=()) -> str:
+    """Compiles the library to `out` (default: in-tree libfluidity_amd.so);
+    `defines` are extra -D macros (e.g. FL_SWEEP_WAVES=4) for tuning builds."""
+    lib = out or LIB
+    tag = f"{os.getpid()}"
     objs = []
     for src in SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(CSRC, src.replace(".hip", f".{tag}.o"))
+        cmd = [HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c",
+               os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     for o in objs:
         os.remove(o)
-    return LIB
+    return lib
+
+
+def build_shim_driver(verbose: bool = False) -> str:
+    """Compiles tests/cpp/shim_driver (a host program using the reference-named
+    C++ template API) against the in-tree library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "shim_driver.cpp")
+    out = os.path.join(root, "tests", "cpp", "shim_driver")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", src, "-o", out,
+           f"-L{HERE}", "-lfluidity_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../fluidity_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    args = [a for a in sys.argv[1:] if a != "-v"]
+    out = None
+    defs = []
+    for a in args:
+        if a.startswith("--out="):
+            out = a[len("--out="):]
+        elif a.startswith("-D"):
+            defs.append(a[2:])
+    print(build(verbose="-v" in sys.argv, out=out, defines=defs))

@@ -38,6 +38,15 @@ template <typename T> __device__ __forceinline__ int signum(T x) {
   return (T(0) < x) - (x < T(0));
 }
 
+// Division and square root.  FP64: IEEE correctly rounded (the hardware
+// div_scale/div_fmas/div_fixup sequence and the Newton-refined rsq), which is
+// what bit parity with the reference needs.  FP32 (bulk-precision extension):
+// v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far inside the FP32 error budget.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Material parameters; derived constants are computed once on the host with
 // the same expressions the reference evaluates per call.
 template <typename T> struct Material {
@@ -63,7 +72,7 @@ __device__ __forceinline__ Prim<T, ND> prim_of(const T* U, const Material<T>& m)
   T sum = T(0);
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
-    w.vel[i] = U[2 + i] / U[0];
+    w.vel[i] = dv(U[2 + i], U[0]);
     sum += w.vel[i] * w.vel[i];
   }
   w.p = m.gm1 * (U[1] - T(0.5) * U[0] * sum);
@@ -74,8 +83,8 @@ __device__ __forceinline__ Prim<T, ND> prim_of(const T* U, const Material<T>& m)
 // StiffFluid::sound_speed (material/stiff_fluid.hpp:95-98).
 template <typename T>
 __device__ __forceinline__ T sound_speed(T p, T rho, const Material<T>& m) {
-  if (m.stiff) return sqrt(m.gamma * (p + m.p_inf) / rho);
-  return sqrt(m.gamma * p / rho);
+  if (m.stiff) return sq(dv(m.gamma * (p + m.p_inf), rho));
+  return sq(dv(m.gamma * p, rho));
 }
 
 // Conservative Euler flux in dimension D (state/state_impl.hpp:293-310 with
@@ -105,15 +114,15 @@ __device__ __forceinline__ T max_wavespeed(const T* U, const Prim<T, ND>& w,
 // Superbee::limit_single (limiting/superbee_limiter.hpp:72-89).
 template <typename T>
 __device__ __forceinline__ T limit_single(int limiter, T left, T right) {
-  const T r = left / right;
+  const T r = dv(left, right);
   if (limiter == kLimSuperbee) {
     if (r <= T(0) || right == T(0)) return T(0);
     if (r <= T(0.5)) return T(2) * r;
     if (r <= T(1)) return T(1);
-    const T Er = T(2) / (T(1) + r);
+    const T Er = dv(T(2), T(1) + r);
     return smin(Er, smin(r, T(2)));
   }
-  return (r <= T(0) || right == T(0)) ? T(0) : T(2) * smin(r, T(1)) / (T(1) + r);
+  return (r <= T(0) || right == T(0)) ? T(0) : dv(T(2) * smin(r, T(1)), T(1) + r);
 }
 
 // MHReconstructor::chi's flattening factor about a cell in one dimension
@@ -124,10 +133,10 @@ __device__ __forceinline__ T chi_factor(T p_pprev, T p_prev, T p_next, T p_nnext
                                         T v_prev, T v_next) {
   const T u_factor = v_prev - v_next;
   const T dp = fabs(p_next - p_prev);
-  const bool flatten = u_factor > T(0) && (dp / smin(p_next, p_prev)) > T(0.33);
+  const bool flatten = u_factor > T(0) && dv(dp, smin(p_next, p_prev)) > T(0.33);
   if (!flatten) return T(1);
-  const T z = dp / fabs(p_nnext - p_pprev);
-  const T temp = T(1) - ((z - T(0.75)) / (T(0.85) - T(0.75)));
+  const T z = dv(dp, T(fabs(p_nnext - p_pprev)));
+  const T temp = T(1) - dv(z - T(0.75), T(0.85) - T(0.75));
   return smin(T(1), smax(T(0), temp));
 }
 
@@ -188,7 +197,7 @@ __device__ __forceinline__ void evolve_faces(const T* um, const T* u0, const T* 
 template <typename T>
 __device__ __forceinline__ T hllc_wavespeed(T u, T p, T pstar, T sound, T q) {
   if (pstar <= p) return u + sound;
-  return u + sound * sqrt(T(1) + q * (pstar / p - T(1)));
+  return u + sound * sq(T(1) + q * (dv(pstar, p) - T(1)));
 }
 
 // Hllc::Impl::operator() (flux_method/flux_hllc.hpp:202-245) with
@@ -217,7 +226,7 @@ __device__ __forceinline__ void hllc_flux(const T* ul, const T* ur, const Materi
   }
   const T factorl = ul[0] * (wsl - vl);
   const T factorr = ur[0] * (wsr - vr);
-  const T ws_star = (wr.p - wl.p + vl * factorl - vr * factorr) / (factorl - factorr);
+  const T ws_star = dv(wr.p - wl.p + vl * factorl - vr * factorr, factorl - factorr);
 
   // One side of the star region; the reference's two branches are the same
   // expressions on (ul, wsl) or (ur, wsr).
@@ -228,13 +237,13 @@ __device__ __forceinline__ void hllc_flux(const T* ul, const T* ur, const Materi
   T f[S];
   euler_flux<T, ND, D>(st, w, f);
   const T state_factor = ws - w.vel[D];
-  const T scale_factor = state_factor / (ws - ws_star);
+  const T scale_factor = dv(state_factor, ws - ws_star);
   T us[S];
 #pragma unroll
   for (int k = 0; k < S; ++k) us[k] = st[k] * scale_factor;
   us[2 + D] = ws_star * st[0] * scale_factor;
   us[1] = scale_factor *
-          (st[1] + st[0] * (ws_star - w.vel[D]) * (ws_star + w.p / (st[0] * state_factor)));
+          (st[1] + st[0] * (ws_star - w.vel[D]) * (ws_star + dv(w.p, st[0] * state_factor)));
 #pragma unroll
   for (int k = 0; k < S; ++k) F[k] = f[k] + (us[k] - st[k]) * ws;
 }
@@ -278,7 +287,7 @@ __device__ __forceinline__ void numerical_flux(const T* ul, const T* ur, T dtdh,
 // m_d <- (-(m_d / rho)) * rho.
 template <typename T, int ND>
 __device__ __forceinline__ void reflect(T* U, int d) {
-  T v = -(U[2 + d] / U[0]);
+  T v = -dv(U[2 + d], U[0]);
   v *= U[0];
   U[2 + d] = v;
 }

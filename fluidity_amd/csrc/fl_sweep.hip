@@ -23,6 +23,13 @@
 // so a NaN anywhere surfaces as a NaN max).
 #include "fl_kernels.hpp"
 
+// Waves per SIMD the 2D sweep kernels are compiled for (register budget
+// 512 / FL_SWEEP_WAVES VGPRs).
+#ifndef FL_SWEEP_WAVES
+#define FL_SWEEP_WAVES 2
+#endif
+
+
 namespace fl {
 
 // ---------------------------------------------------------------------------
@@ -58,16 +65,32 @@ __device__ __forceinline__ int64_t map_ghost(int64_t i, int64_t n, int kind, boo
   return i < 0 ? -i - 1 : 2 * n - 1 - i;
 }
 
+// A cell load in flight.  The reflective-ghost velocity flip is applied when
+// the cell is taken, not when the load is issued, so that no instruction
+// touches the destination registers while the load is pending.
+template <typename T, int S>
+struct Pending {
+  T u[S];
+  int flip;  // bit 0: reflect in x, bit 1: reflect in y
+};
+
 template <typename T, int ND>
-__device__ __forceinline__ void load_ghosted(const SweepArgs<T>& a, int64_t x, int64_t y,
-                                             T* u) {
+__device__ __forceinline__ void issue_load(const SweepArgs<T>& a, int64_t x, int64_t y,
+                                           Pending<T, 2 + ND>& q) {
   bool fx = false, fy = false;
   const int64_t xi = map_ghost(x, a.nx, a.bcx, fx);
   int64_t yi = 0;
   if (ND > 1) yi = map_ghost(y, a.ny, a.bcy, fy);
-  load_cell<T, 2 + ND>(a.in, yi * a.nx + xi, a.es, a.cs, u);
-  if (fx) reflect<T, ND>(u, 0);
-  if (ND > 1 && fy) reflect<T, ND>(u, 1);
+  load_cell<T, 2 + ND>(a.in, yi * a.nx + xi, a.es, a.cs, q.u);
+  q.flip = (fx ? 1 : 0) | (fy ? 2 : 0);
+}
+
+template <typename T, int ND>
+__device__ __forceinline__ void take(const Pending<T, 2 + ND>& q, T* u) {
+#pragma unroll
+  for (int k = 0; k < 2 + ND; ++k) u[k] = q.u[k];
+  if (q.flip & 1) reflect<T, ND>(u, 0);
+  if (ND > 1 && (q.flip & 2)) reflect<T, ND>(u, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -130,7 +153,8 @@ __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
 // ---------------------------------------------------------------------------
 
 template <typename T, int ND, int FLUX>
-__global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
+sweep_x_kernel(SweepArgs<T> a) {
   constexpr int S = 2 + ND;
   constexpr int W = kMaxBlock + 2 * kHalo;
   __shared__ T sU[S][W];
@@ -155,26 +179,53 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
 
   // Halo lanes: t < 6 stage columns x0-4..x0-2 (slots 0..2) and
   // x0+B-1..x0+B+1 (slots B+3..B+5) of the output row.
-  const int hslot = (t < 3) ? t : B + t;
+  // Every lane loads one of the six halo cells (lane t takes cell t mod 6), so
+  // the halo loads are unconditional; only lanes t < 6 stage them in LDS.
+  const int hl = t % (2 * kHalo);
+  const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - 1 - kHalo + hslot;
 
   // Register windows along y (2D): rows r-3..r of U, u_x; rows r-4..r of p;
   // rows r-3..r of u_y; K_y of rows r-4..r-2.
   T Uw[4][S];
-  T pw[5], vxw[4], vyw[4], kyw[3];
+  T pw[5], vxw[4], vyw[4];
+  // K_y window as named scalars: an array indexed by the sign of the pressure
+  // difference would be demoted to scratch memory.
+  T ky0 = T(1), ky1 = T(1), ky2 = T(1);
 #pragma unroll
   for (int i = 0; i < 5; ++i) pw[i] = T(0);
 #pragma unroll
   for (int i = 0; i < 4; ++i) { vxw[i] = T(0); vyw[i] = T(0); }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) kyw[i] = T(1);
 
   unsigned long long wsmax = 0ull;
   const int64_t rbeg = (ND == 2) ? y0 - 3 : 0;
   const int64_t rlast = (ND == 2) ? yend + 2 : 0;
 
-  for (int64_t r = rbeg; r <= rlast; ++r) {
-    // ---- load the newest row of this lane's column and its primitives
+  // Software pipelining with two row buffers in fixed registers (A for even,
+  // B for odd steps of the unrolled loop): the loads of row r+2 and of the
+  // halo cells of the output row two steps ahead are issued while row r is
+  // computed, without ever copying a register whose load is still in flight.
+  // All loads are unconditional (rows past the end are re-loaded, clamped), so
+  // the compiler can count the loads issued after each one and wait only for
+  // the one it needs.
+  auto hrow = [&](int64_t r) { return (ND == 2) ? r - 3 : int64_t(0); };
+  Pending<T, S> QA, QB, HA, HB;
+  issue_load<T, ND>(a, c, rbeg, QA);
+  issue_load<T, ND>(a, c, min(rbeg + 1, rlast), QB);
+  issue_load<T, ND>(a, hcol, hrow(rbeg), HA);
+  issue_load<T, ND>(a, hcol, hrow(min(rbeg + 1, rlast)), HB);
+
+  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
+  // two-step loop is fully unrolled so that each copy names its buffers
+  // statically.
+  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
+#pragma unroll
+  for (int step = 0; step < 2; ++step) {
+    const int64_t r = r0 + step;
+    if (r > rlast) break;
+    Pending<T, S>& Q = step ? QB : QA;
+    Pending<T, S>& H = step ? HB : HA;
+    // ---- the newest row of this lane's column and its primitives
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -184,7 +235,11 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
-    load_ghosted<T, ND>(a, c, r, Uw[3]);
+    take<T, ND>(Q, Uw[3]);
+    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
+    T uh[S];  // halo cell of output row hrow(r)
+    take<T, ND>(H, uh);
+    issue_load<T, ND>(a, hcol, hrow(min(r + 2, rlast)), H);
     {
       const Prim<T, ND> w = prim_of<T, ND>(Uw[3], a.mat);
       pw[4] = w.p;
@@ -193,9 +248,9 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
     }
     if (ND == 2) {
       if (r >= y0 + 1) {  // K_y of row r-2 (needs rows r-4..r)
-        kyw[0] = kyw[1];
-        kyw[1] = kyw[2];
-        kyw[2] = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
+        ky0 = ky1;
+        ky1 = ky2;
+        ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
       }
       if (r < y0 + 3) continue;  // block-uniform: warm-up rows
     }
@@ -210,8 +265,6 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
     sP[s] = p0;
     sV[s] = vx0;
     if (t < 2 * kHalo) {
-      T uh[S];
-      load_ghosted<T, ND>(a, hcol, jo, uh);
       const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
 #pragma unroll
       for (int k = 0; k < S; ++k) sU[k][hslot] = uh[k];
@@ -240,7 +293,8 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
     T chi = smin(Limits<T>::max(), smin(sK[s], sK[s + sx]));
     if (ND == 2) {
       const int sy = signum(pw[2] - pw[0]);
-      chi = smin(chi, smin(kyw[1], kyw[1 + sy]));
+      const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
+      chi = smin(chi, smin(ky1, ky_nb));
     }
     T R[S], L[S];
     evolve_faces<T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
@@ -272,6 +326,7 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
       }
     }
   }
+  }
   if (a.ws_out) block_atomic_max(wsmax, a.ws_out);
 }
 
@@ -280,7 +335,8 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_x_kernel(SweepArgs<T> a) {
 // ---------------------------------------------------------------------------
 
 template <typename T, int FLUX>
-__global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
+sweep_y_kernel(SweepArgs<T> a) {
   constexpr int ND = 2, S = 4;
   constexpr int W = kMaxBlock + 2 * kHalo;
   __shared__ T sP[W];
@@ -302,25 +358,46 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
 
   // Halo lanes stage columns x0-3..x0-1 (slots 0..2) and x0+B..x0+B+2
   // (slots B+3..B+5).
-  const int hslot = (t < 3) ? t : B + t;
+  // Every lane loads one of the six halo cells (lane t takes cell t mod 6);
+  // only lanes t < 6 stage them in LDS.
+  const int hl = t % (2 * kHalo);
+  const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - kHalo + hslot;
 
   // Windows along y: U rows r-4..r, p rows r-4..r, u_y rows r-3..r,
   // K_y rows r-4..r-2, x flattening chi_x rows r-3..r.
   T Uw[5][S];
-  T pw[5], vyw[4], kyw[3], cxw[4];
+  T pw[5], vyw[4], cxw[4];
+  T ky0 = T(1), ky1 = T(1), ky2 = T(1);  // named scalars (see the x sweep)
 #pragma unroll
   for (int i = 0; i < 5; ++i) pw[i] = T(0);
 #pragma unroll
   for (int i = 0; i < 4; ++i) { vyw[i] = T(0); cxw[i] = T(1); }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) kyw[i] = T(1);
   T Rprev[S], Gprev[S];
 #pragma unroll
   for (int k = 0; k < S; ++k) { Rprev[k] = T(0); Gprev[k] = T(0); }
 
   unsigned long long wsmax = 0ull;
-  for (int64_t r = y0 - 4; r <= yend + 3; ++r) {
+  // Software pipelining with two row buffers in fixed registers (see the x
+  // sweep): rows r+2 (and their halo cells) load while row r is computed.
+  const int64_t rbeg = y0 - 4, rlast = yend + 3;
+  // All loads are unconditional (see the x sweep); the halo cells of row r are
+  // loaded on the same schedule as the row itself.
+  Pending<T, S> QA, QB, HA, HB;
+  issue_load<T, ND>(a, c, rbeg, QA);
+  issue_load<T, ND>(a, c, rbeg + 1, QB);
+  issue_load<T, ND>(a, hcol, rbeg, HA);
+  issue_load<T, ND>(a, hcol, rbeg + 1, HB);
+  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
+  // two-step loop is fully unrolled so that each copy names its buffers
+  // statically.
+  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
+#pragma unroll
+  for (int step = 0; step < 2; ++step) {
+    const int64_t r = r0 + step;
+    if (r > rlast) break;
+    Pending<T, S>& Q = step ? QB : QA;
+    Pending<T, S>& H = step ? HB : HA;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -332,7 +409,11 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
       vyw[i] = vyw[i + 1];
       cxw[i] = cxw[i + 1];
     }
-    load_ghosted<T, ND>(a, c, r, Uw[4]);
+    take<T, ND>(Q, Uw[4]);
+    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
+    T uh[S];
+    take<T, ND>(H, uh);
+    issue_load<T, ND>(a, hcol, min(r + 2, rlast), H);
     const Prim<T, ND> w = prim_of<T, ND>(Uw[4], a.mat);
     pw[4] = w.p;
     vyw[3] = w.vel[1];
@@ -342,8 +423,6 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
       sP[s] = w.p;
       sV[s] = w.vel[0];
       if (t < 2 * kHalo) {
-        T uh[S];
-        load_ghosted<T, ND>(a, hcol, r, uh);
         const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
         sP[hslot] = wh.p;
         sV[hslot] = wh.vel[0];
@@ -361,16 +440,17 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
 
     // ---- y flattening K_y of row r-2
     if (r >= y0) {
-      kyw[0] = kyw[1];
-      kyw[1] = kyw[2];
-      kyw[2] = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
+      ky0 = ky1;
+      ky1 = ky2;
+      ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
     }
 
     // ---- face states of row jr = r-3
     if (r >= y0 + 2) {
       const int sy = signum(pw[2] - pw[0]);
       T chi = smin(Limits<T>::max(), cxw[0]);
-      chi = smin(chi, smin(kyw[1], kyw[1 + sy]));
+      const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
+      chi = smin(chi, smin(ky1, ky_nb));
       T R[S], L[S];
       evolve_faces<T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, a.limiter, a.form, a.mat,
                              R, L);
@@ -395,6 +475,7 @@ __global__ void __launch_bounds__(kMaxBlock) sweep_y_kernel(SweepArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < S; ++k) Rprev[k] = R[k];
     }
+  }
   }
   if (a.ws_out) block_atomic_max(wsmax, a.ws_out);
 }

@@ -11,7 +11,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfluidity_amd.so")
+# FLUIDITY_AMD_LIB selects another build of the same library (tuning builds);
+# by default the in-tree library is loaded.
+LIB_PATH = os.environ.get("FLUIDITY_AMD_LIB") or os.path.join(HERE, "libfluidity_amd.so")
 
 FL_OK, FL_ERR_ARGUMENT, FL_ERR_UNSUPPORTED, FL_ERR_HIP, FL_ERR_NUMERIC = range(5)
 FL_F64, FL_F32 = 0, 1

@@ -1,0 +1,297 @@
+// fluidity_amd.hpp -- the reference's split-solver template API over the C ABI.
+//
+// Header-only C++17.  The types keep the reference's names and template
+// composition so that an instantiation written for the reference,
+//
+//   using solver_t = fluid::solver::SplitSolver<
+//       fluid::solver::FaceFlux<
+//           fluid::recon::MHReconstructor<fluid::limit::VanLeer<fluid::limit::cons_form_t>>,
+//           fluid::flux::Hllc, fluid::material::IdealGas<double>>,
+//       fluid::solver::BoundaryLoader<4>, fluid::Num<2>>;
+//   solver_t solver{in};
+//   solver.solve(in, out, mat, dtdh, setter);
+//
+// (solver/split_solver.hpp:37-126, solver/flux_solver.hpp:27-41,
+// reconstruction/muscl_reconstructor.hpp:31-264, limiting/van_leer_limiter.hpp,
+// flux_method/flux_hllc.hpp, material/ideal_gas.hpp) compiles against this
+// header and runs the gfx950 kernels.  The compile-time choices become the
+// runtime fl_scheme the C ABI takes.  Iterators are GridView values (a device
+// pointer plus interior sizes) in place of the reference's MultidimIterator;
+// solve() swaps the caller's in/out between sweeps as the reference does
+// (solver/cuda/split_solver.cuh:75-78).  Errors throw fluid::amd::Error instead
+// of the reference's print-and-terminate (utility/debug.hpp:42-67).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "../fluidity_amd.h"
+
+namespace fluid {
+
+// Num<D> / Dimension<D> (dimension/dimension.hpp).
+template <std::size_t D> struct Num : std::integral_constant<std::size_t, D> {};
+template <std::size_t D> struct Dimension : std::integral_constant<std::size_t, D> {};
+using dimx_t = Dimension<0>;
+using dimy_t = Dimension<1>;
+static constexpr dimx_t dim_x{};
+static constexpr dimy_t dim_y{};
+
+namespace amd {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(int status) {
+  if (status != FL_OK)
+    throw Error(status, std::string(fl_status_string(status)) + ": " + fl_last_error());
+}
+
+// Owns the fl_context of one device.
+class Context {
+ public:
+  explicit Context(int device = 0) { check(fl_ctx_create(device, &ctx_)); }
+  ~Context() { fl_ctx_destroy(ctx_); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  fl_context* get() const { return ctx_; }
+  static Context& instance(int device = 0) {
+    static Context c(device);
+    return c;
+  }
+
+ private:
+  fl_context* ctx_ = nullptr;
+};
+
+template <typename T> constexpr int dtype_of();
+template <> constexpr int dtype_of<double>() { return FL_F64; }
+template <> constexpr int dtype_of<float>() { return FL_F32; }
+
+// The iterator role of MultidimIterator<State<T, conservative, D>, ..., gpu>:
+// a device pointer to interior cells and the interior sizes.
+template <typename T, std::size_t D, int Layout = FL_LAYOUT_AOS>
+struct GridView {
+  using value_t = T;
+  static constexpr std::size_t dimensions = D;
+  T* data = nullptr;
+  int64_t n[3] = {1, 1, 1};
+
+  GridView() = default;
+  GridView(T* d, int64_t nx, int64_t ny = 1) : data(d) {
+    n[0] = nx;
+    n[1] = ny;
+  }
+  std::size_t size(std::size_t dim) const { return static_cast<std::size_t>(n[dim]); }
+  fl_field field() const {
+    fl_field f{};
+    f.data = data;
+    f.dtype = dtype_of<T>();
+    f.layout = Layout;
+    f.ndim = static_cast<int32_t>(D);
+    f.n[0] = n[0];
+    f.n[1] = n[1];
+    f.n[2] = n[2];
+    return f;
+  }
+};
+
+}  // namespace amd
+
+// ---- limiting (limiting/limiter_traits.hpp, limiting/*.hpp) ----------------
+namespace limit {
+struct cons_form_t { static constexpr int form = FL_FORM_CONSERVATIVE; };
+struct prim_form_t { static constexpr int form = FL_FORM_PRIMITIVE; };
+template <typename Form> struct VanLeer {
+  static constexpr int id = FL_LIMITER_VAN_LEER;
+  static constexpr int form = Form::form;
+  static constexpr std::size_t width = 4;  // LimiterTraits<VanLeer>::width
+};
+template <typename Form> struct Superbee {
+  static constexpr int id = FL_LIMITER_SUPERBEE;
+  static constexpr int form = Form::form;
+  static constexpr std::size_t width = 4;  // LimiterTraits<Superbee>::width
+};
+}  // namespace limit
+
+// ---- reconstruction (reconstruction/muscl_reconstructor.hpp) ---------------
+namespace recon {
+template <typename Limiter> struct MHReconstructor {
+  using limiter_t = Limiter;
+  static constexpr std::size_t width = Limiter::width;
+};
+}  // namespace recon
+
+// ---- flux methods (flux_method/flux_hllc.hpp, flux_force.hpp) --------------
+namespace flux {
+struct Hllc { static constexpr int id = FL_FLUX_HLLC; };
+struct Force { static constexpr int id = FL_FLUX_FORCE; };
+}  // namespace flux
+
+// ---- materials (material/ideal_gas.hpp, material/stiff_fluid.hpp) ----------
+namespace material {
+template <typename T> struct IdealGas {
+  using value_t = T;
+  static constexpr int eos = FL_EOS_IDEAL;
+  constexpr IdealGas() = default;
+  constexpr explicit IdealGas(T adi) : adi_(adi) {}
+  constexpr T adiabatic() const { return adi_; }
+  constexpr T p_inf() const { return T(0); }
+
+ private:
+  T adi_ = T(1.4);
+};
+template <typename T> struct StiffFluid {
+  using value_t = T;
+  static constexpr int eos = FL_EOS_STIFF;
+  constexpr StiffFluid() = default;
+  constexpr explicit StiffFluid(T adi) : adi_(adi) {}
+  constexpr StiffFluid(T adi, T p_inf) : adi_(adi), p_inf_(p_inf) {}
+  constexpr T adiabatic() const { return adi_; }
+  constexpr T p_inf() const { return p_inf_; }
+
+ private:
+  T adi_ = T(5.5);
+  T p_inf_ = T(0.613);
+};
+}  // namespace material
+
+namespace solver {
+
+// BoundaryKind / BoundaryIndex / BoundarySetter (solver/boundary_loader.hpp:29-124).
+enum class BoundaryKind { transmissive = 0, reflective = 1, periodic = 2 };
+enum class BoundaryIndex { first = 0, second = 1 };
+
+// As in the reference, one kind applies to both ends of a dimension
+// (its bit_index ignores the index: solver/boundary_loader.hpp:105-111).
+struct BoundarySetter {
+  template <typename Dim>
+  void configure(Dim dim, BoundaryIndex, BoundaryKind kind) {
+    kinds_[static_cast<std::size_t>(dim)] = static_cast<int32_t>(kind);
+  }
+  int32_t kind(std::size_t dim) const { return kinds_[dim]; }
+
+ private:
+  int32_t kinds_[3] = {0, 0, 0};
+};
+
+template <std::size_t Padding> struct BoundaryLoader {
+  static constexpr std::size_t padding = Padding;
+  static_assert(Padding == 4, "the MUSCL-Hancock stencil with chi flattening needs 4 ghosts");
+};
+
+template <typename ReconMethod, typename FluxMethod, typename Eos>
+struct FaceFlux {
+  using reconstructor_t = ReconMethod;
+  using flux_method_t = FluxMethod;
+  using eos_t = Eos;
+  using value_t = typename Eos::value_t;
+};
+
+template <typename FluxSolver, typename Loader, typename Dims = Num<1>>
+struct SplitSolver {
+  using flux_solver_t = FluxSolver;
+  using loader_t = Loader;
+  using setter_t = BoundarySetter;
+  using recon_t = typename FluxSolver::reconstructor_t;
+  using limiter_t = typename recon_t::limiter_t;
+  using eos_t = typename FluxSolver::eos_t;
+  static constexpr std::size_t num_dimensions = Dims::value;
+  static constexpr std::size_t padding = Loader::padding;
+
+  SplitSolver() = default;
+  template <typename It> explicit SplitSolver(It&&) {}
+  template <typename It> void set_grid_sizes(It&&) {}
+
+  static fl_scheme scheme(const eos_t& mat, const BoundarySetter& setter) {
+    fl_scheme s{};
+    s.limiter = limiter_t::id;
+    s.form = limiter_t::form;
+    s.flux = FluxSolver::flux_method_t::id;
+    s.eos = eos_t::eos;
+    s.gamma = static_cast<double>(mat.adiabatic());
+    s.p_inf = static_cast<double>(mat.p_inf());
+    for (int d = 0; d < 3; ++d) s.bc[d] = setter.kind(d);
+    return s;
+  }
+
+  // SplitSolver::solve (solver/split_solver.hpp:105-115): all sweeps, x then y,
+  // swapping the caller's in/out between sweeps.
+  template <typename It, typename Mat, typename T>
+  void solve(It& in, It& out, Mat&& mat, T dtdh, const BoundarySetter& setter,
+             void* stream = nullptr) const {
+    static_assert(std::decay_t<It>::dimensions == num_dimensions, "grid / solver dims differ");
+    fl_field fi = in.field(), fo = out.field();
+    const fl_scheme s = scheme(mat, setter);
+    amd::check(fl_split_solve(amd::Context::instance().get(), &fi, &fo, &s,
+                              static_cast<double>(dtdh), stream));
+    if (fi.data != in.data) std::swap(in.data, out.data);
+  }
+
+  // One sweep in dimension Dim (solve_dim<S> -> SplitSolver::invoke).
+  template <typename It, typename Mat, typename T, typename Dim>
+  void sweep(const It& in, const It& out, Mat&& mat, T dtdh, const BoundarySetter& setter,
+             Dim dim, void* stream = nullptr) const {
+    fl_field fi = in.field(), fo = out.field();
+    const fl_scheme s = scheme(mat, setter);
+    amd::check(fl_split_sweep(amd::Context::instance().get(), &fi, &fo,
+                              static_cast<int>(dim), &s, static_cast<double>(dtdh), stream));
+  }
+};
+
+}  // namespace solver
+
+// set_wavespeeds + max_element (simulator/wavespeed_initialization.hpp:34-53,
+// algorithm/max_element.hpp:28-38) fused: the max wavespeed of a grid.
+template <typename It, typename Mat>
+double max_wavespeed(const It& in, const Mat& mat, void* stream = nullptr) {
+  fl_field f = in.field();
+  fl_scheme s{};
+  s.gamma = static_cast<double>(mat.adiabatic());
+  s.p_inf = static_cast<double>(mat.p_inf());
+  s.eos = Mat::eos;
+  double w = 0.0;
+  amd::check(fl_max_wavespeed(amd::Context::instance().get(), &f, &s, &w, stream));
+  return w;
+}
+
+namespace sim {
+
+// The parameter surface of SimulatorImpl (simulator/simulator_impl.hpp:260-291)
+// and its simulate() loop (:222-258), with dt kept on the device.
+template <typename Solver>
+class Simulator {
+ public:
+  using eos_t = typename Solver::eos_t;
+
+  Simulator& configure_cfl(double cfl) { p_.cfl = cfl; return *this; }
+  Simulator& configure_resolution(double res) { p_.dh = res; return *this; }
+  Simulator& configure_sim_time(double t) { p_.sim_time = t; return *this; }
+  Simulator& configure_max_iterations(int64_t it) { p_.max_iters = it; return *this; }
+  Simulator& configure_material(const eos_t& m) { mat_ = m; return *this; }
+  solver::BoundarySetter& setter() { return setter_; }
+
+  // Advances `state` in place (`work` is a same-sized scratch grid).
+  template <typename It>
+  fl_sim_result simulate(It& state, It& work, void* stream = nullptr) {
+    fl_field fs = state.field(), fw = work.field();
+    const fl_scheme s = Solver::scheme(mat_, setter_);
+    fl_sim_result r{};
+    amd::check(fl_simulate(amd::Context::instance().get(), &fs, &fw, &s, &p_, &r, stream));
+    return r;
+  }
+
+ private:
+  fl_sim_params p_{0.9, 0.0, 1e300, int64_t(1) << 40};
+  eos_t mat_{};
+  solver::BoundarySetter setter_{};
+};
+
+}  // namespace sim
+}  // namespace fluid
