@@ -12,14 +12,15 @@
 
 #include <string>
 
-#include "../../include/fluidity_amd.h"
-#include "fl_kernels.hpp"
+#include "fl_abi_internal.hpp"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+namespace fl {
+namespace abi {
+
 int fail(int status, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -29,29 +30,6 @@ int fail(int status, const char* fmt, ...) {
   g_last_error = buf;
   return status;
 }
-
-#define FL_HIP(call)                                                               \
-  do {                                                                             \
-    hipError_t e_ = (call);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(FL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
-  } while (0)
-
-constexpr int kSumBlocks = 1024;
-
-}  // namespace
-
-struct fl_context {
-  int device;
-  fl::StepCtl* ctl;            // device control block
-  unsigned long long* ws_tmp;  // device scratch for fl_max_wavespeed
-  double* partial;             // device partial sums, kSumBlocks * 4
-  fl::StepCtl* host_ctl;       // pinned host mirror of ctl
-  double* host_partial;        // pinned host mirror of partial
-  fl_tuning tuning;
-};
-
-namespace {
 
 bool valid_block(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0; }
 
@@ -229,25 +207,16 @@ int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst
   return FL_OK;
 }
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) (void)hipSetDevice(dev);
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
 double bits_to_double(unsigned long long b) {
   double d;
   memcpy(&d, &b, sizeof(d));
   return d;
 }
 
-}  // namespace
+}  // namespace abi
+}  // namespace fl
+
+using namespace fl::abi;
 
 extern "C" {
 
@@ -286,6 +255,8 @@ int fl_ctx_create(int device, fl_context** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->partial, sizeof(double) * kSumBlocks * 4);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_ctl, sizeof(fl::StepCtl), 0);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_partial, sizeof(double) * kSumBlocks * 4, 0);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->eik, sizeof(fl::ls::EikCtl));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, sizeof(fl::ls::EikCtl), 0);
   if (e != hipSuccess) {
     fl_ctx_destroy(c);
     return fail(FL_ERR_HIP, "context allocation failed: %s", hipGetErrorString(e));
@@ -302,6 +273,8 @@ int fl_ctx_destroy(fl_context* c) {
   if (c->partial) (void)hipFree(c->partial);
   if (c->host_ctl) (void)hipHostFree(c->host_ctl);
   if (c->host_partial) (void)hipHostFree(c->host_partial);
+  if (c->eik) (void)hipFree(c->eik);
+  if (c->host_eik) (void)hipHostFree(c->host_eik);
   delete c;
   return FL_OK;
 }

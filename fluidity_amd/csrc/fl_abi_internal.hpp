@@ -1,0 +1,68 @@
+// fl_abi_internal.hpp -- pieces of the C ABI host code shared by fl_abi.hip
+// (split-solver entry points) and fl_ls_abi.hip (level-set / ghost-fluid entry
+// points): the context, error reporting and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "../../include/fluidity_amd.h"
+#include "fl_kernels.hpp"
+#include "fl_levelset.hpp"
+
+struct fl_context {
+  int device;
+  fl::StepCtl* ctl;            // device control block of fl_simulate
+  unsigned long long* ws_tmp;  // device scratch for fl_max_wavespeed
+  double* partial;             // device partial sums, kSumBlocks * 4
+  fl::StepCtl* host_ctl;       // pinned host mirror of ctl
+  double* host_partial;        // pinned host mirror of partial
+  fl::ls::EikCtl* eik;         // device control words of the eikonal loop
+  fl::ls::EikCtl* host_eik;    // pinned host mirror of eik
+  fl_tuning tuning;
+};
+
+namespace fl {
+namespace abi {
+
+constexpr int kSumBlocks = 1024;
+
+// Records the message for fl_last_error() and returns `status`.
+int fail(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define FL_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::fl::abi::fail(FL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+double bits_to_double(unsigned long long b);
+int check_field(const fl_field* f, const char* name);
+int check_scheme(const fl_scheme* s, int ndim);
+void strides(const fl_field* f, int64_t& es, int64_t& cs);
+int64_t cell_count(const fl_field* f);
+// One sweep; ctl/ws_out as in fl_simulate (NULL: dtdh from the argument, no
+// fused wavespeed).
+int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
+                 const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
+                 unsigned long long* ws_out, hipStream_t stream);
+// atomicMax of the FP64 wavespeed bits of every cell into *dst.
+int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst,
+                  hipStream_t stream);
+
+}  // namespace abi
+}  // namespace fl

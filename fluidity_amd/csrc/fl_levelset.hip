@@ -1,0 +1,602 @@
+// fl_levelset.hip -- gfx950 kernels of the ghost-fluid / level-set step (FP64).
+//
+// Replaces the reference's level-set and ghost-fluid kernels: the interface
+// predicates (levelset/levelset.hpp:166-217), set_velocities
+// (levelset/cuda/velocity_initialization.cuh:108-143), the WENO-HJ5 + Godunov
+// + RK3 evolve kernel (scheme/cuda/evolve.cuh:51-157 with
+// scheme/stencils/weno_hj_5.hpp, scheme/schemes/godunov_upwind.hpp,
+// scheme/evaluators/levelset_hamiltonian.hpp, scheme/updaters/runge_kutta_3.hpp),
+// the projection (levelset/cuda/levelset_projection.cuh:54-66), the fast
+// iterative eikonal solver (scheme/cuda/fast_iterative_method.cuh) and the
+// Riemann ghost-fluid star states and extrapolation
+// (ghost_fluid/riemann_ghost_fluid.hpp, ghost_fluid/ghost_fluid_extrapolator.hpp).
+//
+// One thread per cell; level-set fields are small next to the conserved
+// state and read through L2.  Where the reference is undefined the kernels
+// follow these rules: RK3 stages are global; level-set ghosts mirror the
+// interior and a zero gradient gives a zero normal; level set m moves with
+// u . n_m; the eikonal solve is a Jacobi iteration to the exact fixed point
+// inside a band of cells, from sources carrying |phi|, with neighbours across
+// the interface entering with negated distance; ghost-fluid states are interpolated bilinearly,
+// rotated through the primitive state, stored at the interfacial cell and
+// extrapolated by Jacobi sweeps over 0 < phi <= 6 dh / sqrt(2).  Every
+// kernel is a deterministic function of its inputs, so results do not depend
+// on launch geometry or scheduling.
+#include "fl_levelset.hpp"
+
+namespace fl {
+namespace ls {
+
+// Mirrored ghost index; clamped so that no input can address outside the grid.
+__device__ __forceinline__ int64_t mirror(int64_t i, int64_t n) {
+  if (i < 0) i = -i - 1;
+  if (i >= n) i = 2 * n - 1 - i;
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+__device__ __forceinline__ double phi_at(const double* __restrict__ phi, const Grid& g,
+                                         int64_t x, int64_t y) {
+  const int64_t xi = mirror(x, g.nx);
+  const int64_t yi = g.nd > 1 ? mirror(y, g.ny) : 0;
+  return phi[yi * g.nx + xi];
+}
+
+// levelset.hpp:166-217 as mask bits.
+__device__ __forceinline__ uint8_t mask_value(double p, double dh) {
+  const int inside = (p <= 0.0) || (p <= -0.0);
+  const int outside = p > 0.0;
+  const int on_boundary = (p == 0.0) || (p == -0.0);
+  const int in_ifc = (inside || on_boundary) && (p >= -dh);
+  const int out_ifc = outside && (p < fabs(dh));
+  return (uint8_t)(inside | (outside << 1) | (on_boundary << 2) | (in_ifc << 3) |
+                   (out_ifc << 4));
+}
+
+// MultidimIterator::norm (iterator/multidim_iterator.hpp:291-306).
+__device__ __forceinline__ void norm_at(const double* __restrict__ phi, const Grid& g,
+                                        int64_t x, int64_t y, double dh, double* nrm) {
+  const double fact = 0.5 / dh;
+  double mag = 0.0;
+  for (int d = 0; d < g.nd; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    nrm[d] = fact * (phi_at(phi, g, x + dx, y + dy) - phi_at(phi, g, x - dx, y - dy));
+    mag += nrm[d] * nrm[d];
+  }
+  const double s = sqrt(mag);
+  for (int d = 0; d < g.nd; ++d) nrm[d] = (mag == 0.0) ? 0.0 : nrm[d] / s;
+}
+
+template <int S>
+__device__ __forceinline__ void load_state(const StateView& v, const Grid& g, int64_t x,
+                                           int64_t y, double* u) {
+  const int64_t xi = mirror(x, g.nx);
+  const int64_t yi = g.nd > 1 ? mirror(y, g.ny) : 0;
+  const double* p = v.data + (yi * g.nx + xi) * v.es;
+#pragma unroll
+  for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
+}
+
+template <int S>
+__device__ __forceinline__ void store_state(const StateView& v, int64_t i, const double* u) {
+  double* p = v.data + i * v.es;
+#pragma unroll
+  for (int k = 0; k < S; ++k) p[k * v.cs] = u[k];
+}
+
+__device__ __forceinline__ int64_t gid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// predicates, velocity, projection
+// ---------------------------------------------------------------------------
+
+__global__ void masks_kernel(const double* __restrict__ phi, Grid g, double dh,
+                             uint8_t* __restrict__ masks) {
+  const int64_t i = gid();
+  if (i < g.nx * g.ny) masks[i] = mask_value(phi[i], dh);
+}
+
+// set_velocities: V = u . n of the first material whose phi <= 0.
+// target < 0: the normal of the present material's level set (the
+// reference's field); target = m: the normal of level set m.
+template <int ND>
+__global__ void velocity_kernel(const double* __restrict__ phi0, const double* __restrict__ phi1,
+                                StateView u0, StateView u1, int nmat, Grid g, double dh,
+                                int target, double* __restrict__ V) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  double v = 0.0;
+  for (int m = 0; m < nmat; ++m) {
+    const double* phi = m == 0 ? phi0 : phi1;
+    const double p = phi[i];
+    if ((p <= 0.0) || (p <= -0.0)) {
+      double nrm[2];
+      norm_at(target < 0 ? phi : (target == 0 ? phi0 : phi1), g, x, y, dh, nrm);
+      double u[2 + ND];
+      load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
+      double sum = 0.0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) sum += (u[2 + d] / u[0]) * nrm[d];
+      v = sum;
+      break;
+    }
+  }
+  V[i] = v;
+}
+
+__global__ void project2_kernel(double* __restrict__ phi0, double* __restrict__ phi1,
+                                int64_t count) {
+  const int64_t i = gid();
+  if (i >= count) return;
+  const double avg = 0.5 * (phi0[i] + phi1[i]);
+  const double a = phi0[i] - avg;
+  phi0[i] = a;
+  phi1[i] = -a;
+}
+
+// ---------------------------------------------------------------------------
+// WENO-HJ5 + Godunov + RK3
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// HJWeno5::diff (scheme/stencils/weno_hj_5.hpp:97-143), std::pow(x, 2) as x*x.
+__device__ __forceinline__ double weno_diff(double d1, double d2, double d3, double d4,
+                                            double d5) {
+  const double phi_1 = 2.0 * d1 - 7.0 * d2 + 11.0 * d3;
+  const double phi_2 = -d2 + 5.0 * d3 + 2.0 * d4;
+  const double phi_3 = 2.0 * d3 + 5.0 * d4 - d5;
+  const double f1 = 13.0 / 12.0;
+  const double f2 = 1.0 / 4.0;
+  const double s11 = d1 - 2.0 * d2 + d3, s12 = d1 - 4.0 * d2 + 3.0 * d3;
+  const double s21 = d2 - 2.0 * d3 + d4, s22 = d2 - d4;
+  const double s31 = d3 - 2.0 * d4 + d5, s32 = 3.0 * d3 - 4.0 * d4 + d5;
+  double alpha_1 = f1 * (s11 * s11) + f2 * (s12 * s12);
+  double alpha_2 = f1 * (s21 * s21) + f2 * (s22 * s22);
+  double alpha_3 = f1 * (s31 * s31) + f2 * (s32 * s32);
+  const double e =
+      10.0e-6 * smax(smax(smax(smax(d1 * d1, d2 * d2), d3 * d3), d4 * d4), d5 * d5) + 10.0e-99;
+  const double a1e = alpha_1 + e, a2e = alpha_2 + e, a3e = alpha_3 + e;
+  alpha_1 = 0.1 / (a1e * a1e);
+  alpha_2 = 0.6 / (a2e * a2e);
+  alpha_3 = 0.3 / (a3e * a3e);
+  const double w_denom = (alpha_1 + alpha_2 + alpha_3) * 6.0;
+  return (alpha_1 * phi_1 + alpha_2 * phi_2 + alpha_3 * phi_3) / w_denom;
+}
+
+// L = V * H(phi) (levelset_hamiltonian.hpp:63-98, godunov_upwind.hpp:94-166).
+__device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int64_t x, int64_t y,
+                         double v, double dh) {
+  double result = 0.0;
+  const double h1 = 1.0 / dh;
+  for (int d = 0; d < g.nd; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    double f[7];
+#pragma unroll
+    for (int k = -3; k <= 3; ++k) f[k + 3] = phi_at(phi, g, x + k * dx, y + k * dy);
+    const double bd = weno_diff(h1 * (f[1] - f[0]), h1 * (f[2] - f[1]), h1 * (f[3] - f[2]),
+                                h1 * (f[4] - f[3]), h1 * (f[5] - f[4]));
+    const double fd = weno_diff(h1 * (f[6] - f[5]), h1 * (f[5] - f[4]), h1 * (f[4] - f[3]),
+                                h1 * (f[3] - f[2]), h1 * (f[2] - f[1]));
+    double back, fwrd;
+    if (v <= 0.0) {
+      back = smax(bd, 0.0);
+      fwrd = smin(fd, 0.0);
+    } else {
+      back = smin(bd, 0.0);
+      fwrd = smax(fd, 0.0);
+    }
+    result += smax(back * back, fwrd * fwrd);
+  }
+  return v * sqrt(result);
+}
+
+// One RK3 stage (scheme/updaters/runge_kutta_3.hpp:117-148):
+//  stage 0: out = phi_n - dt L(phi_n)
+//  stage 1: out = 0.75 phi_n + 0.25 (s1 - dt L(s1))
+//  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
+__global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
+                                 const double* __restrict__ src, const double* __restrict__ V,
+                                 Grid g, double dt, double dh, double* __restrict__ out) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  const double l = ls_rhs(src, g, x, y, V[i], dh);
+  if (stage == 0) {
+    out[i] = phi_n[i] - dt * l;
+  } else if (stage == 1) {
+    out[i] = 0.75 * phi_n[i] + 0.25 * (src[i] - dt * l);
+  } else {
+    const double fact_13 = 1.0 / 3.0;
+    const double fact_23 = 2.0 * fact_13;
+    const double s3 = src[i] - dt * l;
+    out[i] = (fact_13 * phi_n[i]) + (fact_23 * s3);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// eikonal reinitialisation
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int signum(double x) { return (0.0 < x) - (x < 0.0); }
+
+// Sources (fast_iterative_init, scheme/cuda/fast_iterative_method.cuh:48-85):
+// a cell beside a sign change with the smaller |phi|; distance |phi| there.
+// info = source | (signum(phi) + 1) << 1.
+__global__ void eik_init_kernel(const double* __restrict__ phi, Grid g,
+                                double* __restrict__ d, uint8_t* __restrict__ info) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  const double c = phi[i];
+  int source = 0;
+  for (int dd = 0; dd < g.nd && !source; ++dd) {
+    const int64_t dx = (dd == 0), dy = (dd == 1);
+    for (int s = 0; s < 2 && !source; ++s) {
+      const int64_t ox = x + (s == 0 ? dx : -dx), oy = y + (s == 0 ? dy : -dy);
+      if (ox < 0 || ox >= g.nx || oy < 0 || oy >= g.ny) continue;
+      const double o = phi[oy * g.nx + ox];
+      if (signum(c) != signum(o) && fabs(c) < fabs(o)) source = 1;
+    }
+  }
+  info[i] = (uint8_t)(source | ((signum(c) + 1) << 1));
+  d[i] = source ? fabs(c) : DBL_MAX;
+}
+
+// Neighbour distance seen from a cell with sign bits sc: negated across the
+// interface unless not yet reached.
+__device__ __forceinline__ double eik_nbr(const double* __restrict__ d,
+                                          const uint8_t* __restrict__ info, int64_t j,
+                                          uint8_t sc) {
+  const double v = d[j];
+  return ((info[j] & 6) == sc || v == DBL_MAX) ? v : -v;
+}
+
+__global__ void eik_ctl_init_kernel(EikCtl* c) {
+  c->flags[0] = c->flags[1] = c->flags[2] = 0;
+  c->done = 0;
+  c->iters = 0;
+}
+
+// Sweep k of the Godunov eikonal update
+// (scheme/schemes/godunov_upwind_scheme.hpp:37-53, f = 1) restricted to the
+// band: a cell takes q when q < d and q <= limit.  A sweep after one that
+// changed nothing marks the loop done; later sweeps return at once, so a
+// batch of sweeps can be launched ahead of the convergence test.  Every
+// active sweep writes every cell, so once converged both buffers agree.
+__global__ void eik_sweep_kernel(const double* __restrict__ d, const uint8_t* __restrict__ info,
+                                 Grid g, double dh, double limit, double* __restrict__ e,
+                                 EikCtl* __restrict__ ctl, int k) {
+  const int64_t i = gid();
+  if (ctl->done) return;
+  if (k > 0 && ctl->flags[(k - 1) % 3] == 0) {
+    if (i == 0) ctl->done = 1;
+    return;
+  }
+  if (i == 0) {
+    ctl->flags[(k + 1) % 3] = 0;
+    ctl->iters = k + 1;
+  }
+  if (i >= g.nx * g.ny) return;
+  const double cur = d[i];
+  double nv = cur;
+  const uint8_t me = info[i];
+  if (!(me & 1)) {
+    const uint8_t sc = me & 6;
+    const int64_t x = i % g.nx, y = i / g.nx;
+    const double fh = 1.0 * dh;
+    const double xm = x > 0 ? eik_nbr(d, info, i - 1, sc) : DBL_MAX;
+    const double xp = x < g.nx - 1 ? eik_nbr(d, info, i + 1, sc) : DBL_MAX;
+    const double a = smin(xm, xp);
+    double q;
+    if (g.nd == 1) {
+      q = a + fh;
+    } else {
+      const double ym = y > 0 ? eik_nbr(d, info, i - g.nx, sc) : DBL_MAX;
+      const double yp = y < g.ny - 1 ? eik_nbr(d, info, i + g.nx, sc) : DBL_MAX;
+      const double b = smin(ym, yp);
+      const double amb = a - b;
+      q = fabs(amb) >= fh ? smin(a, b) + fh : 0.5 * (a + b + sqrt(2.0 * fh * fh - (amb * amb)));
+    }
+    if (q < cur && q <= limit) {
+      nv = q;
+      ctl->flags[k % 3] = 1;
+    }
+  }
+  e[i] = nv;
+}
+
+__global__ void eik_finish_kernel(const double* __restrict__ phi, const double* __restrict__ d,
+                                  Grid g, double limit, double clamp, double* __restrict__ out,
+                                  uint8_t* __restrict__ band_mask) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int in_band = d[i] <= limit;
+  const double dist = in_band ? d[i] : clamp;
+  out[i] = (double)signum(phi[i]) * dist;
+  if (band_mask) band_mask[i] = (uint8_t)in_band;
+}
+
+// ---------------------------------------------------------------------------
+// Riemann ghost fluid
+// ---------------------------------------------------------------------------
+
+// Conservative <-> primitive with the reference's expressions
+// (state/state_impl.hpp:117-142, 340-406); stiff: pressure ignores p_inf,
+// energy uses it (material/stiff_fluid.hpp:77-98).
+template <int ND>
+__device__ __forceinline__ void cons_to_prim(const double* u, const Eos& e, double* w) {
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const double v = u[2 + i] / u[0];
+    sum += v * v;
+  }
+  w[0] = u[0];
+  w[1] = (e.gamma - 1.0) * (u[1] - 0.5 * u[0] * sum);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) w[2 + i] = u[2 + i] / u[0];
+}
+
+template <int ND>
+__device__ __forceinline__ void prim_to_cons(const double* w, const Eos& e, double* u) {
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) sum += w[2 + i] * w[2 + i];
+  const double eos = e.stiff ? (w[1] + e.gamma * e.p_inf) / ((e.gamma - 1.0) * w[0])
+                             : w[1] / ((e.gamma - 1.0) * w[0]);
+  u[0] = w[0];
+  u[1] = w[0] * (0.5 * sum + eos);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) u[2 + i] = w[2 + i] * w[0];
+}
+
+template <int ND>
+__device__ __forceinline__ double pressure(const double* u, const Eos& e) {
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const double v = u[2 + i] / u[0];
+    sum += v * v;
+  }
+  return (e.gamma - 1.0) * (u[1] - 0.5 * u[0] * sum);
+}
+
+template <int ND>
+__device__ __forceinline__ double sound(const double* u, const Eos& e) {
+  const double p = pressure<ND>(u, e);
+  return e.stiff ? sqrt(e.gamma * (p + e.p_inf) / u[0]) : sqrt(e.gamma * p / u[0]);
+}
+
+// Hllc::solve_star_conservative_left (flux_method/flux_hllc.hpp:301-335) with
+// wavespeed_slr, wavespeed_star and star_state_cons (:420-571), x direction.
+template <int ND>
+__device__ void hllc_star_left(const double* ul, const double* ur, const Eos& el,
+                               const Eos& er, double* out) {
+  const double al = sound<ND>(ul, el), ar = sound<ND>(ur, er);
+  const double pl = pressure<ND>(ul, el), pr = pressure<ND>(ur, er);
+  const double vl = ul[2] / ul[0], vr = ur[2] / ur[0];
+  const double p_pvrs = 0.5 * ((pl + pr) - (0.25 * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
+  const double p_star = smax(0.0, p_pvrs);
+  const double ql = (el.gamma + 1.0) / (2.0 * el.gamma);
+  const double qr = (er.gamma + 1.0) / (2.0 * er.gamma);
+  const double wsl = (p_star <= pl) ? vl + -al : vl + -al * sqrt(1.0 + ql * (p_star / pl - 1.0));
+  const double wsr = (p_star <= pr) ? vr + ar : vr + ar * sqrt(1.0 + qr * (p_star / pr - 1.0));
+  const double factorl = ul[0] * (wsl - vl);
+  const double factorr = ur[0] * (wsr - vr);
+  const double ws_star = (pr - pl + vl * factorl - vr * factorr) / (factorl - factorr);
+  const double state_factor = wsl - vl;
+  const double scale_factor = state_factor / (wsl - ws_star);
+#pragma unroll
+  for (int k = 0; k < 2 + ND; ++k) out[k] = ul[k] * scale_factor;
+  out[2] = ws_star * ul[0] * scale_factor;
+  out[1] = scale_factor * (ul[1] + ul[0] * (ws_star - vl) * (ws_star + pl / (ul[0] * state_factor)));
+}
+
+// math::interp (math/interp.hpp:56-165) as a proper (bi)linear interpolation.
+template <int ND>
+__device__ void interp_state(const StateView& v, const Grid& g, int64_t x, int64_t y,
+                             const double* w, double* out) {
+  constexpr int S = 2 + ND;
+  int64_t o0[2] = {0, 0}, o1[2] = {0, 0};
+  double fr[2] = {0.0, 0.0};
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const int sign = signum(w[d]);
+    const double abs_w = fabs(w[d]);
+    const double abs_off = floor(abs_w);
+    o0[d] = (int64_t)(sign * abs_off);
+    o1[d] = o0[d] + sign;
+    fr[d] = abs_w - abs_off;
+  }
+  double a[S], b[S];
+  if constexpr (ND == 1) {
+    load_state<S>(v, g, x + o0[0], y, a);
+    load_state<S>(v, g, x + o1[0], y, b);
+#pragma unroll
+    for (int k = 0; k < S; ++k) out[k] = a[k] * (1.0 - fr[0]) + b[k] * fr[0];
+  } else {
+    double c[S], e[S];
+    load_state<S>(v, g, x + o0[0], y + o0[1], a);
+    load_state<S>(v, g, x + o1[0], y + o0[1], b);
+    load_state<S>(v, g, x + o0[0], y + o1[1], c);
+    load_state<S>(v, g, x + o1[0], y + o1[1], e);
+    const double factor = 1.0 - fr[0];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const double close = a[k] * factor + b[k] * fr[0];
+      const double far = c[k] * factor + e[k] * fr[0];
+      out[k] = close * (1.0 - fr[1]) + far * fr[1];
+    }
+  }
+}
+
+// RiemannGFM<3>::invoke (ghost_fluid/riemann_ghost_fluid.hpp:50-195).
+template <int ND>
+__global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* __restrict__ phi_o,
+                                StateView um, StateView uo, Eos em, Eos eo, Grid g, double dh,
+                                double* __restrict__ star, uint8_t* __restrict__ has_star) {
+  constexpr int S = 2 + ND;
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  has_star[i] = 0;
+  const double p = phi_m[i];
+  if (!(mask_value(p, dh) & 8)) return;
+  for (int d = 0; d < ND; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    if (phi_at(phi_m, g, x - dx, y - dy) == 0.0 || phi_at(phi_m, g, x + dx, y + dy) == 0.0) return;
+  }
+  const uint8_t mo = mask_value(phi_o[i], dh);
+  if (!((mo & 16) || (mo & 4))) return;
+  double nrm[2] = {0.0, 0.0};
+  norm_at(phi_m, g, x, y, dh, nrm);
+  const double abs_phi = fabs(p);
+  const double dh_three_two = 1.5 * dh;
+  double w[2] = {0.0, 0.0}, wl[S], wr[S];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) w[d] = nrm[d] * (abs_phi - dh_three_two) / dh;
+  interp_state<ND>(um, g, x, y, w, wl);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) w[d] = nrm[d] * (abs_phi + dh_three_two) / dh;
+  interp_state<ND>(uo, g, x, y, w, wr);
+  double Wl[S], Wr[S], vl[2] = {0.0, 0.0};
+  cons_to_prim<ND>(wl, em, Wl);
+  cons_to_prim<ND>(wr, eo, Wr);
+  double vl_n = 0.0, vr_n = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    vl[d] = Wl[2 + d];
+    vl_n += Wl[2 + d] * nrm[d];
+    vr_n += Wr[2 + d] * nrm[d];
+  }
+  Wl[2] = vl_n;
+  Wr[2] = vr_n;
+#pragma unroll
+  for (int d = 1; d < ND; ++d) {
+    Wl[2 + d] = 0.0;
+    Wr[2 + d] = 0.0;
+  }
+  prim_to_cons<ND>(Wl, em, wl);
+  prim_to_cons<ND>(Wr, eo, wr);
+  double st[S], Ws[S];
+  hllc_star_left<ND>(wl, wr, em, eo, st);
+  cons_to_prim<ND>(st, em, Ws);
+  const double vs_n = Ws[2];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) Ws[2 + d] = vl[d] + (vs_n - vl_n) * nrm[d];
+  double out[S];
+  prim_to_cons<ND>(Ws, em, out);
+#pragma unroll
+  for (int k = 0; k < S; ++k) star[i * S + k] = out[k];
+  has_star[i] = 1;
+}
+
+// Extrapolation source field, in both Jacobi buffers: the material's own state
+// (AoS scratch), with the star state at interfacial cells.
+template <int ND>
+__global__ void gfm_seed_kernel(StateView um, const double* __restrict__ star,
+                                const uint8_t* __restrict__ has_star, Grid g,
+                                double* __restrict__ A, double* __restrict__ B) {
+  constexpr int S = 2 + ND;
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  double u[S];
+  if (has_star[i]) {
+#pragma unroll
+    for (int k = 0; k < S; ++k) u[k] = star[i * S + k];
+  } else {
+    load_state<S>(um, g, i % g.nx, i / g.nx, u);
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    A[i * S + k] = u[k];
+    B[i * S + k] = u[k];
+  }
+}
+
+// GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202),
+// one Jacobi sweep over the ghost band 0 < phi <= 6 dh / sqrt(2).
+template <int ND>
+__global__ void gfm_extrap_kernel(const double* __restrict__ phi_m, const double* __restrict__ A,
+                                  Grid g, double dh, double upper, double* __restrict__ B) {
+  constexpr int S = 2 + ND;
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const double p = phi_m[i];
+  if (!(p > 0.0 && p <= upper)) return;  // both buffers hold the seed here
+  const int64_t x = i % g.nx, y = i / g.nx;
+  double nrm[2] = {0.0, 0.0};
+  norm_at(phi_m, g, x, y, dh, nrm);
+  const int sg = signum(p);
+  double l1 = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    nrm[d] = -sg * nrm[d];
+    l1 += fabs(nrm[d]);
+  }
+  double acc[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const int dir = phi_at(phi_m, g, x - dx, y - dy) < phi_at(phi_m, g, x + dx, y + dy) ? -1 : 1;
+    const int64_t ux = mirror(x + dir * dx, g.nx);
+    const int64_t uy = ND > 1 ? mirror(y + dir * dy, g.ny) : 0;
+    const double* src = A + (uy * g.nx + ux) * S;
+    const double wgt = l1 > 1e-2 ? fabs(nrm[d]) : 0.5;
+#pragma unroll
+    for (int k = 0; k < S; ++k) acc[k] += src[k] * wgt;
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) B[i * S + k] = (l1 > 1e-2) ? acc[k] / l1 : acc[k];
+}
+
+template <int ND>
+__global__ void gfm_writeback_kernel(const double* __restrict__ phi_m, const double* __restrict__ A,
+                                     Grid g, double upper, StateView um) {
+  constexpr int S = 2 + ND;
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const double p = phi_m[i];
+  if (p > 0.0 && p <= upper) store_state<S>(um, i, A + i * S);
+}
+
+// combine_into_first (material/cuda/combine_materials.cuh:39-63): the first
+// material whose phi <= 0 provides the output state.
+template <int ND>
+__global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, StateView u1,
+                               Grid g, StateView out) {
+  constexpr int S = 2 + ND;
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  double s[S];
+  const double p = phi0[i];
+  load_state<S>(((p <= 0.0) || (p <= -0.0)) ? u0 : u1, g, i % g.nx, i / g.nx, s);
+  store_state<S>(out, i, s);
+}
+
+#define FL_LS_INST(ND)                                                                       \
+  template __global__ void velocity_kernel<ND>(const double*, const double*, StateView,      \
+                                               StateView, int, Grid, double, int, double*); \
+  template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
+                                               StateView, Eos, Eos, Grid, double, double*,   \
+                                               uint8_t*);                                   \
+  template __global__ void gfm_seed_kernel<ND>(StateView, const double*, const uint8_t*,     \
+                                               Grid, double*, double*);                      \
+  template __global__ void gfm_extrap_kernel<ND>(const double*, const double*, Grid, double,  \
+                                                 double, double*);                           \
+  template __global__ void gfm_writeback_kernel<ND>(const double*, const double*, Grid,      \
+                                                    double, StateView);                      \
+  template __global__ void combine_kernel<ND>(const double*, StateView, StateView, Grid,     \
+                                              StateView);
+
+FL_LS_INST(1)
+FL_LS_INST(2)
+
+}  // namespace ls
+}  // namespace fl

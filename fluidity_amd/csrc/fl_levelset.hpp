@@ -1,0 +1,70 @@
+// fl_levelset.hpp -- argument blocks and declarations of the level-set and
+// ghost-fluid kernels (fl_levelset.hip), shared with the C ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+namespace fl {
+namespace ls {
+
+struct Grid {
+  int64_t nx, ny;  // ny = 1 in 1D
+  int nd;
+  int pad;
+};
+
+// A conservative state grid: element stride and component stride in doubles
+// (AoS: es = S, cs = 1; SoA: es = 1, cs = cells).
+struct StateView {
+  double* data;
+  int64_t es, cs;
+};
+
+// Control words of the eikonal Jacobi loop: flags[k % 3] records a change in
+// sweep k, done ends the loop, iters counts the sweeps run.
+struct EikCtl {
+  int flags[3];
+  int done;
+  long long iters;
+};
+
+struct Eos {
+  double gamma, p_inf;
+  int stiff;
+  int pad;
+};
+
+__global__ void masks_kernel(const double* phi, Grid g, double dh, uint8_t* masks);
+template <int ND>
+__global__ void velocity_kernel(const double* phi0, const double* phi1, StateView u0,
+                                StateView u1, int nmat, Grid g, double dh, int target, double* V);
+__global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
+__global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
+                                 const double* V, Grid g, double dt, double dh, double* out);
+__global__ void eik_init_kernel(const double* phi, Grid g, double* d, uint8_t* info);
+__global__ void eik_ctl_init_kernel(EikCtl* c);
+__global__ void eik_sweep_kernel(const double* d, const uint8_t* info, Grid g, double dh,
+                                 double limit, double* e, EikCtl* ctl, int k);
+__global__ void eik_finish_kernel(const double* phi, const double* d, Grid g, double limit,
+                                  double clamp, double* out, uint8_t* band_mask);
+template <int ND>
+__global__ void gfm_star_kernel(const double* phi_m, const double* phi_o, StateView um,
+                                StateView uo, Eos em, Eos eo, Grid g, double dh, double* star,
+                                uint8_t* has_star);
+template <int ND>
+__global__ void gfm_seed_kernel(StateView um, const double* star, const uint8_t* has_star,
+                                Grid g, double* A, double* B);
+template <int ND>
+__global__ void gfm_extrap_kernel(const double* phi_m, const double* A, Grid g, double dh,
+                                  double upper, double* B);
+template <int ND>
+__global__ void gfm_writeback_kernel(const double* phi_m, const double* A, Grid g,
+                                     double upper, StateView um);
+template <int ND>
+__global__ void combine_kernel(const double* phi0, StateView u0, StateView u1, Grid g,
+                               StateView out);
+
+}  // namespace ls
+}  // namespace fl

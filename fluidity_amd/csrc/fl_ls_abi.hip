@@ -1,0 +1,489 @@
+// fl_ls_abi.hip -- host side of the level-set and ghost-fluid entry points of
+// include/fluidity_amd.h: argument checks and launches of the kernels in
+// fl_levelset.hip, and the two-material step loop.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include "fl_abi_internal.hpp"
+
+using namespace fl::abi;
+namespace ls = fl::ls;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kEikBatch = 8;  // eikonal sweeps launched per convergence test
+
+dim3 grid_for(int64_t cells) { return dim3((unsigned)((cells + kBlock - 1) / kBlock)); }
+
+int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+int check_shape(int ndim, const int64_t* n, ls::Grid* g) {
+  if (ndim != 1 && ndim != 2) return fail(FL_ERR_ARGUMENT, "ndim %d must be 1 or 2", ndim);
+  if (!n) return fail(FL_ERR_ARGUMENT, "n is NULL");
+  for (int d = 0; d < ndim; ++d)
+    if (n[d] < 4 || n[d] > ((int64_t)1 << 30))
+      return fail(FL_ERR_ARGUMENT, "n[%d] = %lld out of range [4, 2^30]", d, (long long)n[d]);
+  g->nx = n[0];
+  g->ny = ndim > 1 ? n[1] : 1;
+  g->nd = ndim;
+  g->pad = 0;
+  if (g->nx * g->ny > ((int64_t)1 << 32))
+    return fail(FL_ERR_ARGUMENT, "grid of %lld cells too large", (long long)(g->nx * g->ny));
+  return FL_OK;
+}
+
+int check_dh(double dh) {
+  if (!(dh > 0.0) || !isfinite(dh)) return fail(FL_ERR_ARGUMENT, "dh %g must be > 0", dh);
+  return FL_OK;
+}
+
+// An FP64 state field over grid g.
+int check_state(const fl_field* f, const ls::Grid& g, const char* name) {
+  int rc = check_field(f, name);
+  if (rc) return rc;
+  if (f->dtype != FL_F64)
+    return fail(FL_ERR_UNSUPPORTED, "%s: the ghost-fluid path runs in FP64 only", name);
+  if (f->ndim != g.nd || f->n[0] != g.nx || (g.nd > 1 && f->n[1] != g.ny))
+    return fail(FL_ERR_ARGUMENT, "%s does not match the level-set grid", name);
+  return FL_OK;
+}
+
+ls::StateView view_of(const fl_field* f) {
+  ls::StateView v;
+  v.data = (double*)f->data;
+  strides(f, v.es, v.cs);
+  return v;
+}
+
+ls::Eos eos_of(const fl_scheme* s) {
+  ls::Eos e;
+  e.gamma = s->gamma;
+  e.p_inf = s->p_inf;
+  e.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
+  e.pad = 0;
+  return e;
+}
+
+int launch_masks(const double* phi, const ls::Grid& g, double dh, uint8_t* masks,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(ls::masks_kernel, grid_for(g.nx * g.ny), dim3(kBlock), 0, st, phi, g, dh,
+                     masks);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int launch_velocity(int nmat, const double* const* phis, const fl_field* states,
+                    const ls::Grid& g, double dh, int target, double* V, hipStream_t st) {
+  const ls::StateView u0 = view_of(&states[0]);
+  const ls::StateView u1 = nmat > 1 ? view_of(&states[1]) : u0;
+  const double* p1 = nmat > 1 ? phis[1] : phis[0];
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::velocity_kernel<1>, grid_for(g.nx), dim3(kBlock), 0, st, phis[0], p1,
+                       u0, u1, nmat, g, dh, target, V);
+  else
+    hipLaunchKernelGGL(ls::velocity_kernel<2>, grid_for(g.nx * g.ny), dim3(kBlock), 0, st,
+                       phis[0], p1, u0, u1, nmat, g, dh, target, V);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// RK3: phi -> s1 -> s2 -> out, s1/s2 in work.
+int launch_advect(const double* phi, const double* V, const ls::Grid& g, double dt, double dh,
+                  double* out, double* work, hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
+  double* s1 = work;
+  double* s2 = work + cells;
+  const dim3 grid = grid_for(cells);
+  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, dh,
+                     s1);
+  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1,
+                     V, g, dt, dh, s2);
+  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2,
+                     V, g, dt, dh, out);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int launch_project(double* phi0, double* phi1, int64_t cells, hipStream_t st) {
+  hipLaunchKernelGGL(ls::project2_kernel, grid_for(cells), dim3(kBlock), 0, st, phi0, phi1,
+                     cells);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int64_t reinit_work_bytes(const ls::Grid& g) {
+  const int64_t cells = g.nx * g.ny;
+  return 2 * align256(cells * 8) + align256(cells);
+}
+
+int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, int band,
+               double* out, uint8_t* band_mask, void* work, int64_t* iterations,
+               hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
+  char* w = (char*)work;
+  double* buf[2] = {(double*)w, (double*)(w + align256(cells * 8))};
+  uint8_t* info = (uint8_t*)(w + 2 * align256(cells * 8));
+  const double limit = band * dh;
+  const double clamp = (band + 1) * dh;
+  const dim3 grid = grid_for(cells);
+  hipLaunchKernelGGL(ls::eik_init_kernel, grid, dim3(kBlock), 0, st, phi, g, buf[0], info);
+  hipLaunchKernelGGL(ls::eik_ctl_init_kernel, dim3(1), dim3(1), 0, st, c->eik);
+  FL_HIP(hipGetLastError());
+  // Every sweep strictly lowers some cell, so the loop ends; the bound only
+  // guards the host against a device that never reports convergence.
+  const int64_t max_sweeps = cells + 2;
+  int64_t launched = 0;
+  for (;;) {
+    for (int b = 0; b < kEikBatch; ++b, ++launched) {
+      const int k = (int)launched;
+      hipLaunchKernelGGL(ls::eik_sweep_kernel, grid, dim3(kBlock), 0, st,
+                         (const double*)buf[k & 1], (const uint8_t*)info, g, dh, limit,
+                         buf[(k + 1) & 1], c->eik, k);
+    }
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipMemcpyAsync(c->host_eik, c->eik, sizeof(ls::EikCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    const ls::EikCtl h = *c->host_eik;
+    if (h.done || h.flags[(h.iters - 1) % 3] == 0) break;
+    if (launched >= max_sweeps)
+      return fail(FL_ERR_NUMERIC, "eikonal solve did not converge in %lld sweeps",
+                  (long long)launched);
+  }
+  const int64_t iters = c->host_eik->iters;
+  hipLaunchKernelGGL(ls::eik_finish_kernel, grid, dim3(kBlock), 0, st, phi,
+                     (const double*)buf[iters & 1], g, limit, clamp, out, band_mask);
+  FL_HIP(hipGetLastError());
+  if (iterations) *iterations = iters;
+  return FL_OK;
+}
+
+int launch_star(const double* phi_m, const double* phi_o, const fl_field* sm,
+                const fl_field* so, const fl_scheme* em, const fl_scheme* eo, const ls::Grid& g,
+                double dh, double* star, uint8_t* has, hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::gfm_star_kernel<1>, grid, dim3(kBlock), 0, st, phi_m, phi_o,
+                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has);
+  else
+    hipLaunchKernelGGL(ls::gfm_star_kernel<2>, grid, dim3(kBlock), 0, st, phi_m, phi_o,
+                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+template <int ND>
+void extrapolate_nd(const double* phi_m, ls::StateView um, const double* star,
+                    const uint8_t* has, const ls::Grid& g, double dh, int sweeps, double* work,
+                    hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
+  const dim3 grid = grid_for(cells);
+  double* buf[2] = {work, work + cells * (2 + ND)};
+  const double upper = 6.0 * dh * (1.0 / sqrt(2.0));
+  hipLaunchKernelGGL(ls::gfm_seed_kernel<ND>, grid, dim3(kBlock), 0, st, um, star, has, g,
+                     buf[0], buf[1]);
+  for (int k = 0; k < sweeps; ++k)
+    hipLaunchKernelGGL(ls::gfm_extrap_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m,
+                       (const double*)buf[k & 1], g, dh, upper, buf[(k + 1) & 1]);
+  hipLaunchKernelGGL(ls::gfm_writeback_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m,
+                     (const double*)buf[sweeps & 1], g, upper, um);
+}
+
+int launch_extrapolate(const double* phi_m, const fl_field* sm, const double* star,
+                       const uint8_t* has, const ls::Grid& g, double dh, int sweeps,
+                       double* work, hipStream_t st) {
+  if (g.nd == 1)
+    extrapolate_nd<1>(phi_m, view_of(sm), star, has, g, dh, sweeps, work, st);
+  else
+    extrapolate_nd<2>(phi_m, view_of(sm), star, has, g, dh, sweeps, work, st);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int launch_combine(const double* phi0, const fl_field* s0, const fl_field* s1,
+                   const fl_field* out, const ls::Grid& g, hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::combine_kernel<1>, grid, dim3(kBlock), 0, st, phi0, view_of(s0),
+                       view_of(s1), g, view_of(out));
+  else
+    hipLaunchKernelGGL(ls::combine_kernel<2>, grid, dim3(kBlock), 0, st, phi0, view_of(s0),
+                       view_of(s1), g, view_of(out));
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// Scratch layout of fl_mm_simulate.
+struct MmWork {
+  double* star[2];
+  double* extrap;   // 2 * cells * S
+  double* V[2];
+  double* adv;      // 2 * cells
+  double* tphi[2];
+  void* reinit;
+  uint8_t* has[2];
+  int64_t bytes;
+};
+
+MmWork mm_layout(void* base, const ls::Grid& g) {
+  const int64_t cells = g.nx * g.ny;
+  const int S = 2 + g.nd;
+  char* p = (char*)base;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* r = p ? p + off : nullptr;
+    off += align256(bytes);
+    return r;
+  };
+  MmWork w;
+  w.star[0] = (double*)take(cells * S * 8);
+  w.star[1] = (double*)take(cells * S * 8);
+  w.extrap = (double*)take(2 * cells * S * 8);
+  w.V[0] = (double*)take(cells * 8);
+  w.V[1] = (double*)take(cells * 8);
+  w.adv = (double*)take(2 * cells * 8);
+  w.tphi[0] = (double*)take(cells * 8);
+  w.tphi[1] = (double*)take(cells * 8);
+  w.reinit = take(reinit_work_bytes(g));
+  w.has[0] = (uint8_t*)take(cells);
+  w.has[1] = (uint8_t*)take(cells);
+  w.bytes = off;
+  return w;
+}
+
+bool aligned(const void* p) { return ((uintptr_t)p & 255) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int fl_ls_masks(const double* phi, int ndim, const int64_t* n, double dh, uint8_t* masks,
+                void* stream) {
+  ls::Grid g;
+  int rc = check_shape(ndim, n, &g);
+  if (rc) return rc;
+  if (!phi || !masks) return fail(FL_ERR_ARGUMENT, "phi or masks is NULL");
+  if ((rc = check_dh(dh))) return rc;
+  return launch_masks(phi, g, dh, masks, (hipStream_t)stream);
+}
+
+int fl_ls_velocity(int nmat, const double* const* phis, const fl_field* states, double dh,
+                   int target, double* V, void* stream) {
+  if (nmat < 1 || nmat > 2) return fail(FL_ERR_UNSUPPORTED, "nmat %d: 1 or 2 materials", nmat);
+  if (!phis || !states || !V) return fail(FL_ERR_ARGUMENT, "phis, states or V is NULL");
+  if (target >= nmat) return fail(FL_ERR_ARGUMENT, "target %d >= nmat %d", target, nmat);
+  ls::Grid g;
+  int rc = check_shape(states[0].ndim, states[0].n, &g);
+  if (rc) return rc;
+  if ((rc = check_dh(dh))) return rc;
+  for (int m = 0; m < nmat; ++m) {
+    if (!phis[m]) return fail(FL_ERR_ARGUMENT, "phis[%d] is NULL", m);
+    if ((rc = check_state(&states[m], g, "states[m]"))) return rc;
+  }
+  return launch_velocity(nmat, phis, states, g, dh, target, V, (hipStream_t)stream);
+}
+
+int fl_ls_advect(const double* phi, const double* V, int ndim, const int64_t* n, double dt,
+                 double dh, double* out, double* work, void* stream) {
+  ls::Grid g;
+  int rc = check_shape(ndim, n, &g);
+  if (rc) return rc;
+  if (!phi || !V || !out || !work) return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  if ((rc = check_dh(dh))) return rc;
+  if (!(dt >= 0.0) || !isfinite(dt)) return fail(FL_ERR_ARGUMENT, "dt %g must be >= 0", dt);
+  if (out == phi || out == work) return fail(FL_ERR_ARGUMENT, "out aliases phi or work");
+  return launch_advect(phi, V, g, dt, dh, out, work, (hipStream_t)stream);
+}
+
+int fl_ls_project(double* phi0, double* phi1, int64_t cells, void* stream) {
+  if (!phi0 || !phi1) return fail(FL_ERR_ARGUMENT, "phi0 or phi1 is NULL");
+  if (phi0 == phi1) return fail(FL_ERR_ARGUMENT, "phi0 and phi1 alias");
+  if (cells < 0) return fail(FL_ERR_ARGUMENT, "cells < 0");
+  if (cells == 0) return FL_OK;
+  return launch_project(phi0, phi1, cells, (hipStream_t)stream);
+}
+
+int64_t fl_ls_reinit_work_bytes(int ndim, const int64_t* n) {
+  ls::Grid g;
+  if (check_shape(ndim, n, &g)) return -1;
+  return reinit_work_bytes(g);
+}
+
+int fl_ls_reinit(fl_context* c, const double* phi, int ndim, const int64_t* n, double dh,
+                 int band, double* out, uint8_t* band_mask, void* work, int64_t* iterations,
+                 void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  ls::Grid g;
+  int rc = check_shape(ndim, n, &g);
+  if (rc) return rc;
+  if (!phi || !out || !work) return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  if ((rc = check_dh(dh))) return rc;
+  if (band < 1 || band > (1 << 20)) return fail(FL_ERR_ARGUMENT, "band %d out of range", band);
+  if (out == phi) return fail(FL_ERR_ARGUMENT, "out aliases phi");
+  if (!aligned(work)) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
+  DeviceGuard dg(c->device);
+  return run_reinit(c, phi, g, dh, band, out, band_mask, work, iterations, (hipStream_t)stream);
+}
+
+int fl_gfm_star(const double* phi_m, const double* phi_o, const fl_field* state_m,
+                const fl_field* state_o, const fl_scheme* eos_m, const fl_scheme* eos_o,
+                double dh, double* star, uint8_t* has_star, void* stream) {
+  if (!phi_m || !phi_o || !star || !has_star || !state_m)
+    return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  ls::Grid g;
+  int rc = check_shape(state_m->ndim, state_m->n, &g);
+  if (rc) return rc;
+  if ((rc = check_state(state_m, g, "state_m"))) return rc;
+  if ((rc = check_state(state_o, g, "state_o"))) return rc;
+  if ((rc = check_scheme(eos_m, g.nd))) return rc;
+  if ((rc = check_scheme(eos_o, g.nd))) return rc;
+  if ((rc = check_dh(dh))) return rc;
+  return launch_star(phi_m, phi_o, state_m, state_o, eos_m, eos_o, g, dh, star, has_star,
+                     (hipStream_t)stream);
+}
+
+int fl_gfm_extrapolate(const double* phi_m, fl_field* state_m, const double* star,
+                       const uint8_t* has_star, double dh, int sweeps, double* work,
+                       void* stream) {
+  if (!phi_m || !state_m || !star || !has_star || !work)
+    return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  ls::Grid g;
+  int rc = check_shape(state_m->ndim, state_m->n, &g);
+  if (rc) return rc;
+  if ((rc = check_state(state_m, g, "state_m"))) return rc;
+  if ((rc = check_dh(dh))) return rc;
+  if (sweeps < 0 || sweeps > 4096) return fail(FL_ERR_ARGUMENT, "sweeps %d out of range", sweeps);
+  return launch_extrapolate(phi_m, state_m, star, has_star, g, dh, sweeps, work,
+                            (hipStream_t)stream);
+}
+
+int fl_mm_combine(const double* phi0, const fl_field* state0, const fl_field* state1,
+                  fl_field* out, void* stream) {
+  if (!phi0 || !state0) return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  ls::Grid g;
+  int rc = check_shape(state0->ndim, state0->n, &g);
+  if (rc) return rc;
+  if ((rc = check_state(state0, g, "state0"))) return rc;
+  if ((rc = check_state(state1, g, "state1"))) return rc;
+  if ((rc = check_state(out, g, "out"))) return rc;
+  return launch_combine(phi0, state0, state1, out, g, (hipStream_t)stream);
+}
+
+int64_t fl_mm_work_bytes(int ndim, const int64_t* n) {
+  ls::Grid g;
+  if (check_shape(ndim, n, &g)) return -1;
+  return mm_layout(nullptr, g).bytes;
+}
+
+int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_params* p,
+                   const fl_mm_params* mp, void* work, int64_t work_bytes,
+                   fl_sim_result* result, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  if (!mats || !p || !work) return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
+  if (nmat != 2) return fail(FL_ERR_UNSUPPORTED, "nmat %d: the projection needs 2 materials", nmat);
+  ls::Grid g;
+  int rc = check_shape(mats[0].state.ndim, mats[0].state.n, &g);
+  if (rc) return rc;
+  for (int m = 0; m < 2; ++m) {
+    if ((rc = check_state(&mats[m].state, g, "mats[m].state"))) return rc;
+    if ((rc = check_state(&mats[m].work, g, "mats[m].work"))) return rc;
+    if (mats[m].state.layout != mats[m].work.layout || mats[m].state.data == mats[m].work.data)
+      return fail(FL_ERR_ARGUMENT, "mats[%d]: work must be a distinct buffer of the same layout", m);
+    if (!mats[m].phi) return fail(FL_ERR_ARGUMENT, "mats[%d].phi is NULL", m);
+    if ((rc = check_scheme(&mats[m].scheme, g.nd))) return rc;
+  }
+  if (!(p->cfl > 0.0) || !isfinite(p->cfl)) return fail(FL_ERR_ARGUMENT, "cfl must be > 0");
+  if ((rc = check_dh(p->dh))) return rc;
+  if (p->max_iters < 0) return fail(FL_ERR_ARGUMENT, "max_iters must be >= 0");
+  if (isnan(p->sim_time)) return fail(FL_ERR_ARGUMENT, "sim_time is NaN");
+  const int band = mp && mp->band ? mp->band : 6;
+  const int sweeps = mp && mp->extrap_sweeps ? mp->extrap_sweeps : 8;
+  if (band < 1 || band > (1 << 20) || sweeps < 0 || sweeps > 4096)
+    return fail(FL_ERR_ARGUMENT, "band %d or extrap_sweeps %d out of range", band, sweeps);
+  if (!aligned(work)) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
+  const MmWork w = mm_layout(work, g);
+  if (work_bytes < w.bytes)
+    return fail(FL_ERR_ARGUMENT, "work of %lld bytes, need %lld", (long long)work_bytes,
+                (long long)w.bytes);
+  DeviceGuard dg(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t cells = g.nx * g.ny;
+  const int S = 2 + g.nd;
+  const double dh = p->dh;
+  double t = 0.0, dt = 0.0, maxws = 0.0;
+  int64_t it = 0;
+  while (t < p->sim_time && it < p->max_iters) {
+    // ghost cells: star states of both materials from the same data, then
+    // extrapolation into each material's ghost band
+    for (int m = 0; m < 2; ++m) {
+      const int o = 1 - m;
+      if ((rc = launch_star(mats[m].phi, mats[o].phi, &mats[m].state, &mats[o].state,
+                            &mats[m].scheme, &mats[o].scheme, g, dh, w.star[m], w.has[m], st)))
+        return rc;
+    }
+    for (int m = 0; m < 2; ++m)
+      if ((rc = launch_extrapolate(mats[m].phi, &mats[m].state, w.star[m], w.has[m], g, dh,
+                                   sweeps, w.extrap, st)))
+        return rc;
+    // dt from the maximum wavespeed over both grids
+    hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+    for (int m = 0; m < 2; ++m)
+      if ((rc = launch_max_ws(&mats[m].state, &mats[m].scheme, c->ws_tmp, st))) return rc;
+    FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    maxws = bits_to_double(c->host_ctl->ws_bits[0]);
+    if (!(maxws > 0.0) || isnan(maxws) || isinf(maxws)) {
+      if (result) {
+        result->run_time = t;
+        result->iters = it;
+        result->last_dt = dt;
+        result->max_wavespeed = maxws;
+      }
+      return fail(FL_ERR_NUMERIC, "max wavespeed %g at step %lld (NaN or vacuum)", maxws,
+                  (long long)it);
+    }
+    dt = p->cfl * dh / maxws;
+    const double dtdh = dt / dh;
+    // each material's split solve
+    for (int m = 0; m < 2; ++m) {
+      fl_field* u = &mats[m].state;
+      fl_field* v = &mats[m].work;
+      if ((rc = launch_sweep(c, u, v, 0, &mats[m].scheme, dtdh, nullptr, nullptr, st))) return rc;
+      if (g.nd > 1) {
+        if ((rc = launch_sweep(c, v, u, 1, &mats[m].scheme, dtdh, nullptr, nullptr, st)))
+          return rc;
+      } else {
+        FL_HIP(hipMemcpyAsync(u->data, v->data, (size_t)(cells * S * 8), hipMemcpyDeviceToDevice,
+                              st));
+      }
+    }
+    // level sets: speeds from the old level sets and new states, advection,
+    // projection, reinitialisation back into each material's phi
+    const double* phis[2] = {mats[0].phi, mats[1].phi};
+    const fl_field states[2] = {mats[0].state, mats[1].state};
+    for (int m = 0; m < 2; ++m)
+      if ((rc = launch_velocity(2, phis, states, g, dh, m, w.V[m], st))) return rc;
+    for (int m = 0; m < 2; ++m)
+      if ((rc = launch_advect(mats[m].phi, w.V[m], g, dt, dh, w.tphi[m], w.adv, st))) return rc;
+    if ((rc = launch_project(w.tphi[0], w.tphi[1], cells, st))) return rc;
+    for (int m = 0; m < 2; ++m)
+      if ((rc = run_reinit(c, w.tphi[m], g, dh, band, mats[m].phi, nullptr, w.reinit, nullptr,
+                           st)))
+        return rc;
+    t += dt;
+    it += 1;
+  }
+  FL_HIP(hipStreamSynchronize(st));
+  if (result) {
+    result->run_time = t;
+    result->iters = it;
+    result->last_dt = dt;
+    result->max_wavespeed = maxws;
+  }
+  return FL_OK;
+}
+
+}  // extern "C"

@@ -28,7 +28,13 @@ EXPORTED = [
     "fl_status_string", "fl_last_error", "fl_version", "fl_ctx_create", "fl_ctx_destroy",
     "fl_ctx_set_tuning", "fl_ctx_get_tuning", "fl_split_sweep", "fl_split_solve",
     "fl_max_wavespeed", "fl_simulate", "fl_conservation_sums",
+    "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
+    "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate",
+    "fl_mm_combine", "fl_mm_work_bytes", "fl_mm_simulate",
 ]
+
+# entry points returning a size rather than a status
+_SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes")
 
 
 class Field(C.Structure):
@@ -82,6 +88,22 @@ class Tuning(C.Structure):
     ]
 
 
+class Material(C.Structure):
+    _fields_ = [
+        ("state", Field),
+        ("work", Field),
+        ("phi", C.c_void_p),
+        ("scheme", Scheme),
+    ]
+
+
+class MmParams(C.Structure):
+    _fields_ = [
+        ("band", C.c_int32),
+        ("extrap_sweeps", C.c_int32),
+    ]
+
+
 class FluidityError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{message} (status {status})")
@@ -112,8 +134,23 @@ def _load():
     L.fl_max_wavespeed.argtypes = [vp, fp, sp, C.POINTER(C.c_double), vp]
     L.fl_simulate.argtypes = [vp, fp, fp, sp, C.POINTER(SimParams), C.POINTER(SimResult), vp]
     L.fl_conservation_sums.argtypes = [vp, fp, C.POINTER(C.c_double), vp]
+    i64p = C.POINTER(C.c_int64)
+    L.fl_ls_masks.argtypes = [vp, C.c_int, i64p, C.c_double, vp, vp]
+    L.fl_ls_velocity.argtypes = [C.c_int, C.POINTER(vp), fp, C.c_double, C.c_int, vp, vp]
+    L.fl_ls_advect.argtypes = [vp, vp, C.c_int, i64p, C.c_double, C.c_double, vp, vp, vp]
+    L.fl_ls_project.argtypes = [vp, vp, C.c_int64, vp]
+    L.fl_ls_reinit_work_bytes.argtypes = [C.c_int, i64p]
+    L.fl_ls_reinit.argtypes = [vp, vp, C.c_int, i64p, C.c_double, C.c_int, vp, vp, vp, i64p, vp]
+    L.fl_gfm_star.argtypes = [vp, vp, fp, fp, sp, sp, C.c_double, vp, vp, vp]
+    L.fl_gfm_extrapolate.argtypes = [vp, fp, vp, vp, C.c_double, C.c_int, vp, vp]
+    L.fl_mm_combine.argtypes = [vp, fp, fp, fp, vp]
+    L.fl_mm_work_bytes.argtypes = [C.c_int, i64p]
+    L.fl_mm_simulate.argtypes = [vp, C.POINTER(Material), C.c_int, C.POINTER(SimParams),
+                                 C.POINTER(MmParams), vp, C.c_int64, C.POINTER(SimResult), vp]
     for name in EXPORTED:
-        if name not in ("fl_status_string", "fl_last_error", "fl_version"):
+        if name in _SIZE_RETURNING:
+            getattr(L, name).restype = C.c_int64
+        elif name not in ("fl_status_string", "fl_last_error", "fl_version"):
             getattr(L, name).restype = C.c_int
     return L
 

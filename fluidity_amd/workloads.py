@@ -252,6 +252,95 @@ def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D
     return Riemann2D(np.ascontiguousarray(U), q, (x0, y0), gamma, 1.0 / n, rejected)
 
 
+@dataclass
+class ShockBubble:
+    U0: np.ndarray       # (ny, nx, 4) material 0 (ambient gas) over the whole grid
+    U1: np.ndarray       # (ny, nx, 4) material 1 (bubble) over the whole grid
+    phi0: np.ndarray     # (ny, nx) signed distance, <= 0 in material 0
+    phi1: np.ndarray     # -phi0
+    gamma0: float
+    gamma1: float
+    p_inf1: float
+    stiff: bool
+    mach: float
+    radius: float
+    centre: tuple
+    shock_x: float
+    dh: float
+
+
+def prim_to_cons_eos(W: np.ndarray, gamma: float, p_inf: float = 0.0,
+                     stiff: bool = False) -> np.ndarray:
+    """prim->cons with the stiffened-gas energy (p + gamma p_inf) / ((gamma-1) rho)
+    when stiff (material/stiff_fluid.hpp:77-98), else the ideal gas one."""
+    if not stiff:
+        return prim_to_cons(W, gamma)
+    nd = W.shape[-1] - 2
+    rho, pr = W[..., 0], W[..., 1]
+    vss = np.zeros_like(rho)
+    for i in range(nd):
+        vss = vss + W[..., 2 + i] * W[..., 2 + i]
+    eos = (pr + gamma * p_inf) / ((gamma - 1.0) * rho)
+    U = np.empty_like(W)
+    U[..., 0] = rho
+    U[..., 1] = rho * (0.5 * vss + eos)
+    for i in range(nd):
+        U[..., 2 + i] = W[..., 2 + i] * rho
+    return U
+
+
+def shock_bubble(nx: int = 2048, ny: int = 1024, seed: int = 1002,
+                 stiff: bool = False) -> ShockBubble:
+    """A planar shock in gas (gamma 1.4, rho 1, p 1 ahead of it) running in +x
+    onto a bubble of a second material on [0, nx dh] x [0, 1], dh = 1/ny.
+    Radius uniform in [0.1, 0.25], centre x in [0.5, 1.5], y in [r, 1 - r],
+    shock Mach number uniform in [1.2, 3] with the post-shock state from the
+    Rankine-Hugoniot relations, shock 0.05 ahead of the bubble.  Bubble:
+    ideal gas gamma 1.67 at density ratio 0.14 in pressure equilibrium, or
+    (stiff) a stiffened gas gamma 4.4, p_inf 6e8 at density 1000 in units
+    where the ambient pressure is 1e5.  phi is the exact signed distance to the
+    circle; each material's state is its own phase over the whole grid."""
+    rng = SplitMix64(seed)
+    r = rng.uniform(0.1, 0.25)
+    cx = rng.uniform(0.5, 1.5)
+    cy = rng.uniform(r, 1.0 - r)
+    mach = rng.uniform(1.2, 3.0)
+    g0 = 1.4
+    rho_a, p_a = (1.0, 1.0) if not stiff else (1.0, 1e5)
+    a_a = math.sqrt(g0 * p_a / rho_a)
+    m2 = mach * mach
+    rho_s = rho_a * (g0 + 1.0) * m2 / ((g0 - 1.0) * m2 + 2.0)
+    p_s = p_a * (2.0 * g0 * m2 - (g0 - 1.0)) / (g0 + 1.0)
+    u_s = mach * a_a * (1.0 - rho_a / rho_s)
+    xs = cx - r - 0.05
+    dh = 1.0 / ny
+    x = (np.arange(nx) + 0.5) * dh
+    y = (np.arange(ny) + 0.5) * dh
+    X, Y = np.meshgrid(x, y)
+    W0 = np.empty((ny, nx, 4))
+    post = X < xs
+    W0[..., 0] = np.where(post, rho_s, rho_a)
+    W0[..., 1] = np.where(post, p_s, p_a)
+    W0[..., 2] = np.where(post, u_s, 0.0)
+    W0[..., 3] = 0.0
+    if stiff:
+        g1, pinf1, rho_b = 4.4, 6e8, 1000.0
+    else:
+        g1, pinf1, rho_b = 1.67, 0.0, 0.14 * rho_a
+    W1 = np.empty((ny, nx, 4))
+    W1[..., 0] = rho_b
+    W1[..., 1] = p_a
+    W1[..., 2] = 0.0
+    W1[..., 3] = 0.0
+    phi1 = np.hypot(X - cx, Y - cy) - r
+    phi0 = -phi1
+    U0 = prim_to_cons(W0, g0)
+    U1 = prim_to_cons_eos(W1, g1, pinf1, stiff)
+    return ShockBubble(np.ascontiguousarray(U0), np.ascontiguousarray(U1),
+                       np.ascontiguousarray(phi0), np.ascontiguousarray(phi1), g0, g1, pinf1,
+                       stiff, mach, r, (cx, cy), xs, dh)
+
+
 def aos_to_soa(U: np.ndarray) -> np.ndarray:
     """(.., S) AoS -> (S, ..) SoA planes."""
     return np.ascontiguousarray(np.moveaxis(U, -1, 0))

@@ -144,6 +144,107 @@ int fl_simulate(fl_context* ctx, fl_field* state, fl_field* work, const fl_schem
  * FP32 bulk path. */
 int fl_conservation_sums(fl_context* ctx, const fl_field* in, double* sums, void* stream);
 
+/* ---- level set and Riemann ghost fluid -------------------------------------
+ * Level-set fields phi are FP64 device arrays of n[0] (1D) or n[0]*n[1] (2D)
+ * values, x fastest, interior cells only; phi <= 0 inside the material.  State
+ * fields given with them are FP64 (AoS or SoA) of the same shape.  `work`
+ * arguments are caller-owned device scratch of the stated size, 256-byte
+ * aligned, not aliasing any other argument. */
+
+/* Interface masks, one byte per cell: 1 inside (phi <= 0), 2 outside
+ * (phi > 0), 4 on the boundary (phi == 0), 8 inside-interfacial
+ * (inside && phi >= -dh), 16 outside-interfacial (outside && phi < |dh|).
+ * Replaces the predicates of levelset/levelset.hpp:166-217. */
+int fl_ls_masks(const double* phi, int ndim, const int64_t* n, double dh, uint8_t* masks,
+                void* stream);
+
+/* Level-set speed V = u . n, u the velocity of the first material whose
+ * phi <= 0 (V = 0 where none is), n the normalised central-difference
+ * gradient: of that material's phi when target < 0 (the reference's field),
+ * of phis[target] otherwise (the field level set `target` is advected with).
+ * phis and states are host arrays of nmat (1 or 2) entries.  Replaces
+ * set_velocities (levelset/cuda/velocity_initialization.cuh:108-143). */
+int fl_ls_velocity(int nmat, const double* const* phis, const fl_field* states, double dh,
+                   int target, double* V, void* stream);
+
+/* One SSP-RK3 step of phi_t + V |grad phi| = 0 with WENO-HJ5 one-sided
+ * derivatives and Godunov upwinding; ghost cells mirror the interior.
+ * work: 2 * cells doubles.  Replaces scheme::evolve with the level-set
+ * evolver (scheme/cuda/evolve.cuh:51-157, scheme/updaters/runge_kutta_3.hpp,
+ * scheme/stencils/weno_hj_5.hpp, scheme/schemes/godunov_upwind.hpp,
+ * scheme/evaluators/levelset_hamiltonian.hpp). */
+int fl_ls_advect(const double* phi, const double* V, int ndim, const int64_t* n, double dt,
+                 double dh, double* out, double* work, void* stream);
+
+/* Two-material projection: avg = (phi0 + phi1) / 2; phi0 -= avg;
+ * phi1 = -phi0.  Replaces levelset::project (levelset/cuda/levelset_projection.cuh:54-66). */
+int fl_ls_project(double* phi0, double* phi1, int64_t cells, void* stream);
+
+/* Reinitialises phi to signed distance within `band` cells of the
+ * interface: sources beside each sign change keep |phi|, every other cell
+ * solves the Godunov eikonal update by Jacobi sweeps to the fixed point;
+ * cells not reached get +-(band + 1) dh.  band_mask (may be NULL) marks the
+ * cells reached; *iterations (may be NULL) receives the sweep count.
+ * work: fl_ls_reinit_work_bytes() bytes.  Synchronises `stream` once per few
+ * sweeps.  Replaces solver::eikonal with the fast iterative method
+ * (scheme/cuda/fast_iterative_method.cuh:48-461,
+ * scheme/schemes/godunov_upwind_scheme.hpp:37-79). */
+int64_t fl_ls_reinit_work_bytes(int ndim, const int64_t* n);
+int fl_ls_reinit(fl_context* ctx, const double* phi, int ndim, const int64_t* n, double dh,
+                 int band, double* out, uint8_t* band_mask, void* work, int64_t* iterations,
+                 void* stream);
+
+/* Riemann ghost-fluid star states of material m against material o: at each
+ * inside-interfacial cell of m not beside a phi_m == 0 cell whose phi_o is
+ * outside-interfacial or zero, the states interpolated at -+1.5 dh along the
+ * normal are solved with HLLC in the normal frame; star (cells * (2+ndim)
+ * doubles, AoS) receives the left star state with the tangential velocity
+ * of m, has_star marks those cells.  eos_m/eos_o give each side's material.
+ * Replaces RiemannGFM::invoke (ghost_fluid/riemann_ghost_fluid.hpp:50-195). */
+int fl_gfm_star(const double* phi_m, const double* phi_o, const fl_field* state_m,
+                const fl_field* state_o, const fl_scheme* eos_m, const fl_scheme* eos_o,
+                double dh, double* star, uint8_t* has_star, void* stream);
+
+/* Fills the ghost band 0 < phi_m <= 6 dh / sqrt(2) of state_m by `sweeps`
+ * Jacobi sweeps of the normal-weighted upwind average, seeded with the star
+ * states.  work: 2 * cells * (2+ndim) doubles.  Replaces
+ * GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202). */
+int fl_gfm_extrapolate(const double* phi_m, fl_field* state_m, const double* star,
+                       const uint8_t* has_star, double dh, int sweeps, double* work,
+                       void* stream);
+
+/* out = state0 where phi0 <= 0, else state1.  Replaces combine_into_first
+ * (material/cuda/combine_materials.cuh:39-63). */
+int fl_mm_combine(const double* phi0, const fl_field* state0, const fl_field* state1,
+                  fl_field* out, void* stream);
+
+/* One material of a ghost-fluid run: its states over the whole grid, a
+ * same-shaped buffer, its level set and its scheme (eos, gamma, p_inf and
+ * the solver options). */
+typedef struct fl_material {
+  fl_field  state;
+  fl_field  work;
+  double*   phi;
+  fl_scheme scheme;
+} fl_material;
+
+typedef struct fl_mm_params {
+  int32_t band;           /* eikonal band in cells (default 6)            */
+  int32_t extrap_sweeps;  /* ghost extrapolation sweeps (default 8)       */
+} fl_mm_params;
+
+/* The two-material loop of MultimaterialSimulator::simulate_mm
+ * (simulator/multimaterial_simulator.hpp:227-303, 367-483).  Per step: star
+ * states of both materials, then extrapolation of each; dt = cfl dh / max
+ * wavespeed over both materials' grids; each material's split solve; level
+ * set m advected with u . n_m; projection; reinitialisation.  Stops when
+ * run_time >= sim_time or iters >= max_iters.  work: fl_mm_work_bytes().
+ * Synchronises `stream` every step.  mp may be NULL (defaults). */
+int64_t fl_mm_work_bytes(int ndim, const int64_t* n);
+int fl_mm_simulate(fl_context* ctx, fl_material* mats, int nmat, const fl_sim_params* params,
+                   const fl_mm_params* mp, void* work, int64_t work_bytes,
+                   fl_sim_result* result, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,31 @@ int    orc_simulate(double* U, double* work, const int64_t* n, int nd,
                     double sim_time, int64_t max_iters, int nthreads,
                     double* run_time, int64_t* iters, double* dt_hist);
 
+/* ---- level set / ghost fluid (fl_oracle_ls.c) ---- */
+/* mask bits: 1 inside, 2 outside, 4 on boundary, 8 inside interfacial,
+ * 16 outside interfacial (levelset/levelset.hpp:166-217). */
+uint8_t orc_ls_mask_value(double phi, double dh);
+void orc_ls_masks(const double* phi, const int64_t* n, int nd, double dh, uint8_t* masks);
+void orc_ls_velocity(int nmat, const double* const* phis, const double* const* states,
+                     const int64_t* n, int nd, double dh, int target, double* V);
+void orc_ls_advect(const double* phi, const double* V, const int64_t* n, int nd, double dt,
+                   double dh, double* out);
+void orc_ls_project2(double* phi0, double* phi1, int64_t count);
+int  orc_ls_reinit(const double* phi, const int64_t* n, int nd, double dh, int band,
+                   double* out, uint8_t* band_mask, int64_t* iterations);
+void orc_gfm_star(const double* phi_m, const double* phi_o, const double* U_m,
+                  const double* U_o, const orc_scheme* eos_m, const orc_scheme* eos_o,
+                  const int64_t* n, int nd, double dh, double* star, uint8_t* has_star);
+void orc_gfm_extrapolate(const double* phi_m, double* U_m, const double* star,
+                         const uint8_t* has_star, const int64_t* n, int nd, double dh,
+                         int sweeps);
+void orc_mm_combine(const double* phi0, const double* U0, const double* U1, const int64_t* n,
+                    int nd, double* out);
+int  orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const int64_t* n,
+                     int nd, const orc_scheme* s0, const orc_scheme* s1, double cfl, double dh,
+                     double sim_time, int64_t max_iters, int band, int extrap_sweeps,
+                     int nthreads, double* run_time, int64_t* iters, double* dt_hist);
+
 #ifdef __cplusplus
 }
 #endif

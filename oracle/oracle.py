@@ -95,6 +95,30 @@ def lib():
         L.orc_simulate.argtypes = [dp, dp, i64p, C.c_int, sp, C.c_double, C.c_double,
                                    C.c_double, C.c_int64, C.c_int, dp, i64p, dp]
         L.orc_simulate.restype = C.c_int
+        u8p = C.POINTER(C.c_uint8)
+        L.orc_ls_mask_value.argtypes = [C.c_double, C.c_double]
+        L.orc_ls_mask_value.restype = C.c_uint8
+        L.orc_ls_masks.argtypes = [dp, i64p, C.c_int, C.c_double, u8p]
+        L.orc_ls_masks.restype = None
+        L.orc_ls_velocity.argtypes = [C.c_int, C.POINTER(dp), C.POINTER(dp), i64p, C.c_int,
+                                      C.c_double, C.c_int, dp]
+        L.orc_ls_velocity.restype = None
+        L.orc_ls_advect.argtypes = [dp, dp, i64p, C.c_int, C.c_double, C.c_double, dp]
+        L.orc_ls_advect.restype = None
+        L.orc_ls_project2.argtypes = [dp, dp, C.c_int64]
+        L.orc_ls_project2.restype = None
+        L.orc_ls_reinit.argtypes = [dp, i64p, C.c_int, C.c_double, C.c_int, dp, u8p, i64p]
+        L.orc_ls_reinit.restype = C.c_int
+        L.orc_gfm_star.argtypes = [dp, dp, dp, dp, sp, sp, i64p, C.c_int, C.c_double, dp, u8p]
+        L.orc_gfm_star.restype = None
+        L.orc_gfm_extrapolate.argtypes = [dp, dp, dp, u8p, i64p, C.c_int, C.c_double, C.c_int]
+        L.orc_gfm_extrapolate.restype = None
+        L.orc_mm_combine.argtypes = [dp, dp, dp, i64p, C.c_int, dp]
+        L.orc_mm_combine.restype = None
+        L.orc_mm_simulate.argtypes = [dp, dp, dp, dp, i64p, C.c_int, sp, sp, C.c_double,
+                                      C.c_double, C.c_double, C.c_int64, C.c_int, C.c_int,
+                                      C.c_int, dp, i64p, dp]
+        L.orc_mm_simulate.restype = C.c_int
         _lib = L
     return _lib
 
@@ -239,3 +263,122 @@ def simulate(U, s, cfl, dh, sim_time=float("inf"), max_iters=1 << 62, nthreads=0
     if dts is not None:
         dts = dts[: it.value]
     return U, rt.value, it.value, dts
+
+
+# ---- level set / ghost fluid ---------------------------------------------------
+# Level-set fields are float64 arrays of shape (ny, nx) in 2D, (nx,) in 1D.
+
+def _u8p(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _ls_dims(phi: np.ndarray):
+    if phi.ndim == 1:
+        return 1, np.array([phi.shape[0], 1], dtype=np.int64)
+    return 2, np.array([phi.shape[1], phi.shape[0]], dtype=np.int64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ls_mask_value(phi, dh):
+    return int(lib().orc_ls_mask_value(phi, dh))
+
+
+def ls_masks(phi, dh):
+    phi = _f64(phi)
+    nd, n = _ls_dims(phi)
+    m = np.zeros(phi.shape, dtype=np.uint8)
+    lib().orc_ls_masks(_dp(phi), _n_ptr(n), nd, dh, _u8p(m))
+    return m
+
+
+def ls_velocity(phis, states, dh, target=-1):
+    phis = [_f64(p) for p in phis]
+    states = [_f64(u) for u in states]
+    nd, n = _ls_dims(phis[0])
+    dp = C.POINTER(C.c_double)
+    pa = (dp * len(phis))(*[_dp(p) for p in phis])
+    ua = (dp * len(states))(*[_dp(u) for u in states])
+    V = np.zeros(phis[0].shape)
+    lib().orc_ls_velocity(len(phis), pa, ua, _n_ptr(n), nd, dh, target, _dp(V))
+    return V
+
+
+def ls_advect(phi, V, dt, dh):
+    phi, V = _f64(phi), _f64(V)
+    nd, n = _ls_dims(phi)
+    out = np.empty_like(phi)
+    lib().orc_ls_advect(_dp(phi), _dp(V), _n_ptr(n), nd, dt, dh, _dp(out))
+    return out
+
+
+def ls_project2(phi0, phi1):
+    a = np.array(phi0, dtype=np.float64, order="C")
+    b = np.array(phi1, dtype=np.float64, order="C")
+    lib().orc_ls_project2(_dp(a), _dp(b), a.size)
+    return a, b
+
+
+def ls_reinit(phi, dh, band):
+    """Returns (phi_out, band_mask, iterations)."""
+    phi = _f64(phi)
+    nd, n = _ls_dims(phi)
+    out = np.empty_like(phi)
+    bm = np.zeros(phi.shape, dtype=np.uint8)
+    it = C.c_int64(0)
+    rc = lib().orc_ls_reinit(_dp(phi), _n_ptr(n), nd, dh, band, _dp(out), _u8p(bm), C.byref(it))
+    if rc:
+        raise RuntimeError(f"orc_ls_reinit failed: {rc}")
+    return out, bm, it.value
+
+
+def gfm_star(phi_m, phi_o, U_m, U_o, s_m, s_o, dh):
+    """Returns (star, has_star): star states (AoS, zero where has_star is 0)."""
+    phi_m, phi_o, U_m, U_o = _f64(phi_m), _f64(phi_o), _f64(U_m), _f64(U_o)
+    nd, n = _ls_dims(phi_m)
+    star = np.zeros_like(U_m)
+    has = np.zeros(phi_m.shape, dtype=np.uint8)
+    lib().orc_gfm_star(_dp(phi_m), _dp(phi_o), _dp(U_m), _dp(U_o), C.byref(s_m), C.byref(s_o),
+                       _n_ptr(n), nd, dh, _dp(star), _u8p(has))
+    return star, has
+
+
+def gfm_extrapolate(phi_m, U_m, star, has_star, dh, sweeps):
+    phi_m, star = _f64(phi_m), _f64(star)
+    U = np.array(U_m, dtype=np.float64, order="C")
+    has = np.ascontiguousarray(has_star, dtype=np.uint8)
+    nd, n = _ls_dims(phi_m)
+    lib().orc_gfm_extrapolate(_dp(phi_m), _dp(U), _dp(star), _u8p(has), _n_ptr(n), nd, dh, sweeps)
+    return U
+
+
+def mm_combine(phi0, U0, U1):
+    phi0, U0, U1 = _f64(phi0), _f64(U0), _f64(U1)
+    nd, n = _ls_dims(phi0)
+    out = np.empty_like(U0)
+    lib().orc_mm_combine(_dp(phi0), _dp(U0), _dp(U1), _n_ptr(n), nd, _dp(out))
+    return out
+
+
+def mm_simulate(U0, U1, phi0, phi1, s0, s1, cfl, dh, sim_time=float("inf"), max_iters=1,
+                band=6, extrap_sweeps=8, nthreads=0):
+    """Two-material ghost-fluid run; returns (U0, U1, phi0, phi1, run_time, iters, dts)."""
+    U0 = np.array(U0, dtype=np.float64, order="C")
+    U1 = np.array(U1, dtype=np.float64, order="C")
+    phi0 = np.array(phi0, dtype=np.float64, order="C")
+    phi1 = np.array(phi1, dtype=np.float64, order="C")
+    nd, n = _ls_dims(phi0)
+    if max_iters > 10_000_000:
+        raise ValueError("mm_simulate needs a bounded max_iters")
+    dts = np.zeros(int(max_iters))
+    rt = C.c_double(0.0)
+    it = C.c_int64(0)
+    rc = lib().orc_mm_simulate(_dp(U0), _dp(U1), _dp(phi0), _dp(phi1), _n_ptr(n), nd,
+                               C.byref(s0), C.byref(s1), cfl, dh, sim_time, int(max_iters),
+                               band, extrap_sweeps, nthreads, C.byref(rt), C.byref(it), _dp(dts))
+    if rc:
+        raise RuntimeError(f"orc_mm_simulate failed: {rc}")
+    return U0, U1, phi0, phi1, rt.value, it.value, dts[: it.value]

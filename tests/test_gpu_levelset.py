@@ -1,0 +1,245 @@
+"""Parity of the gfx950 level-set / ghost-fluid path (through the C ABI) with
+the CPU oracle: masks, band masks and sweep counts exactly, FP64 fields bit
+for bit.  Inputs are the seeded shock-bubble workload at reduced and full
+size, and random fields for the single kernels."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fluidity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    from fluidity_amd import levelset
+    return levelset
+
+
+@pytest.fixture(scope="module")
+def E():
+    from fluidity_amd import euler
+    return euler
+
+
+def _dev(a, layout=0):
+    if layout == 1:
+        a = np.moveaxis(a, -1, 0)
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _host(t, layout=0):
+    a = t.cpu().numpy()
+    if layout == 1:
+        a = np.moveaxis(a, 0, -1)
+    return np.ascontiguousarray(a)
+
+
+def _same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    if a.dtype == np.float64:
+        na, nb = np.isnan(a), np.isnan(b)
+        assert np.array_equal(na, nb)
+        bad = ~na & (a.view(np.uint64) != b.view(np.uint64))
+        assert not bad.any(), f"{bad.sum()} of {a.size} differ; first at {np.argwhere(bad)[0]}"
+    else:
+        assert np.array_equal(a, b)
+
+
+def _bubble(nx, ny, stiff=False):
+    return W.shock_bubble(nx, ny, seed=1002, stiff=stiff)
+
+
+def _schemes(O, E, sb):
+    eos1 = 1 if sb.stiff else 0
+    os0 = O.scheme(gamma=sb.gamma0)
+    os1 = O.scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
+    es0 = E.Scheme(gamma=sb.gamma0)
+    es1 = E.Scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
+    return os0, os1, es0, es1
+
+
+def _noisy_phi(shape, seed):
+    rng = np.random.default_rng(seed)
+    ny, nx = shape if len(shape) == 2 else (1, shape[0])
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / max(ny, 1)
+    X, Y = np.meshgrid(x, y)
+    phi = 1.3 * (np.hypot(X - 0.45, Y - 0.55) - 0.27) + 0.01 * rng.normal(size=X.shape)
+    phi[phi.shape[0] // 2, 3] = 0.0                        # exact zeros
+    phi[phi.shape[0] // 3, 5] = -0.0
+    return np.ascontiguousarray(phi.reshape(shape))
+
+
+# ---- single kernels ------------------------------------------------------------
+
+@pytest.mark.parametrize("shape", [(257,), (67, 131)])
+def test_masks(L, oracle, shape):
+    phi = _noisy_phi(shape, 1)
+    dh = 1.0 / shape[-1]
+    phi.flat[7] = dh
+    phi.flat[11] = -dh
+    _same(_host(L.masks(_dev(phi), dh)), oracle.ls_masks(phi, dh))
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("target", [-1, 0, 1])
+def test_velocity(L, oracle, layout, target):
+    sb = _bubble(192, 96)
+    V = L.set_velocities([_dev(sb.phi0), _dev(sb.phi1)],
+                         [_dev(sb.U0, layout), _dev(sb.U1, layout)], sb.dh, target,
+                         layout=layout)
+    _same(_host(V), oracle.ls_velocity([sb.phi0, sb.phi1], [sb.U0, sb.U1], sb.dh, target))
+
+
+@pytest.mark.parametrize("shape", [(300,), (70, 90)])
+def test_advect(L, oracle, shape):
+    phi = _noisy_phi(shape, 2)
+    rng = np.random.default_rng(3)
+    V = rng.uniform(-1.0, 1.0, size=shape)
+    dh = 1.0 / shape[-1]
+    dt = 0.3 * dh
+    out = L.evolve(_dev(phi), _dev(V), dt, dh)
+    _same(_host(out), oracle.ls_advect(phi, V, dt, dh))
+
+
+def test_project(L, oracle):
+    rng = np.random.default_rng(4)
+    a, b = rng.normal(size=(40, 50)), rng.normal(size=(40, 50))
+    ta, tb = _dev(a), _dev(b)
+    L.project(ta, tb)
+    ra, rb = oracle.ls_project2(a, b)
+    _same(_host(ta), ra)
+    _same(_host(tb), rb)
+
+
+@pytest.mark.parametrize("band", [3, 6])
+@pytest.mark.parametrize("shape", [(300,), (80, 96)])
+def test_reinit(L, oracle, shape, band):
+    phi = _noisy_phi(shape, 5)
+    dh = 1.0 / shape[-1]
+    out, bm, it = L.eikonal(_dev(phi), dh, band)
+    ro, rbm, rit = oracle.ls_reinit(phi, dh, band)
+    assert it == rit
+    _same(_host(out), ro)
+    _same(_host(bm), rbm)
+
+
+def test_reinit_band_covers_grid(L, oracle):
+    """A band wider than the grid: every cell reached, many sweeps (several
+    convergence tests on the host)."""
+    phi = _noisy_phi((48, 40), 6)
+    dh = 1.0 / 40
+    out, bm, it = L.eikonal(_dev(phi), dh, 200)
+    ro, rbm, rit = oracle.ls_reinit(phi, dh, 200)
+    assert it == rit and it > 16
+    assert rbm.all()
+    _same(_host(out), ro)
+
+
+@pytest.mark.parametrize("stiff", [False, True])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_gfm_star_and_extrapolate(L, E, oracle, stiff, layout):
+    sb = _bubble(192, 96, stiff)
+    os0, os1, es0, es1 = _schemes(oracle, E, sb)
+    U0, U1 = _dev(sb.U0, layout), _dev(sb.U1, layout)
+    p0, p1 = _dev(sb.phi0), _dev(sb.phi1)
+    for (pm, po, Um, Uo, em, eo, npm, npo, nUm, nUo, om, oo) in [
+            (p0, p1, U0, U1, es0, es1, sb.phi0, sb.phi1, sb.U0, sb.U1, os0, os1),
+            (p1, p0, U1, U0, es1, es0, sb.phi1, sb.phi0, sb.U1, sb.U0, os1, os0)]:
+        star, has = L.riemann_gfm_star(pm, po, Um, Uo, em, eo, sb.dh, layout)
+        rstar, rhas = oracle.gfm_star(npm, npo, nUm, nUo, om, oo, sb.dh)
+        _same(_host(has), rhas)
+        assert rhas.sum() > 50
+        _same(_host(star).reshape(rstar.shape), rstar)
+        L.extrapolate(pm, Um, star, has, sb.dh, 8, layout)
+        ref = oracle.gfm_extrapolate(npm, nUm, rstar, rhas, sb.dh, 8)
+        _same(_host(Um, layout), ref)
+
+
+def test_gfm_1d(L, E, oracle):
+    n = 200
+    dh = 1.0 / n
+    x = (np.arange(n) + 0.5) * dh
+    phi0 = np.ascontiguousarray(x - 0.4137)
+    phi1 = -phi0
+    U0 = W.prim_to_cons(np.broadcast_to(np.array([1.0, 2.0, 0.3]), (n, 3)).copy(), 1.4)
+    U1 = W.prim_to_cons(np.broadcast_to(np.array([0.2, 0.5, -0.2]), (n, 3)).copy(), 1.67)
+    os0, os1 = oracle.scheme(gamma=1.4), oracle.scheme(gamma=1.67)
+    es0, es1 = E.Scheme(gamma=1.4), E.Scheme(gamma=1.67)
+    star, has = L.riemann_gfm_star(_dev(phi0), _dev(phi1), _dev(U0), _dev(U1), es0, es1, dh)
+    rstar, rhas = oracle.gfm_star(phi0, phi1, U0, U1, os0, os1, dh)
+    _same(_host(has), rhas)
+    assert rhas.sum() == 1
+    _same(_host(star).reshape(rstar.shape), rstar)
+
+
+def test_combine(L, oracle):
+    sb = _bubble(160, 80)
+    out = L.combine_into_first(_dev(sb.phi0), _dev(sb.U0), _dev(sb.U1))
+    _same(_host(out), oracle.mm_combine(sb.phi0, sb.U0, sb.U1))
+
+
+# ---- the two-material step ---------------------------------------------------------
+
+def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8):
+    es0, es1 = _schemes_e(E, sb)
+    sim = L.MultimaterialSimulator(layout=layout)
+    U0, U1 = _dev(sb.U0, layout), _dev(sb.U1, layout)
+    p0, p1 = _dev(sb.phi0), _dev(sb.phi1)
+    sim.add_material(U0, p0, es0)
+    sim.add_material(U1, p1, es1)
+    r = sim.simulate_mm(0.5, sb.dh, max_iters=steps, band=band, extrap_sweeps=sweeps)
+    return r, _host(U0, layout), _host(U1, layout), _host(p0), _host(p1)
+
+
+def _schemes_e(E, sb):
+    eos1 = 1 if sb.stiff else 0
+    return E.Scheme(gamma=sb.gamma0), E.Scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
+
+
+@pytest.mark.parametrize("stiff,layout", [(False, 0), (False, 1), (True, 0)])
+def test_mm_simulate_bitwise(L, E, oracle, stiff, layout):
+    sb = _bubble(256, 128, stiff)
+    os0, os1, _, _ = _schemes(oracle, E, sb)
+    steps = 12
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, steps, layout)
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
+                           max_iters=steps)
+    assert r.iters == R[5] == steps
+    assert r.run_time == R[4]
+    assert r.last_dt == R[6][-1]
+    _same(U0, R[0])
+    _same(U1, R[1])
+    _same(p0, R[2])
+    _same(p1, R[3])
+
+
+def test_mm_simulate_full_size(L, E, oracle):
+    """The 2048 x 1024 shock-bubble workload: two steps bit for bit against
+    the oracle, then more steps checked through size-independent properties
+    (finite states, phi1 == -phi0 exactly after projection + reinitialisation,
+    the bubble's area conserved to first order)."""
+    sb = _bubble(2048, 1024)
+    os0, os1, _, _ = _schemes(oracle, E, sb)
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, 2)
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
+                           max_iters=2, nthreads=16)
+    _same(U0, R[0])
+    _same(U1, R[1])
+    _same(p0, R[2])
+    _same(p1, R[3])
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, 20)
+    assert r.iters == 20
+    assert np.isfinite(U0).all() and np.isfinite(U1).all()
+    assert np.array_equal(p1, -p0)
+    area0 = (sb.phi1 <= 0).sum()
+    area = (p1 <= 0).sum()
+    assert abs(area - area0) / area0 < 0.05
