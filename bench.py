@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
     ap.add_argument("--y-rows", type=int, default=0)
-    ap.add_argument("--mode", choices=["bench", "sweeps"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost"], default="bench",
                     help="sweeps: a plain device copy of one grid (HBM calibration for "
                          "counter runs) followed by --steps x sweeps and y sweeps")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
@@ -67,6 +67,11 @@ def main():
     from fluidity_amd import native as N
     from fluidity_amd import workloads as W
 
+    if args.mode == "ghost":
+        stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+        print(json.dumps(ghost_fluid_case(torch, N, W, stream, args.steps, args.warmup)),
+              flush=True)
+        return
     n = args.n
     wl = W.riemann_2d(n=n, seed=1001)
     if args.mode == "sweeps":
@@ -158,6 +163,8 @@ def main():
             del s2, a2, w2
             torch.cuda.empty_cache()
 
+        variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, cfl, args.cpu_steps)
@@ -183,6 +190,40 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024):
+    """Two-material ghost-fluid steps on the seeded shock-bubble workload
+    (ideal-gas bubble, level sets reinitialised every 5 steps in a 6-cell
+    band).  A cell-update is one cell of the domain advanced one full
+    multimaterial step (both materials' sweeps, ghost states, level-set
+    advection, projection and the share of reinitialisation)."""
+    from fluidity_amd import euler as E
+    from fluidity_amd import levelset as L
+    sb = W.shock_bubble(nx, ny, seed=1002)
+    dev = torch.device("cuda:0")
+    sim = L.MultimaterialSimulator(layout=N.FL_LAYOUT_SOA)
+    with torch.cuda.stream(stream):
+        U0 = torch.from_numpy(W.aos_to_soa(sb.U0)).to(dev)
+        U1 = torch.from_numpy(W.aos_to_soa(sb.U1)).to(dev)
+        p0 = torch.from_numpy(sb.phi0).to(dev)
+        p1 = torch.from_numpy(sb.phi1).to(dev)
+    stream.synchronize()
+    sim.add_material(U0, p0, E.Scheme(gamma=sb.gamma0))
+    sim.add_material(U1, p1, E.Scheme(gamma=sb.gamma1))
+    kw = dict(band=6, extrap_sweeps=8, reinit_every=5, stream=stream)
+    sim.simulate_mm(0.5, sb.dh, max_iters=warmup, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = sim.simulate_mm(0.5, sb.dh, max_iters=steps, **kw)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert r.iters == steps
+    cells = nx * ny
+    return {"value": round(steps * cells / el, 1), "unit": "cell-updates/s",
+            "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps,
+            "workload": f"shock-bubble {nx}x{ny} FP64, 2 materials, Mach {sb.mach:.3f}, "
+                        "reinit every 5 steps, band 6, SoA"}
 
 
 def sweeps_mode(args, wl, torch, E, N, W):

@@ -400,8 +400,10 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
   if (isnan(p->sim_time)) return fail(FL_ERR_ARGUMENT, "sim_time is NaN");
   const int band = mp && mp->band ? mp->band : 6;
   const int sweeps = mp && mp->extrap_sweeps ? mp->extrap_sweeps : 8;
-  if (band < 1 || band > (1 << 20) || sweeps < 0 || sweeps > 4096)
-    return fail(FL_ERR_ARGUMENT, "band %d or extrap_sweeps %d out of range", band, sweeps);
+  const int every = mp && mp->reinit_every ? mp->reinit_every : 1;
+  if (band < 1 || band > (1 << 20) || sweeps < 0 || sweeps > 4096 || every < 1)
+    return fail(FL_ERR_ARGUMENT, "band %d, extrap_sweeps %d or reinit_every %d out of range",
+                band, sweeps, every);
   if (!aligned(work)) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
   const MmWork w = mm_layout(work, g);
   if (work_bytes < w.bytes)
@@ -469,10 +471,17 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     for (int m = 0; m < 2; ++m)
       if ((rc = launch_advect(mats[m].phi, w.V[m], g, dt, dh, w.tphi[m], w.adv, st))) return rc;
     if ((rc = launch_project(w.tphi[0], w.tphi[1], cells, st))) return rc;
-    for (int m = 0; m < 2; ++m)
-      if ((rc = run_reinit(c, w.tphi[m], g, dh, band, mats[m].phi, nullptr, w.reinit, nullptr,
-                           st)))
-        return rc;
+    const bool reinit = (it % every) == every - 1;
+    for (int m = 0; m < 2; ++m) {
+      if (reinit) {
+        if ((rc = run_reinit(c, w.tphi[m], g, dh, band, mats[m].phi, nullptr, w.reinit, nullptr,
+                             st)))
+          return rc;
+      } else {
+        FL_HIP(hipMemcpyAsync(mats[m].phi, w.tphi[m], (size_t)(cells * 8),
+                              hipMemcpyDeviceToDevice, st));
+      }
+    }
     t += dt;
     it += 1;
   }

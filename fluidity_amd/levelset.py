@@ -167,7 +167,7 @@ class MultimaterialSimulator:
 
     def simulate_mm(self, cfl: float, dh: float, sim_time: float = float("inf"),
                     max_iters: int = 1, band: int = 6, extrap_sweeps: int = 8,
-                    stream=None) -> N.SimResult:
+                    reinit_every: int = 1, stream=None) -> N.SimResult:
         if len(self.materials) != 2:
             raise ValueError("add two materials first")
         mats = (N.Material * 2)()
@@ -180,7 +180,7 @@ class MultimaterialSimulator:
         wbytes = N.lib.fl_mm_work_bytes(nd, n)
         work = _work(wbytes, self.materials[0][0].device)
         p = N.SimParams(cfl, dh, sim_time, max_iters)
-        mp = N.MmParams(band, extrap_sweeps)
+        mp = N.MmParams(band, extrap_sweeps, reinit_every, 0)
         r = N.SimResult()
         N.check(N.lib.fl_mm_simulate(self.ctx.handle, mats, 2, C.byref(p), C.byref(mp),
                                      work.data_ptr(), wbytes, C.byref(r), _stream_handle(stream)))

@@ -101,6 +101,8 @@ class MmParams(C.Structure):
     _fields_ = [
         ("band", C.c_int32),
         ("extrap_sweeps", C.c_int32),
+        ("reinit_every", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -229,15 +229,19 @@ typedef struct fl_material {
 } fl_material;
 
 typedef struct fl_mm_params {
-  int32_t band;           /* eikonal band in cells (default 6)            */
-  int32_t extrap_sweeps;  /* ghost extrapolation sweeps (default 8)       */
+  int32_t band;           /* eikonal band in cells (default 6)                 */
+  int32_t extrap_sweeps;  /* ghost extrapolation sweeps (default 8)            */
+  int32_t reinit_every;   /* reinitialise after every k-th step (default 1,    */
+                          /* the reference's every step)                       */
+  int32_t reserved;
 } fl_mm_params;
 
 /* The two-material loop of MultimaterialSimulator::simulate_mm
  * (simulator/multimaterial_simulator.hpp:227-303, 367-483).  Per step: star
  * states of both materials, then extrapolation of each; dt = cfl dh / max
  * wavespeed over both materials' grids; each material's split solve; level
- * set m advected with u . n_m; projection; reinitialisation.  Stops when
+ * set m advected with u . n_m; projection; reinitialisation after every
+ * reinit_every-th step (level sets copied back otherwise).  Stops when
  * run_time >= sim_time or iters >= max_iters.  work: fl_mm_work_bytes().
  * Synchronises `stream` every step.  mp may be NULL (defaults). */
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n);

@@ -100,7 +100,8 @@ void orc_mm_combine(const double* phi0, const double* U0, const double* U1, cons
 int  orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const int64_t* n,
                      int nd, const orc_scheme* s0, const orc_scheme* s1, double cfl, double dh,
                      double sim_time, int64_t max_iters, int band, int extrap_sweeps,
-                     int nthreads, double* run_time, int64_t* iters, double* dt_hist);
+                     int reinit_every, int nthreads, double* run_time, int64_t* iters,
+                     double* dt_hist);
 
 #ifdef __cplusplus
 }

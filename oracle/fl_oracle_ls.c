@@ -54,7 +54,8 @@
  *      from the same data before any extrapolation; dt from the maximum
  *      wavespeed over every material's whole grid (update_time_delta); each
  *      material's split solve; speed from the old level sets and new states;
- *      advection of each level set; projection; reinitialisation.
+ *      advection of each level set; projection; reinitialisation after
+ *      every reinit_every-th step (the reference's every step is 1).
  *  L8. Level set m is advected with V_m = u . n_m, u the velocity of the
  *      material present and n_m the normal of level set m.  The reference
  *      advects every level set with u . n of the material present, which
@@ -547,7 +548,8 @@ void orc_mm_combine(const double* phi0, const double* U0, const double* U1, cons
 int orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const int64_t* n,
                     int nd, const orc_scheme* s0, const orc_scheme* s1, double cfl, double dh,
                     double sim_time, int64_t max_iters, int band, int extrap_sweeps,
-                    int nthreads, double* run_time, int64_t* iters, double* dt_hist) {
+                    int reinit_every, int nthreads, double* run_time, int64_t* iters,
+                    double* dt_hist) {
   const int S = 2 + nd;
   const int64_t total = n[0] * (nd > 1 ? n[1] : 1);
   double* W = (double*)malloc(sizeof(double) * total * S);
@@ -591,12 +593,14 @@ int orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const in
     orc_ls_advect(phi1, V1, n, nd, dt, dh, tmp);
     memcpy(phi1, tmp, sizeof(double) * total);
     orc_ls_project2(phi0, phi1, total);
-    rc = orc_ls_reinit(phi0, n, nd, dh, band, tmp, NULL, NULL);
-    if (rc) goto done;
-    memcpy(phi0, tmp, sizeof(double) * total);
-    rc = orc_ls_reinit(phi1, n, nd, dh, band, tmp, NULL, NULL);
-    if (rc) goto done;
-    memcpy(phi1, tmp, sizeof(double) * total);
+    if (it % reinit_every == reinit_every - 1) {
+      rc = orc_ls_reinit(phi0, n, nd, dh, band, tmp, NULL, NULL);
+      if (rc) goto done;
+      memcpy(phi0, tmp, sizeof(double) * total);
+      rc = orc_ls_reinit(phi1, n, nd, dh, band, tmp, NULL, NULL);
+      if (rc) goto done;
+      memcpy(phi1, tmp, sizeof(double) * total);
+    }
     if (dt_hist) dt_hist[it] = dt;
     t += dt;
     it += 1;

@@ -117,7 +117,7 @@ def lib():
         L.orc_mm_combine.restype = None
         L.orc_mm_simulate.argtypes = [dp, dp, dp, dp, i64p, C.c_int, sp, sp, C.c_double,
                                       C.c_double, C.c_double, C.c_int64, C.c_int, C.c_int,
-                                      C.c_int, dp, i64p, dp]
+                                      C.c_int, C.c_int, dp, i64p, dp]
         L.orc_mm_simulate.restype = C.c_int
         _lib = L
     return _lib
@@ -364,7 +364,7 @@ def mm_combine(phi0, U0, U1):
 
 
 def mm_simulate(U0, U1, phi0, phi1, s0, s1, cfl, dh, sim_time=float("inf"), max_iters=1,
-                band=6, extrap_sweeps=8, nthreads=0):
+                band=6, extrap_sweeps=8, reinit_every=1, nthreads=0):
     """Two-material ghost-fluid run; returns (U0, U1, phi0, phi1, run_time, iters, dts)."""
     U0 = np.array(U0, dtype=np.float64, order="C")
     U1 = np.array(U1, dtype=np.float64, order="C")
@@ -378,7 +378,8 @@ def mm_simulate(U0, U1, phi0, phi1, s0, s1, cfl, dh, sim_time=float("inf"), max_
     it = C.c_int64(0)
     rc = lib().orc_mm_simulate(_dp(U0), _dp(U1), _dp(phi0), _dp(phi1), _n_ptr(n), nd,
                                C.byref(s0), C.byref(s1), cfl, dh, sim_time, int(max_iters),
-                               band, extrap_sweeps, nthreads, C.byref(rt), C.byref(it), _dp(dts))
+                               band, extrap_sweeps, reinit_every, nthreads, C.byref(rt),
+                               C.byref(it), _dp(dts))
     if rc:
         raise RuntimeError(f"orc_mm_simulate failed: {rc}")
     return U0, U1, phi0, phi1, rt.value, it.value, dts[: it.value]

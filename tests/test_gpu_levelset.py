@@ -189,14 +189,15 @@ def test_combine(L, oracle):
 
 # ---- the two-material step ---------------------------------------------------------
 
-def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8):
+def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8, every=1):
     es0, es1 = _schemes_e(E, sb)
     sim = L.MultimaterialSimulator(layout=layout)
     U0, U1 = _dev(sb.U0, layout), _dev(sb.U1, layout)
     p0, p1 = _dev(sb.phi0), _dev(sb.phi1)
     sim.add_material(U0, p0, es0)
     sim.add_material(U1, p1, es1)
-    r = sim.simulate_mm(0.5, sb.dh, max_iters=steps, band=band, extrap_sweeps=sweeps)
+    r = sim.simulate_mm(0.5, sb.dh, max_iters=steps, band=band, extrap_sweeps=sweeps,
+                        reinit_every=every)
     return r, _host(U0, layout), _host(U1, layout), _host(p0), _host(p1)
 
 
@@ -205,14 +206,15 @@ def _schemes_e(E, sb):
     return E.Scheme(gamma=sb.gamma0), E.Scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
 
 
-@pytest.mark.parametrize("stiff,layout", [(False, 0), (False, 1), (True, 0)])
-def test_mm_simulate_bitwise(L, E, oracle, stiff, layout):
+@pytest.mark.parametrize("stiff,layout,every", [(False, 0, 1), (False, 1, 1), (True, 0, 1),
+                                               (False, 0, 5)])
+def test_mm_simulate_bitwise(L, E, oracle, stiff, layout, every):
     sb = _bubble(256, 128, stiff)
     os0, os1, _, _ = _schemes(oracle, E, sb)
     steps = 12
-    r, U0, U1, p0, p1 = _run_mm(L, E, sb, steps, layout)
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, steps, layout, every=every)
     R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
-                           max_iters=steps)
+                           max_iters=steps, reinit_every=every)
     assert r.iters == R[5] == steps
     assert r.run_time == R[4]
     assert r.last_dt == R[6][-1]
