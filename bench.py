@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
     ap.add_argument("--y-rows", type=int, default=0)
-    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost", "batch"], default="bench",
                     help="sweeps: a plain device copy of one grid (HBM calibration for "
                          "counter runs) followed by --steps x sweeps and y sweeps")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
@@ -67,6 +67,10 @@ def main():
     from fluidity_amd import native as N
     from fluidity_amd import workloads as W
 
+    if args.mode == "batch":
+        stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+        print(json.dumps(batched_case(torch, N, W, stream, args.steps)), flush=True)
+        return
     if args.mode == "ghost":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
         print(json.dumps(ghost_fluid_case(torch, N, W, stream, args.steps, args.warmup)),
@@ -164,6 +168,10 @@ def main():
             torch.cuda.empty_cache()
 
         variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
+        bf = batched_case(torch, N, W, stream, steps=20)
+        for k, v in bf.items():
+            v["vs_f64_hllc"] = round(v["value"] / value, 3)
+            variants[k] = v
 
     cpu = None
     if not args.no_cpu:
@@ -190,6 +198,51 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def batched_case(torch, N, W, stream, steps):
+    """The 512-problem batch (seed 1003: 384 1D problems of 256..65536 cells,
+    128 2D problems of 64^2..1024^2, per-problem gamma and dt), FP64 and FP32
+    bulk, tile shapes from the autotuner, steps replayed from a hipGraph.  A
+    cell-update is one cell of one problem advanced one step.  The FP32 line
+    carries the per-problem relative L1 density error against the FP64 batch,
+    which equals the oracle run on each problem alone bit for bit."""
+    from fluidity_amd import batch as BT
+    specs = W.batched(seed=1003)
+    cells = sum(int(np.prod(sp.n)) for sp in specs)
+    out, ref = {}, None
+    for name, dt in (("batched_f64", torch.float64), ("batched_f32", torch.float32)):
+        probs = BT.problems_from_workload(specs, dt, N.FL_LAYOUT_SOA, max_iters=steps + 2)
+        b = BT.Batch(probs, stream=stream)
+        t = b.autotune(reps=2, stream=stream)
+        b.step(2, graph=True, stream=stream)          # warm-up (captures the graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps, graph=True, stream=stream)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res = b.finish(allow_failed=True, stream=stream)
+        # problems the scheme cannot run stop after their NaN step: count the
+        # cell-updates actually done in the timed steps
+        done = sum(int(np.prod(sp.n)) * max(0, r.iters - 2) for sp, r in zip(specs, res))
+        ok = [bool(np.isfinite(r.max_wavespeed) and r.max_wavespeed > 0) for r in res]
+        line = {"value": round(done / el, 1), "unit": "cell-updates/s",
+                "ms_per_step": round(el * 1e3 / steps, 4), "problems": len(specs),
+                "failed_problems": ok.count(False), "cells": cells,
+                "tuning": [t.x_block, t.x_rows, t.y_block, t.y_rows]}
+        if ref is None:
+            ref = [(b.state(i)[0].clone(), ok[i]) for i in range(len(specs))]
+        else:
+            errs = [float((b.state(i)[0].double() - ref[i][0]).abs().sum() /
+                          ref[i][0].abs().sum())
+                    for i in range(len(specs)) if ref[i][1] and ok[i]]
+            line["density_rel_l1_max"] = float("%.3g" % max(errs))
+            line["density_rel_l1_median"] = float("%.3g" % float(np.median(errs)))
+            line["error_problems"] = len(errs)
+        out[name] = line
+        del b, probs
+        torch.cuda.empty_cache()
+    return out
 
 
 def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024):

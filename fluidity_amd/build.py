@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfluidity_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fl_sweep.hip", "fl_abi.hip", "fl_levelset.hip", "fl_ls_abi.hip"]
+SOURCES = ["fl_sweep.hip", "fl_abi.hip", "fl_levelset.hip", "fl_ls_abi.hip", "fl_batch.hip",
+           "fl_batch_abi.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -1,483 +1,20 @@
-// fl_sweep.hip -- gfx950 kernels of the dimensionally split Euler step.
-//
-// Replaces the reference's solve_dim<S> / SplitSolver::invoke sweep kernel
-// (solver/cuda/split_solver.cuh:39-80, solver/split_solver.hpp:138-162), its
-// set_wavespeeds_impl + reduce_impl CFL passes
-// (simulator/cuda/wavespeed_initialization.cuh:41-97, algorithm/reduce.cuh:44-131)
-// and the host-side dt update (simulator/parameters.hpp:60-99).
-//
-// Work decomposition (both 2D sweeps): a workgroup owns a band of columns and
-// marches down a chunk of rows, so each input row is read from HBM once per
-// sweep and the cross-dimension part of the MUSCL-Hancock flattening factor
-// (a +-3 cell stencil in the non-sweep dimension) comes from per-thread
-// register windows instead of re-reads.
-//  * x sweep: lanes run along x.  Face states, face fluxes and the x-halo are
-//    exchanged through LDS (4 barriers per row).  A workgroup of B lanes
-//    evaluates B cells' face states and writes B-2 updated cells.
-//  * y sweep: lanes run along x, each lane marches its own column, so the
-//    +-4 cell sweep stencil, the face states and the face fluxes stay in
-//    registers; only the x-direction flattening factor uses LDS.
-// The fused CFL epilogue takes the max wavespeed of every updated cell,
-// reduces it per workgroup and issues one 64-bit atomicMax on its bit pattern
-// (non-negative doubles order like their bit patterns; NaN sorts above +inf,
-// so a NaN anywhere surfaces as a NaN max).
-#include "fl_kernels.hpp"
-
-// Waves per SIMD the 2D sweep kernels are compiled for (register budget
-// 512 / FL_SWEEP_WAVES VGPRs).
-#ifndef FL_SWEEP_WAVES
-#define FL_SWEEP_WAVES 2
-#endif
-
+// fl_sweep.hip -- gfx950 kernels of the dimensionally split Euler step on one
+// grid: the x and y sweeps (tile bodies in fl_sweep_tiles.hpp), the
+// standalone wavespeed max, the device step control and FP64 sums.
+#include "fl_sweep_tiles.hpp"
 
 namespace fl {
-
-// ---------------------------------------------------------------------------
-// memory access
-// ---------------------------------------------------------------------------
-
-template <typename T, int S>
-__device__ __forceinline__ void load_cell(const T* __restrict__ base, int64_t idx,
-                                          int64_t es, int64_t cs, T* u) {
-  const T* p = base + idx * es;
-#pragma unroll
-  for (int k = 0; k < S; ++k) u[k] = p[k * cs];
-}
-
-template <typename T, int S>
-__device__ __forceinline__ void store_cell(T* __restrict__ base, int64_t idx, int64_t es,
-                                           int64_t cs, const T* u) {
-  T* p = base + idx * es;
-#pragma unroll
-  for (int k = 0; k < S; ++k) p[k * cs] = u[k];
-}
-
-// Ghost index map (solver/boundary_loader.hpp:175-199): transmissive and
-// reflective ghosts mirror the interior, ghost(-k-1) = cell(k) and
-// ghost(N+k) = cell(N-1-k); periodic wraps.  i is clamped to the 4-cell pad
-// first so that lanes past the end of a partial band stay in bounds.
-__device__ __forceinline__ int64_t map_ghost(int64_t i, int64_t n, int kind, bool& flip) {
-  i = i < -4 ? -4 : (i > n + 3 ? n + 3 : i);
-  flip = false;
-  if (i >= 0 && i < n) return i;
-  if (kind == kBcPeriodic) return i < 0 ? i + n : i - n;
-  flip = (kind == kBcReflective);
-  return i < 0 ? -i - 1 : 2 * n - 1 - i;
-}
-
-// A cell load in flight.  The reflective-ghost velocity flip is applied when
-// the cell is taken, not when the load is issued, so that no instruction
-// touches the destination registers while the load is pending.
-template <typename T, int S>
-struct Pending {
-  T u[S];
-  int flip;  // bit 0: reflect in x, bit 1: reflect in y
-};
-
-template <typename T, int ND>
-__device__ __forceinline__ void issue_load(const SweepArgs<T>& a, int64_t x, int64_t y,
-                                           Pending<T, 2 + ND>& q) {
-  bool fx = false, fy = false;
-  const int64_t xi = map_ghost(x, a.nx, a.bcx, fx);
-  int64_t yi = 0;
-  if (ND > 1) yi = map_ghost(y, a.ny, a.bcy, fy);
-  load_cell<T, 2 + ND>(a.in, yi * a.nx + xi, a.es, a.cs, q.u);
-  q.flip = (fx ? 1 : 0) | (fy ? 2 : 0);
-}
-
-template <typename T, int ND>
-__device__ __forceinline__ void take(const Pending<T, 2 + ND>& q, T* u) {
-#pragma unroll
-  for (int k = 0; k < 2 + ND; ++k) u[k] = q.u[k];
-  if (q.flip & 1) reflect<T, ND>(u, 0);
-  if (ND > 1 && (q.flip & 2)) reflect<T, ND>(u, 1);
-}
-
-// ---------------------------------------------------------------------------
-// fused CFL epilogue
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ unsigned long long ws_bits(double w) {
-  w = (w == 0.0) ? 0.0 : w;  // -0 -> +0 so that the bit order is the value order
-  return (unsigned long long)__double_as_longlong(w);
-}
-
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffull), off, 64);
-    const unsigned hi = __shfl_xor((unsigned)(v >> 32), off, 64);
-    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
-// Block-wide max of v; thread 0 issues one atomicMax.  Must be reached by every
-// thread of the block.
-__device__ __forceinline__ void block_atomic_max(unsigned long long v, unsigned long long* dst) {
-  __shared__ unsigned long long red[kMaxBlock / 64];
-  v = wave_max_u64(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int i = 1; i < nw; ++i) v = red[i] > v ? red[i] : v;
-    atomicMax(dst, v);
-  }
-}
-
-template <typename T, int ND>
-__device__ __forceinline__ double cell_wavespeed64(const T* u, const Material<double>& m) {
-  double ud[2 + ND];
-#pragma unroll
-  for (int k = 0; k < 2 + ND; ++k) ud[k] = (double)u[k];
-  const Prim<double, ND> w = prim_of<double, ND>(ud, m);
-  return max_wavespeed<double, ND>(ud, w, m);
-}
-
-template <typename T>
-__device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
-  if (a.ctl) {
-    if (!a.ctl->active) return false;
-    dtdh = (T)a.ctl->dtdh;
-  } else {
-    dtdh = a.dtdh_host;
-  }
-  return true;
-}
-
-// ---------------------------------------------------------------------------
-// x sweep (also the 1D sweep)
-// ---------------------------------------------------------------------------
 
 template <typename T, int ND, int FLUX>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
 sweep_x_kernel(SweepArgs<T> a) {
-  constexpr int S = 2 + ND;
-  constexpr int W = kMaxBlock + 2 * kHalo;
-  __shared__ T sU[S][W];
-  __shared__ T sP[W];
-  __shared__ T sV[W];
-  __shared__ T sK[W];
-  __shared__ T sL[S][kMaxBlock];
-  __shared__ T sG[S][kMaxBlock];
-
-  T dtdh;
-  if (!sweep_dtdh(a, dtdh)) return;
-  const T half_dtdh = T(0.5) * dtdh;
-  const T dhdt = T(1) / dtdh;
-
-  const int B = blockDim.x;
-  const int t = threadIdx.x;
-  const int s = t + kHalo;                       // LDS slot of this lane's column
-  const int64_t x0 = (int64_t)blockIdx.x * (B - 2);
-  const int64_t c = x0 - 1 + t;                  // column whose face states this lane builds
-  const int64_t y0 = (ND == 2) ? (int64_t)blockIdx.y * a.rows_per_block : 0;
-  const int64_t yend = (ND == 2) ? min(y0 + (int64_t)a.rows_per_block, a.ny) : 1;
-
-  // Halo lanes: t < 6 stage columns x0-4..x0-2 (slots 0..2) and
-  // x0+B-1..x0+B+1 (slots B+3..B+5) of the output row.
-  // Every lane loads one of the six halo cells (lane t takes cell t mod 6), so
-  // the halo loads are unconditional; only lanes t < 6 stage them in LDS.
-  const int hl = t % (2 * kHalo);
-  const int hslot = (hl < 3) ? hl : B + hl;
-  const int64_t hcol = x0 - 1 - kHalo + hslot;
-
-  // Register windows along y (2D): rows r-3..r of U, u_x; rows r-4..r of p;
-  // rows r-3..r of u_y; K_y of rows r-4..r-2.
-  T Uw[4][S];
-  T pw[5], vxw[4], vyw[4];
-  // K_y window as named scalars: an array indexed by the sign of the pressure
-  // difference would be demoted to scratch memory.
-  T ky0 = T(1), ky1 = T(1), ky2 = T(1);
-#pragma unroll
-  for (int i = 0; i < 5; ++i) pw[i] = T(0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) { vxw[i] = T(0); vyw[i] = T(0); }
-
-  unsigned long long wsmax = 0ull;
-  const int64_t rbeg = (ND == 2) ? y0 - 3 : 0;
-  const int64_t rlast = (ND == 2) ? yend + 2 : 0;
-
-  // Software pipelining with two row buffers in fixed registers (A for even,
-  // B for odd steps of the unrolled loop): the loads of row r+2 and of the
-  // halo cells of the output row two steps ahead are issued while row r is
-  // computed, without ever copying a register whose load is still in flight.
-  // All loads are unconditional (rows past the end are re-loaded, clamped), so
-  // the compiler can count the loads issued after each one and wait only for
-  // the one it needs.
-  auto hrow = [&](int64_t r) { return (ND == 2) ? r - 3 : int64_t(0); };
-  Pending<T, S> QA, QB, HA, HB;
-  issue_load<T, ND>(a, c, rbeg, QA);
-  issue_load<T, ND>(a, c, min(rbeg + 1, rlast), QB);
-  issue_load<T, ND>(a, hcol, hrow(rbeg), HA);
-  issue_load<T, ND>(a, hcol, hrow(min(rbeg + 1, rlast)), HB);
-
-  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
-  // two-step loop is fully unrolled so that each copy names its buffers
-  // statically.
-  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
-#pragma unroll
-  for (int step = 0; step < 2; ++step) {
-    const int64_t r = r0 + step;
-    if (r > rlast) break;
-    Pending<T, S>& Q = step ? QB : QA;
-    Pending<T, S>& H = step ? HB : HA;
-    // ---- the newest row of this lane's column and its primitives
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-      for (int k = 0; k < S; ++k) Uw[i][k] = Uw[i + 1][k];
-      vxw[i] = vxw[i + 1];
-      vyw[i] = vyw[i + 1];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
-    take<T, ND>(Q, Uw[3]);
-    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
-    T uh[S];  // halo cell of output row hrow(r)
-    take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, hrow(min(r + 2, rlast)), H);
-    {
-      const Prim<T, ND> w = prim_of<T, ND>(Uw[3], a.mat);
-      pw[4] = w.p;
-      vxw[3] = w.vel[0];
-      vyw[3] = w.vel[ND - 1];
-    }
-    if (ND == 2) {
-      if (r >= y0 + 1) {  // K_y of row r-2 (needs rows r-4..r)
-        ky0 = ky1;
-        ky1 = ky2;
-        ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
-      }
-      if (r < y0 + 3) continue;  // block-uniform: warm-up rows
-    }
-    const int64_t jo = (ND == 2) ? r - 3 : 0;    // output row
-    const T* U0 = (ND == 2) ? Uw[0] : Uw[3];
-    const T p0 = (ND == 2) ? pw[1] : pw[4];
-    const T vx0 = (ND == 2) ? vxw[0] : vxw[3];
-
-    // ---- phase A: stage the row's states, pressures and x velocities
-#pragma unroll
-    for (int k = 0; k < S; ++k) sU[k][s] = U0[k];
-    sP[s] = p0;
-    sV[s] = vx0;
-    if (t < 2 * kHalo) {
-      const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
-#pragma unroll
-      for (int k = 0; k < S; ++k) sU[k][hslot] = uh[k];
-      sP[hslot] = wh.p;
-      sV[hslot] = wh.vel[0];
-    }
-    __syncthreads();
-
-    // ---- phase B: x flattening factors K_x (own column, plus the two
-    // columns just outside the lane range)
-    sK[s] = chi_factor(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1]);
-    if (t < 2) {
-      const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
-      sK[hs] = chi_factor(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1], sV[hs + 1]);
-    }
-    const int sx = signum(sP[s + 1] - sP[s - 1]);
-    T um[S], up[S];
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-      um[k] = sU[k][s - 1];
-      up[k] = sU[k][s + 1];
-    }
-    __syncthreads();
-
-    // ---- phase C: chi (muscl_reconstructor.hpp:106-114) and face states
-    T chi = smin(Limits<T>::max(), smin(sK[s], sK[s + sx]));
-    if (ND == 2) {
-      const int sy = signum(pw[2] - pw[0]);
-      const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
-      chi = smin(chi, smin(ky1, ky_nb));
-    }
-    T R[S], L[S];
-    evolve_faces<T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
-#pragma unroll
-    for (int k = 0; k < S; ++k) sL[k][t] = L[k];
-    __syncthreads();
-
-    // ---- phase D: face flux between column c and c+1
-    T G[S];
-    if (t < B - 1) {
-      T Lr[S];
-#pragma unroll
-      for (int k = 0; k < S; ++k) Lr[k] = sL[k][t + 1];
-      numerical_flux<T, ND, 0, FLUX>(R, Lr, dtdh, dhdt, a.mat, G);
-#pragma unroll
-      for (int k = 0; k < S; ++k) sG[k][t] = G[k];
-    }
-    __syncthreads();
-
-    // ---- phase E: conservative update U + dtdh (F_{c-1/2} - F_{c+1/2})
-    if (t >= 1 && t <= B - 2 && c < a.nx) {
-      T o[S];
-#pragma unroll
-      for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k][t - 1] - G[k]) * dtdh;
-      store_cell<T, S>(a.out, jo * a.nx + c, a.es, a.cs, o);
-      if (a.ws_out) {
-        const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(o, a.mat64));
-        wsmax = b > wsmax ? b : wsmax;
-      }
-    }
-  }
-  }
-  if (a.ws_out) block_atomic_max(wsmax, a.ws_out);
+  sweep_x_tile<T, ND, FLUX>(a, blockIdx.x, blockIdx.y);
 }
-
-// ---------------------------------------------------------------------------
-// y sweep
-// ---------------------------------------------------------------------------
 
 template <typename T, int FLUX>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
 sweep_y_kernel(SweepArgs<T> a) {
-  constexpr int ND = 2, S = 4;
-  constexpr int W = kMaxBlock + 2 * kHalo;
-  __shared__ T sP[W];
-  __shared__ T sV[W];
-  __shared__ T sK[W];
-
-  T dtdh;
-  if (!sweep_dtdh(a, dtdh)) return;
-  const T half_dtdh = T(0.5) * dtdh;
-  const T dhdt = T(1) / dtdh;
-
-  const int B = blockDim.x;
-  const int t = threadIdx.x;
-  const int s = t + kHalo;
-  const int64_t x0 = (int64_t)blockIdx.x * B;
-  const int64_t c = x0 + t;
-  const int64_t y0 = (int64_t)blockIdx.y * a.rows_per_block;
-  const int64_t yend = min(y0 + (int64_t)a.rows_per_block, a.ny);
-
-  // Halo lanes stage columns x0-3..x0-1 (slots 0..2) and x0+B..x0+B+2
-  // (slots B+3..B+5).
-  // Every lane loads one of the six halo cells (lane t takes cell t mod 6);
-  // only lanes t < 6 stage them in LDS.
-  const int hl = t % (2 * kHalo);
-  const int hslot = (hl < 3) ? hl : B + hl;
-  const int64_t hcol = x0 - kHalo + hslot;
-
-  // Windows along y: U rows r-4..r, p rows r-4..r, u_y rows r-3..r,
-  // K_y rows r-4..r-2, x flattening chi_x rows r-3..r.
-  T Uw[5][S];
-  T pw[5], vyw[4], cxw[4];
-  T ky0 = T(1), ky1 = T(1), ky2 = T(1);  // named scalars (see the x sweep)
-#pragma unroll
-  for (int i = 0; i < 5; ++i) pw[i] = T(0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) { vyw[i] = T(0); cxw[i] = T(1); }
-  T Rprev[S], Gprev[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) { Rprev[k] = T(0); Gprev[k] = T(0); }
-
-  unsigned long long wsmax = 0ull;
-  // Software pipelining with two row buffers in fixed registers (see the x
-  // sweep): rows r+2 (and their halo cells) load while row r is computed.
-  const int64_t rbeg = y0 - 4, rlast = yend + 3;
-  // All loads are unconditional (see the x sweep); the halo cells of row r are
-  // loaded on the same schedule as the row itself.
-  Pending<T, S> QA, QB, HA, HB;
-  issue_load<T, ND>(a, c, rbeg, QA);
-  issue_load<T, ND>(a, c, rbeg + 1, QB);
-  issue_load<T, ND>(a, hcol, rbeg, HA);
-  issue_load<T, ND>(a, hcol, rbeg + 1, HB);
-  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
-  // two-step loop is fully unrolled so that each copy names its buffers
-  // statically.
-  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
-#pragma unroll
-  for (int step = 0; step < 2; ++step) {
-    const int64_t r = r0 + step;
-    if (r > rlast) break;
-    Pending<T, S>& Q = step ? QB : QA;
-    Pending<T, S>& H = step ? HB : HA;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int k = 0; k < S; ++k) Uw[i][k] = Uw[i + 1][k];
-      pw[i] = pw[i + 1];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      vyw[i] = vyw[i + 1];
-      cxw[i] = cxw[i + 1];
-    }
-    take<T, ND>(Q, Uw[4]);
-    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
-    T uh[S];
-    take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, min(r + 2, rlast), H);
-    const Prim<T, ND> w = prim_of<T, ND>(Uw[4], a.mat);
-    pw[4] = w.p;
-    vyw[3] = w.vel[1];
-
-    // ---- x flattening of row r: min(K_x(c), K_x(c + sign_x))
-    if (r >= y0 - 1) {  // block-uniform
-      sP[s] = w.p;
-      sV[s] = w.vel[0];
-      if (t < 2 * kHalo) {
-        const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
-        sP[hslot] = wh.p;
-        sV[hslot] = wh.vel[0];
-      }
-      __syncthreads();
-      sK[s] = chi_factor(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1]);
-      if (t < 2) {
-        const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
-        sK[hs] = chi_factor(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1], sV[hs + 1]);
-      }
-      const int sx = signum(sP[s + 1] - sP[s - 1]);
-      __syncthreads();
-      cxw[3] = smin(sK[s], sK[s + sx]);
-    }
-
-    // ---- y flattening K_y of row r-2
-    if (r >= y0) {
-      ky0 = ky1;
-      ky1 = ky2;
-      ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
-    }
-
-    // ---- face states of row jr = r-3
-    if (r >= y0 + 2) {
-      const int sy = signum(pw[2] - pw[0]);
-      T chi = smin(Limits<T>::max(), cxw[0]);
-      const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
-      chi = smin(chi, smin(ky1, ky_nb));
-      T R[S], L[S];
-      evolve_faces<T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, a.limiter, a.form, a.mat,
-                             R, L);
-      if (r >= y0 + 3) {
-        // face between rows jr-1 and jr
-        T G[S];
-        numerical_flux<T, ND, 1, FLUX>(Rprev, L, dtdh, dhdt, a.mat, G);
-        const int64_t jo = r - 4;
-        if (jo >= y0 && c < a.nx) {
-          T o[S];
-#pragma unroll
-          for (int k = 0; k < S; ++k) o[k] = Uw[0][k] + (Gprev[k] - G[k]) * dtdh;
-          store_cell<T, S>(a.out, jo * a.nx + c, a.es, a.cs, o);
-          if (a.ws_out) {
-            const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(o, a.mat64));
-            wsmax = b > wsmax ? b : wsmax;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < S; ++k) Gprev[k] = G[k];
-      }
-#pragma unroll
-      for (int k = 0; k < S; ++k) Rprev[k] = R[k];
-    }
-  }
-  }
-  if (a.ws_out) block_atomic_max(wsmax, a.ws_out);
+  sweep_y_tile<T, FLUX>(a, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,9 @@ EXPORTED = [
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
     "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate",
     "fl_mm_combine", "fl_mm_work_bytes", "fl_mm_simulate",
+    "fl_batch_create", "fl_batch_destroy", "fl_batch_set_tuning", "fl_batch_get_tuning",
+    "fl_batch_autotune", "fl_batch_reset", "fl_batch_step", "fl_batch_finish",
+    "fl_batch_conservation_sums",
 ]
 
 # entry points returning a size rather than a status
@@ -106,6 +109,15 @@ class MmParams(C.Structure):
     ]
 
 
+class BatchProblem(C.Structure):
+    _fields_ = [
+        ("state", Field),
+        ("work", Field),
+        ("scheme", Scheme),
+        ("params", SimParams),
+    ]
+
+
 class FluidityError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{message} (status {status})")
@@ -149,6 +161,15 @@ def _load():
     L.fl_mm_work_bytes.argtypes = [C.c_int, i64p]
     L.fl_mm_simulate.argtypes = [vp, C.POINTER(Material), C.c_int, C.POINTER(SimParams),
                                  C.POINTER(MmParams), vp, C.c_int64, C.POINTER(SimResult), vp]
+    L.fl_batch_create.argtypes = [vp, C.POINTER(BatchProblem), C.c_int, C.POINTER(vp), vp]
+    L.fl_batch_destroy.argtypes = [vp]
+    L.fl_batch_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.fl_batch_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.fl_batch_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning), vp]
+    L.fl_batch_reset.argtypes = [vp, vp]
+    L.fl_batch_step.argtypes = [vp, C.c_int64, C.c_int, vp]
+    L.fl_batch_finish.argtypes = [vp, C.POINTER(SimResult), vp]
+    L.fl_batch_conservation_sums.argtypes = [vp, C.POINTER(C.c_double), vp]
     for name in EXPORTED:
         if name in _SIZE_RETURNING:
             getattr(L, name).restype = C.c_int64

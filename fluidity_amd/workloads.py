@@ -237,6 +237,13 @@ def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D
         if ok:
             break
         rejected += 1
+    U = _quadrant_grid(n, x0, y0, q, gamma)
+    return Riemann2D(U, q, (x0, y0), gamma, 1.0 / n, rejected)
+
+
+def _quadrant_grid(n: int, x0: float, y0: float, q, gamma: float) -> np.ndarray:
+    """Conservative n x n grid of four quadrant states (Q1 x >= x0, y >= y0,
+    then anticlockwise); cell i is right of the split if i/n >= x0."""
     x = np.arange(n) / n
     right = x >= x0
     top = x >= y0
@@ -248,8 +255,62 @@ def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D
                 (i_right, j_top)]
             mask = (top[:, None] == j_top) & (right[None, :] == i_right)
             W[mask] = Q[qi]
-    U = prim_to_cons(W, gamma)
-    return Riemann2D(np.ascontiguousarray(U), q, (x0, y0), gamma, 1.0 / n, rejected)
+    return np.ascontiguousarray(prim_to_cons(W, gamma))
+
+
+def _draw_quadrants(rng: SplitMix64, gamma: float):
+    rejected = 0
+    while True:
+        q = [draw_prim(rng, 2) for _ in range(4)]
+        ok = (no_vacuum(q[1], q[0], gamma, 0) and no_vacuum(q[2], q[3], gamma, 0)
+              and no_vacuum(q[3], q[0], gamma, 1) and no_vacuum(q[2], q[1], gamma, 1))
+        if ok:
+            return q, rejected
+        rejected += 1
+
+
+@dataclass
+class BatchProblem:
+    nd: int
+    n: tuple             # (nx,) or (nx, ny)
+    gamma: float
+    dh: float
+    U: np.ndarray        # conservative AoS, float64
+    states: list         # primitive states: [wl, wr] (1D) or the four quadrants (2D)
+    split: tuple         # (0.5,) or (x0, y0)
+
+
+def batched(seed: int = 1003, n1d: int = 384, n2d: int = 128, len1d=(256, 65536),
+            side2d=(64, 1024)) -> list:
+    """Independent Riemann problems of uneven size: n1d 1D problems with N
+    log-uniform in len1d (split at x = 0.5) and n2d 2D four-quadrant problems
+    with side log-uniform in side2d (split point uniform in [0.3, 0.7]^2), each
+    with gamma uniform in [1.2, 1.67] and dh = 1/N.  States are drawn as in the
+    single-problem workloads from one SplitMix64 stream, vacuum-producing
+    draws rejected."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(n1d + n2d):
+        gamma = rng.uniform(1.2, 1.67)
+        if i < n1d:
+            n = int(round(rng.log_uniform(*len1d)))
+            while True:
+                wl = draw_prim(rng, 1)
+                wr = draw_prim(rng, 1)
+                if no_vacuum(wl, wr, gamma):
+                    break
+            x = np.arange(n) / n
+            W = np.where((x < 0.5)[:, None], np.array(wl)[None, :], np.array(wr)[None, :])
+            U = np.ascontiguousarray(prim_to_cons(W.astype(np.float64), gamma))
+            out.append(BatchProblem(1, (n,), gamma, 1.0 / n, U, [wl, wr], (0.5,)))
+        else:
+            n = int(round(rng.log_uniform(*side2d)))
+            x0 = rng.uniform(0.3, 0.7)
+            y0 = rng.uniform(0.3, 0.7)
+            q, _ = _draw_quadrants(rng, gamma)
+            U = _quadrant_grid(n, x0, y0, q, gamma)
+            out.append(BatchProblem(2, (n, n), gamma, 1.0 / n, U, q, (x0, y0)))
+    return out
 
 
 @dataclass

@@ -144,6 +144,54 @@ int fl_simulate(fl_context* ctx, fl_field* state, fl_field* work, const fl_schem
  * FP32 bulk path. */
 int fl_conservation_sums(fl_context* ctx, const fl_field* in, double* sums, void* stream);
 
+/* ---- batched problems -------------------------------------------------------
+ * Many independent grids of uneven size (1D and 2D, each with its own
+ * material, CFL dt and stop condition) advanced together: one step is a
+ * control kernel plus one persistent launch per sweep phase whose workgroups
+ * take tiles from a shared counter; steps are captured once in a hipGraph and
+ * replayed.  Each problem's result is bit-identical to fl_simulate on that
+ * problem alone.  The batch as a whole replaces a host loop of
+ * SimulatorImpl::simulate calls (simulator/simulator_impl.hpp:222-258), one
+ * per problem, with Parameters::update_time_delta / dt_dh per problem
+ * (simulator/parameters.hpp:60-99). */
+typedef struct fl_batch fl_batch;
+
+typedef struct fl_batch_problem {
+  fl_field      state;   /* initial data; the result after fl_batch_finish     */
+  fl_field      work;    /* same-shaped scratch buffer                         */
+  fl_scheme     scheme;  /* material, limiter, BCs; one flux for the batch     */
+  fl_sim_params params;  /* cfl, dh, sim_time, max_iters of this problem       */
+} fl_batch_problem;
+
+/* Builds the device tables of a batch (all problems share one dtype; the
+ * batch owns its tables, allocated here) and computes each problem's initial
+ * wavespeed on `stream`.  The problems' buffers stay owned by the caller. */
+int fl_batch_create(fl_context* ctx, const fl_batch_problem* problems, int nprob,
+                    fl_batch** batch, void* stream);
+int fl_batch_destroy(fl_batch* batch);
+/* Tile shapes of the two sweep phases (x_block/x_rows: x tiles of 2D problems
+ * and 1D tiles; y_block/y_rows: y tiles).  Zeros keep the current values. */
+int fl_batch_set_tuning(fl_batch* batch, const fl_tuning* tuning);
+int fl_batch_get_tuning(const fl_batch* batch, fl_tuning* tuning);
+/* Times every candidate tile shape of each phase on this batch (`reps` timed
+ * launches each, states untouched) and keeps the fastest.  Synchronises. */
+int fl_batch_autotune(fl_batch* batch, int reps, fl_tuning* chosen, void* stream);
+/* Restarts every problem's clock and iteration count from its current state. */
+int fl_batch_reset(fl_batch* batch, void* stream);
+/* Enqueues `steps` steps (problems past their stop condition do nothing);
+ * use_graph replays a captured hipGraph of one step.  Asynchronous. */
+int fl_batch_step(fl_batch* batch, int64_t steps, int use_graph, void* stream);
+/* Puts every result in its state buffer and reports each problem's run time,
+ * iterations, last dt and final max wavespeed (results: nprob entries, may be
+ * NULL).  Synchronises.  A problem whose maximum wavespeed turns NaN takes
+ * that step with a NaN dt and then stops, as the reference loop does, while
+ * the others go on; FL_ERR_NUMERIC then names the first such problem (the
+ * results are filled all the same). */
+int fl_batch_finish(fl_batch* batch, fl_sim_result* results, void* stream);
+/* FP64 sums of each conserved component per problem (4 slots per problem,
+ * the 4th 0 for 1D) of the state buffers; call after fl_batch_finish. */
+int fl_batch_conservation_sums(fl_batch* batch, double* sums, void* stream);
+
 /* ---- level set and Riemann ghost fluid -------------------------------------
  * Level-set fields phi are FP64 device arrays of n[0] (1D) or n[0]*n[1] (2D)
  * values, x fastest, interior cells only; phi <= 0 inside the material.  State

@@ -1,0 +1,182 @@
+"""Batched Euler step on the GPU: every problem of a batch against the CPU
+oracle run on that problem alone (FP64 bit for bit, including the per-problem
+step count and run time), the hipGraph replay against plain launches, the
+autotuner's choice against the default tiling, and the FP32 bulk mode's
+per-problem density error against the FP64 oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fluidity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def B():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    from fluidity_amd import batch
+    return batch
+
+
+def _specs():
+    return W.batched(seed=1003, n1d=24, n2d=10, len1d=(64, 3000), side2d=(16, 160))
+
+
+def _host(T, nd, layout=1):
+    a = T.double().cpu().numpy()
+    if layout == 1:
+        a = np.moveaxis(a, 0, -1)
+    return np.ascontiguousarray(a)
+
+
+def _oracle_runs(oracle, specs, steps, sim_times=None):
+    out = []
+    for i, sp in enumerate(specs):
+        s = oracle.scheme(gamma=sp.gamma, bc=(0, 0))
+        st = float("inf") if sim_times is None else sim_times[i]
+        U, rt, it, _ = oracle.simulate(sp.U, s, 0.9, sp.dh, sim_time=st, max_iters=steps)
+        out.append((U, rt, it))
+    return out
+
+
+def _bits(a, b):
+    """Bit-identical, NaNs only at the same places (x86 and gfx950 NaN payloads
+    differ)."""
+    a, b = np.asarray(a), np.asarray(b)
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+def _same_float(x, y):
+    return (np.isnan(x) and np.isnan(y)) or x == y
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_batch_f64_bitwise(B, oracle, graph, layout):
+    specs = _specs()
+    steps = 17
+    probs = B.problems_from_workload(specs, torch.float64, layout, max_iters=steps)
+    bt = B.Batch(probs, layout=layout)
+    bt.step(steps, graph=graph)
+    res = bt.finish(allow_failed=True)
+    ref = _oracle_runs(oracle, specs, steps)
+    failed = 0
+    for i, sp in enumerate(specs):
+        assert res[i].iters == ref[i][2], i
+        assert _same_float(res[i].run_time, ref[i][1]), i
+        assert _bits(_host(bt.state(i), sp.nd, layout), ref[i][0]), f"problem {i} ({sp.n})"
+        failed += not np.isfinite(res[i].max_wavespeed)
+    # the seeded set holds problems the scheme cannot run (vacuum or negative
+    # pressure at CFL 0.9); they fail at the same step as on the oracle
+    assert failed == sum(not np.isfinite(r[0]).all() for r in ref)
+
+
+def test_batch_stops_per_problem(B, oracle):
+    """Each problem stops at its own sim_time; later steps leave it alone."""
+    specs = _specs()
+    rng = np.random.default_rng(11)
+    probs = B.problems_from_workload(specs, torch.float64, 1)
+    sim_times = []
+    for i, p in enumerate(probs):
+        st = float(rng.uniform(2, 30)) * specs[i].dh * 0.05
+        p["sim_time"] = st
+        sim_times.append(st)
+    bt = B.Batch(probs)
+    bt.step(60)
+    res = bt.finish(allow_failed=True)
+    ref = _oracle_runs(oracle, specs, 60, sim_times)
+    iters = [r.iters for r in res]
+    assert len(set(iters)) > 3
+    for i, sp in enumerate(specs):
+        assert res[i].iters == ref[i][2]
+        assert _same_float(res[i].run_time, ref[i][1])
+        assert _bits(_host(bt.state(i), sp.nd), ref[i][0]), i
+
+
+def test_batch_autotune_keeps_results(B, oracle):
+    specs = _specs()
+    steps = 9
+    probs = B.problems_from_workload(specs, torch.float64, 1, max_iters=steps)
+    bt = B.Batch(probs)
+    bt.step(4)                        # leaves some 1D results in the work buffers
+    t = bt.autotune(reps=1)
+    assert t.x_block in (64, 128, 256) and t.y_block in (64, 128, 256)
+    bt.step(steps - 4)
+    bt.finish(allow_failed=True)
+    ref = _oracle_runs(oracle, specs, steps)
+    for i, sp in enumerate(specs):
+        assert _bits(_host(bt.state(i), sp.nd), ref[i][0]), i
+
+
+def test_batch_sums(B):
+    specs = _specs()
+    probs = B.problems_from_workload(specs, torch.float64, 1, max_iters=5)
+    bt = B.Batch(probs)
+    bt.step(5)
+    bt.finish(allow_failed=True)
+    sums = bt.conservation_sums()
+    for i, sp in enumerate(specs):
+        U = _host(bt.state(i), sp.nd)
+        ref = U.reshape(-1, U.shape[-1]).sum(axis=0)
+        np.testing.assert_allclose(sums[i, : U.shape[-1]], ref, rtol=1e-12)
+
+
+def test_batch_f32_error(B, oracle):
+    """FP32 bulk with FP64 dt: per-problem relative L1 density error against
+    the FP64 oracle after 20 steps, bound 1e-5 per problem."""
+    specs = _specs()
+    steps = 20
+    probs = B.problems_from_workload(specs, torch.float32, 1, max_iters=steps)
+    bt = B.Batch(probs)
+    bt.step(steps)
+    bt.finish(allow_failed=True)
+    ref = _oracle_runs(oracle, specs, steps)
+    errs = []
+    for i, sp in enumerate(specs):
+        U = _host(bt.state(i), sp.nd)
+        R = ref[i][0]
+        if not np.isfinite(R).all():
+            continue                      # the FP64 scheme itself fails here
+        errs.append(np.abs(U[..., 0] - R[..., 0]).sum() / np.abs(R[..., 0]).sum())
+    print("per-problem rel L1 density error (%d problems): max %.3g median %.3g" %
+          (len(errs), max(errs), float(np.median(errs))))
+    assert len(errs) >= len(specs) - 4
+    assert max(errs) <= 1e-5
+
+
+def test_batch_full_workload_f32(B):
+    """The 512-problem workload (384 1D up to 65536 cells, 128 2D up to
+    1024^2) in FP32: autotuned, graph-replayed steps; per-problem density error
+    against the FP64 batch (bit-identical to the oracle per problem, as the
+    tests above show) within 1e-5."""
+    specs = W.batched(seed=1003)
+    steps = 10
+    p64 = B.problems_from_workload(specs, torch.float64, 1, max_iters=steps)
+    b64 = B.Batch(p64)
+    b64.step(steps)
+    r64 = b64.finish(allow_failed=True)
+    ref = [b64.state(i)[0].double().clone() for i in range(len(specs))]
+    del b64, p64
+    p32 = B.problems_from_workload(specs, torch.float32, 1, max_iters=steps)
+    b32 = B.Batch(p32)
+    b32.autotune(reps=1)
+    b32.step(steps)
+    r32 = b32.finish(allow_failed=True)
+    errs = []
+    for i in range(len(specs)):
+        if not np.isfinite(r64[i].max_wavespeed):
+            continue                      # the FP64 scheme itself fails here
+        d = b32.state(i)[0].double()
+        R = ref[i]
+        errs.append(float((d - R).abs().sum() / R.abs().sum()))
+        assert r32[i].iters == r64[i].iters == steps
+    print("per-problem rel L1 density error (%d of %d problems): max %.3g median %.3g" %
+          (len(errs), len(specs), max(errs), float(np.median(errs))))
+    assert len(errs) >= 0.9 * len(specs)
+    assert max(errs) <= 1e-5
