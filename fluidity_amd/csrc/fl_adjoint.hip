@@ -1,0 +1,372 @@
+// fl_adjoint.hip -- gfx950 kernels of the hand-written adjoint of the split
+// MUSCL-Hancock + HLLC step (FP64).
+//
+// Forward: out_i = U_i + dtdh (G_{i-1/2} - G_{i+1/2}), G_{i+1/2} =
+// HLLC(R_i, L_{i+1}), R_i / L_i the evolved face states of cell i from cells
+// i-1..i+1 (solver/split_solver.hpp:138-162,
+// reconstruction/muscl_reconstructor.hpp:99-172, flux_method/flux_hllc.hpp).
+// Adjoint of one sweep, given lambda = dLoss/d out:
+//  * face kernel, one thread per face i+1/2: recompute R_i, L_{i+1} and G,
+//    mu = dtdh (lambda_{i+1} - lambda_i) (ghost cells carry no lambda), and
+//    back-propagate mu through HLLC and both evolutions into the adjoints of
+//    the four stencil cells i-1..i+2 (stored per face and slot), of dt/dh and
+//    of gamma (stored per face);
+//  * gather kernel, one thread per cell: lambda_in = lambda + the slot
+//    contributions of every face whose stencil holds the cell, also through
+//    ghost cells (periodic wrap, mirror, reflective flip), in a fixed order;
+//  * the per-face dt/dh and gamma adjoints are summed in a fixed-order
+//    two-level reduction, so the whole adjoint is deterministic.
+// dt = cfl dh / max wavespeed couples the step to the state through the
+// wavespeed of the cell holding the maximum (lowest index among ties); the
+// step's finalize kernel adds that term.
+#include "fl_adjoint.hpp"
+#include "fl_adjoint_kernels.hpp"
+
+namespace fl {
+namespace adj {
+
+__device__ __forceinline__ int64_t ghost_index(int64_t i, int64_t n, int bc, bool& flip) {
+  i = i < -4 ? -4 : (i > n + 3 ? n + 3 : i);
+  flip = false;
+  if (i >= 0 && i < n) return i;
+  if (bc == kBcPeriodic) return i < 0 ? i + n : i - n;
+  flip = (bc == kBcReflective);
+  return i < 0 ? -i - 1 : 2 * n - 1 - i;
+}
+
+// State at (x, y) with the ghost rule of each dimension (reflective ghosts
+// get the velocity flip of the forward sweep).
+template <int ND>
+__device__ __forceinline__ void load_state(const AGrid& g, int64_t x, int64_t y, double* u) {
+  bool fx = false, fy = false;
+  const int64_t xi = ghost_index(x, g.nx, g.bcx, fx);
+  const int64_t yi = ND > 1 ? ghost_index(y, g.ny, g.bcy, fy) : 0;
+  const double* p = g.U + (yi * g.nx + xi) * g.es;
+#pragma unroll
+  for (int k = 0; k < 2 + ND; ++k) u[k] = p[k * g.cs];
+  if (fx) reflect<double, ND>(u, 0);
+  if (ND > 1 && fy) reflect<double, ND>(u, 1);
+}
+
+template <int ND>
+__device__ __forceinline__ double pressure_at(const AGrid& g, const Material<double>& m,
+                                              int64_t x, int64_t y) {
+  double u[2 + ND];
+  load_state<ND>(g, x, y, u);
+  return prim_of<double, ND>(u, m).p;
+}
+
+template <int ND>
+__device__ __forceinline__ double velocity_at(const AGrid& g, const Material<double>& m,
+                                              int64_t x, int64_t y, int d) {
+  double u[2 + ND];
+  load_state<ND>(g, x, y, u);
+  return prim_of<double, ND>(u, m).vel[d];
+}
+
+// MHReconstructor::chi (reconstruction/muscl_reconstructor.hpp:59-87) and the
+// minimum over dimensions of evolve (:106-118), as the forward sweeps use it.
+template <int ND>
+__device__ double chi_at(const AGrid& g, const Material<double>& m, int64_t x, int64_t y,
+                         int d) {
+  const int64_t dx = (d == 0), dy = (d == 1);
+  const double p_next = pressure_at<ND>(g, m, x + dx, y + dy);
+  const double p_prev = pressure_at<ND>(g, m, x - dx, y - dy);
+  const double p_nnext = pressure_at<ND>(g, m, x + 2 * dx, y + 2 * dy);
+  const double p_pprev = pressure_at<ND>(g, m, x - 2 * dx, y - 2 * dy);
+  const double v_prev = velocity_at<ND>(g, m, x - dx, y - dy, d);
+  const double v_next = velocity_at<ND>(g, m, x + dx, y + dy, d);
+  return chi_factor<double>(p_pprev, p_prev, p_next, p_nnext, v_prev, v_next);
+}
+
+template <int ND>
+__device__ double chi_min(const AGrid& g, const Material<double>& m, int64_t x, int64_t y) {
+  double chi = DBL_MAX;
+  for (int d = 0; d < ND; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const int sign = signum(pressure_at<ND>(g, m, x + dx, y + dy) -
+                            pressure_at<ND>(g, m, x - dx, y - dy));
+    chi = smin(chi, smin(chi_at<ND>(g, m, x, y, d),
+                         chi_at<ND>(g, m, x + sign * dx, y + sign * dy, d)));
+  }
+  return chi;
+}
+
+template <int ND, int D>
+__global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __restrict__ lam,
+                                                       Material<double> m, int limiter,
+                                                       const StepRec* __restrict__ rec,
+                                                       double* __restrict__ contrib,
+                                                       double* __restrict__ face_g) {
+  constexpr int S = 2 + ND;
+  const int64_t n = D == 0 ? g.nx : g.ny;          // cells along the sweep
+  const int64_t lines = D == 0 ? g.ny : g.nx;
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (n + 1) * lines) return;
+  int64_t i, line;
+  if (D == 0) {
+    i = f % (n + 1) - 1;
+    line = f / (n + 1);
+  } else {
+    i = f / lines - 1;
+    line = f % lines;
+  }
+  auto pos = [&](int64_t k, int64_t& x, int64_t& y) {
+    if (D == 0) {
+      x = k;
+      y = line;
+    } else {
+      x = line;
+      y = k;
+    }
+  };
+  const double dtdh = rec->dtdh;
+  const double half_dtdh = 0.5 * dtdh;
+  double u[4][S];
+  for (int s = 0; s < 4; ++s) {
+    int64_t x, y;
+    pos(i - 1 + s, x, y);
+    load_state<ND>(g, x, y, u[s]);
+  }
+  int64_t xa, ya, xb, yb;
+  pos(i, xa, ya);
+  pos(i + 1, xb, yb);
+  const double chi_a = chi_min<ND>(g, m, xa, ya);
+  const double chi_b = chi_min<ND>(g, m, xb, yb);
+  double R[S], L[S], tmp[S], G[S];
+  evolve_faces<double, ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, kFormCons, m, R, tmp);
+  evolve_faces<double, ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, kFormCons, m, tmp, L);
+  hllc_flux<double, ND, D>(R, L, m, G);
+  double la[S], lb[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    la[k] = 0.0;
+    lb[k] = 0.0;
+  }
+  if (i >= 0) {
+    const double* p = lam + (ya * g.nx + xa) * g.es;
+#pragma unroll
+    for (int k = 0; k < S; ++k) la[k] = p[k * g.cs];
+  }
+  if (i + 1 < n) {
+    const double* p = lam + (yb * g.nx + xb) * g.es;
+#pragma unroll
+    for (int k = 0; k < S; ++k) lb[k] = p[k * g.cs];
+  }
+  double mu[S], g_dtdh = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const double dl = lb[k] - la[k];
+    mu[k] = dtdh * dl;
+    g_dtdh += G[k] * dl;
+  }
+  GMat gm{0.0};
+  double gR[S], gL[S], zero[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    gR[k] = 0.0;
+    gL[k] = 0.0;
+    zero[k] = 0.0;
+  }
+  hllc_adj<ND, D>(R, L, m, mu, gR, gL, gm);
+  double c[4][S];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int k = 0; k < S; ++k) c[s][k] = 0.0;
+  double g_half = 0.0;
+  evolve_adj<ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, m, gR, zero, c[0], c[1], c[2],
+                    g_half, gm);
+  evolve_adj<ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, m, zero, gL, c[1], c[2], c[3],
+                    g_half, gm);
+  double* out = contrib + f * (4 * S);
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int k = 0; k < S; ++k) out[s * S + k] = c[s][k];
+  face_g[2 * f] = g_dtdh + 0.5 * g_half;
+  face_g[2 * f + 1] = gm.gamma;
+}
+
+// lambda_in = lambda_out + contributions of every face whose stencil holds
+// the cell (directly or as a ghost alias).
+template <int ND, int D>
+__global__ void __launch_bounds__(256) gather_adj_kernel(AGrid g, const double* __restrict__ lam_out,
+                                                         double* __restrict__ lam_in,
+                                                         const double* __restrict__ contrib) {
+  constexpr int S = 2 + ND;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= g.nx * g.ny) return;
+  const int64_t x = cell % g.nx, y = cell / g.nx;
+  const int64_t n = D == 0 ? g.nx : g.ny;
+  const int64_t lines = D == 0 ? g.ny : g.nx;
+  const int64_t j = D == 0 ? x : y;
+  const int64_t line = D == 0 ? y : x;
+  const int bc = D == 0 ? g.bcx : g.bcy;
+  double acc[S];
+  const double* lp = lam_out + cell * g.es;
+#pragma unroll
+  for (int k = 0; k < S; ++k) acc[k] = lp[k * g.cs];
+  // virtual positions holding this cell: itself, then ghost aliases
+  int64_t vs[3];
+  int flips[3];
+  int nv = 0;
+  vs[nv] = j;
+  flips[nv++] = 0;
+  if (bc == kBcPeriodic) {
+    if (j >= n - 2) { vs[nv] = j - n; flips[nv++] = 0; }
+    if (j <= 1) { vs[nv] = j + n; flips[nv++] = 0; }
+  } else {
+    const int fl = bc == kBcReflective ? 1 : 0;
+    if (j <= 1) { vs[nv] = -j - 1; flips[nv++] = fl; }
+    if (j >= n - 2) { vs[nv] = 2 * n - 1 - j; flips[nv++] = fl; }
+  }
+  for (int a = 0; a < nv; ++a) {
+    const int64_t v = vs[a];
+    for (int s = 0; s < 4; ++s) {
+      const int64_t i = v + 1 - s;
+      if (i < -1 || i > n - 1) continue;
+      const int64_t f = D == 0 ? line * (n + 1) + (i + 1) : (i + 1) * lines + line;
+      const double* c = contrib + f * (4 * S) + s * S;
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const double ck = (flips[a] && k == 2 + D) ? -c[k] : c[k];
+        acc[k] += ck;
+      }
+    }
+  }
+  double* o = lam_in + cell * g.es;
+#pragma unroll
+  for (int k = 0; k < S; ++k) o[k * g.cs] = acc[k];
+}
+
+// Per-block sums of the per-face (dt/dh, gamma) adjoints; kPartialBlocks
+// blocks of 256 threads, fixed order.
+__global__ void __launch_bounds__(256) partial_sum_kernel(const double* __restrict__ face_g,
+                                                          int64_t nfaces,
+                                                          double* __restrict__ partials) {
+  __shared__ double s0[256], s1[256];
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < nfaces;
+       f += (int64_t)kPartialBlocks * 256) {
+    a0 += face_g[2 * f];
+    a1 += face_g[2 * f + 1];
+  }
+  s0[threadIdx.x] = a0;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s0[0];
+    partials[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+// Records this step's max wavespeed and dt/dh from the control block and
+// clears the argmax slot.
+__global__ void record_kernel(const StepCtl* ctl, int parity, StepRec* rec) {
+  rec->maxws_bits = ctl->ws_bits[parity];
+  rec->dtdh = ctl->dtdh;
+  rec->argmax = ~0ull;
+}
+
+// Lowest index of a cell whose wavespeed bits equal the recorded maximum.
+template <int ND>
+__global__ void __launch_bounds__(256) argmax_kernel(AGrid g, Material<double> m,
+                                                     StepRec* rec) {
+  const unsigned long long target = rec->maxws_bits;
+  unsigned long long best = ~0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.nx * g.ny;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double u[2 + ND];
+    const double* p = g.U + i * g.es;
+#pragma unroll
+    for (int k = 0; k < 2 + ND; ++k) u[k] = p[k * g.cs];
+    const Prim<double, ND> w = prim_of<double, ND>(u, m);
+    double ws = max_wavespeed<double, ND>(u, w, m);
+    ws = ws == 0.0 ? 0.0 : ws;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(ws);
+    if (b == target && (unsigned long long)i < best) best = (unsigned long long)i;
+  }
+  if (best != ~0ull) atomicMin(&rec->argmax, best);
+}
+
+// Sums the partials of the step's sweeps (x then y), turns the dt/dh adjoint
+// into the max-wavespeed adjoint (d(dt/dh)/d maxws = -(dt/dh)/maxws), adds
+// that cell's wavespeed adjoint into lambda_n and accumulates the gamma
+// adjoint.  One workgroup of 256 threads.
+template <int ND>
+__global__ void __launch_bounds__(256) finalize_kernel(AGrid g, Material<double> m,
+                                                       const StepRec* __restrict__ rec,
+                                                       const double* __restrict__ partials,
+                                                       int nsweeps, double* __restrict__ lam_n,
+                                                       double* __restrict__ acc) {
+  __shared__ double s0[256], s1[256];
+  double a0 = 0.0, a1 = 0.0;
+  for (int sw = 0; sw < nsweeps; ++sw)
+    for (int b = threadIdx.x; b < kPartialBlocks; b += 256) {
+      a0 += partials[sw * 2 * kPartialBlocks + 2 * b];
+      a1 += partials[sw * 2 * kPartialBlocks + 2 * b + 1];
+    }
+  s0[threadIdx.x] = a0;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double g_dtdh = s0[0];
+  const double maxws = __longlong_as_double((long long)rec->maxws_bits);
+  const double g_maxws = -g_dtdh * rec->dtdh / maxws;
+  GMat gm{0.0};
+  const unsigned long long c = rec->argmax;
+  if (c < (unsigned long long)(g.nx * g.ny)) {
+    double u[2 + ND], gu[2 + ND];
+    const double* p = g.U + (int64_t)c * g.es;
+#pragma unroll
+    for (int k = 0; k < 2 + ND; ++k) {
+      u[k] = p[k * g.cs];
+      gu[k] = 0.0;
+    }
+    wavespeed_adj<ND>(u, m, g_maxws, gu, gm);
+    double* l = lam_n + (int64_t)c * g.es;
+#pragma unroll
+    for (int k = 0; k < 2 + ND; ++k) l[k * g.cs] += gu[k];
+  }
+  acc[0] += s1[0] + gm.gamma;
+  acc[1] = g_dtdh;
+}
+
+__global__ void acc_clear_kernel(double* acc) {
+  acc[0] = 0.0;
+  acc[1] = 0.0;
+}
+
+#define FL_ADJ_INST(ND, D)                                                                  \
+  template __global__ void face_adj_kernel<ND, D>(AGrid, const double*, Material<double>,    \
+                                                  int, const StepRec*, double*, double*);    \
+  template __global__ void gather_adj_kernel<ND, D>(AGrid, const double*, double*,           \
+                                                    const double*);
+
+FL_ADJ_INST(1, 0)
+FL_ADJ_INST(2, 0)
+FL_ADJ_INST(2, 1)
+template __global__ void argmax_kernel<1>(AGrid, Material<double>, StepRec*);
+template __global__ void argmax_kernel<2>(AGrid, Material<double>, StepRec*);
+template __global__ void finalize_kernel<1>(AGrid, Material<double>, const StepRec*,
+                                            const double*, int, double*, double*);
+template __global__ void finalize_kernel<2>(AGrid, Material<double>, const StepRec*,
+                                            const double*, int, double*, double*);
+
+}  // namespace adj
+}  // namespace fl

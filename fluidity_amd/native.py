@@ -34,10 +34,13 @@ EXPORTED = [
     "fl_batch_create", "fl_batch_destroy", "fl_batch_set_tuning", "fl_batch_get_tuning",
     "fl_batch_autotune", "fl_batch_reset", "fl_batch_step", "fl_batch_finish",
     "fl_batch_conservation_sums",
+    "fl_adj_checkpoint_bytes", "fl_adj_work_bytes", "fl_adj_forward", "fl_adj_backward",
+    "fl_split_step_vjp",
 ]
 
 # entry points returning a size rather than a status
-_SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes")
+_SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes", "fl_adj_checkpoint_bytes",
+                   "fl_adj_work_bytes")
 
 
 class Field(C.Structure):
@@ -170,6 +173,14 @@ def _load():
     L.fl_batch_step.argtypes = [vp, C.c_int64, C.c_int, vp]
     L.fl_batch_finish.argtypes = [vp, C.POINTER(SimResult), vp]
     L.fl_batch_conservation_sums.argtypes = [vp, C.POINTER(C.c_double), vp]
+    L.fl_adj_checkpoint_bytes.argtypes = [fp, C.c_int64, C.c_int]
+    L.fl_adj_work_bytes.argtypes = [fp, C.c_int]
+    L.fl_adj_forward.argtypes = [vp, fp, sp, C.c_double, C.c_double, C.c_int64, C.c_int, vp,
+                                 C.c_int64, vp, C.c_int64, C.POINTER(SimResult), vp]
+    L.fl_adj_backward.argtypes = [vp, fp, sp, C.c_double, C.c_double, C.c_int64, C.c_int, vp,
+                                  C.c_int64, vp, C.c_int64, C.POINTER(C.c_double), vp]
+    L.fl_split_step_vjp.argtypes = [vp, fp, fp, fp, sp, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), vp, C.c_int64, vp]
     for name in EXPORTED:
         if name in _SIZE_RETURNING:
             getattr(L, name).restype = C.c_int64

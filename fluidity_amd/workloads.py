@@ -409,3 +409,71 @@ def aos_to_soa(U: np.ndarray) -> np.ndarray:
 
 def soa_to_aos(P: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(np.moveaxis(P, 0, -1))
+
+
+def _bump_sum(rng: SplitMix64, n: int, count: int = 16, amp=(-0.3, 0.3), width=(0.02, 0.08)):
+    """sum of `count` Gaussian bumps A exp(-|x - c|^2 / w^2) on the periodic
+    unit square (minimum-image distance), centres uniform, n x n cells."""
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    f = np.zeros((n, n))
+    for _ in range(count):
+        a = rng.uniform(*amp)
+        w = rng.uniform(*width)
+        cx = rng.uniform(0.0, 1.0)
+        cy = rng.uniform(0.0, 1.0)
+        dx = X - cx
+        dx -= np.round(dx)
+        dy = Y - cy
+        dy -= np.round(dy)
+        f += a * np.exp(-(dx * dx + dy * dy) / (w * w))
+    return f
+
+
+@dataclass
+class AdjointProblem:
+    U: np.ndarray        # (n, n, 4) conservative AoS initial state
+    target: np.ndarray   # (n, n) target density
+    gamma: float
+    dh: float
+
+
+def adjoint_problem(n: int = 1024, seed: int = 1004, target_seed: int = 1005,
+                    gamma: float = 1.4, flow: bool = False) -> AdjointProblem:
+    """Density 1 + 16 Gaussian bumps (seed), u = v = 0, p = 1 on the periodic
+    unit square; E = p / (gamma - 1) exactly, so the state is an exact
+    steady contact.  The target density is a second bump set (target_seed).
+    flow=True adds the smooth background flow u = 0.5 + 0.1 sin(2 pi y),
+    v = 0.3 + 0.1 sin(2 pi x) and p = 1 + 0.1 cos(2 pi x) cos(2 pi y): the
+    same bumps, but a state at which the scheme is differentiable (no face on
+    a branch point), for finite-difference checks."""
+    rho = 1.0 + _bump_sum(SplitMix64(seed), n)
+    target = 1.0 + _bump_sum(SplitMix64(target_seed), n)
+    U = np.zeros((n, n, 4))
+    U[..., 0] = rho
+    if not flow:
+        U[..., 1] = 1.0 / (gamma - 1.0)
+    else:
+        x = (np.arange(n) + 0.5) / n
+        X, Y = np.meshgrid(x, x)
+        W = np.empty((n, n, 4))
+        W[..., 0] = rho
+        W[..., 1] = 1.0 + 0.1 * np.cos(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+        W[..., 2] = 0.5 + 0.1 * np.sin(2 * np.pi * Y)
+        W[..., 3] = 0.3 + 0.1 * np.sin(2 * np.pi * X)
+        U = prim_to_cons(W, gamma)
+    return AdjointProblem(np.ascontiguousarray(U), np.ascontiguousarray(target), gamma, 1.0 / n)
+
+
+def smooth_directions(n: int, count: int = 64, seed: int = 1006):
+    """`count` random smooth perturbation directions of (rho, rho u, rho v, E)
+    on an n x n periodic grid (each field a sum of 4 Gaussian bumps of width
+    0.05..0.15) plus a gamma component, each normalised to unit RMS."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        d = np.stack([_bump_sum(rng, n, 4, (-1.0, 1.0), (0.05, 0.15)) for _ in range(4)], -1)
+        dg = rng.uniform(-1.0, 1.0)
+        scale = np.sqrt((d * d).mean() + dg * dg / d.size)
+        out.append((np.ascontiguousarray(d / scale), dg / scale))
+    return out

@@ -192,6 +192,44 @@ int fl_batch_finish(fl_batch* batch, fl_sim_result* results, void* stream);
  * the 4th 0 for 1D) of the state buffers; call after fl_batch_finish. */
 int fl_batch_conservation_sums(fl_batch* batch, double* sums, void* stream);
 
+/* ---- adjoint of the split step ------------------------------------------------
+ * Hand-written reverse mode of the FP64 split MUSCL-Hancock + HLLC step
+ * (conservative-form limiting, any limiter and boundary), including the
+ * dependence of dt = cfl dh / max wavespeed on the state and on gamma.
+ * Branches follow the primal (the derivative of the branch taken), the
+ * flattening factor chi is held constant.  The reference has no adjoint; these
+ * entry points differentiate its update call
+ * (solver/split_solver.hpp:105-162 inside simulator/simulator_impl.hpp:222-258).
+ * Adjoint fields use the state's layout. */
+
+/* Bytes of the checkpoint buffer of a run of `steps` steps with a checkpoint
+ * every `checkpoint_every` steps, and of the scratch of one segment. */
+int64_t fl_adj_checkpoint_bytes(const fl_field* state, int64_t steps, int checkpoint_every);
+int64_t fl_adj_work_bytes(const fl_field* state, int checkpoint_every);
+
+/* Runs `steps` steps from `state` (the result is left in `state`) and stores
+ * the state of every checkpoint_every-th step in `checkpoints`.  Synchronises
+ * once per segment; result->run_time is the accumulated simulated time. */
+int fl_adj_forward(fl_context* ctx, fl_field* state, const fl_scheme* scheme, double cfl,
+                   double dh, int64_t steps, int checkpoint_every, void* checkpoints,
+                   int64_t ckpt_bytes, void* work, int64_t work_bytes, fl_sim_result* result,
+                   void* stream);
+
+/* Given dLoss/d(final state) in `grad`, replaces it with dLoss/d(initial
+ * state) and returns dLoss/dgamma, recomputing each segment's steps from its
+ * checkpoint (memory: one segment, not the whole run).  Synchronises. */
+int fl_adj_backward(fl_context* ctx, fl_field* grad, const fl_scheme* scheme, double cfl,
+                    double dh, int64_t steps, int checkpoint_every, const void* checkpoints,
+                    int64_t ckpt_bytes, void* work, int64_t work_bytes, double* dgamma,
+                    void* stream);
+
+/* Vector-Jacobian product of one step U_{n+1} = step(U_n):
+ * lambda = (dU_{n+1}/dU_n)^T lambda_next, dgamma = lambda_next . dU_{n+1}/dgamma.
+ * work: fl_adj_work_bytes(state, 1).  Synchronises. */
+int fl_split_step_vjp(fl_context* ctx, const fl_field* state, const fl_field* lambda_next,
+                      fl_field* lambda, const fl_scheme* scheme, double cfl, double dh,
+                      double* dgamma, void* work, int64_t work_bytes, void* stream);
+
 /* ---- level set and Riemann ghost fluid -------------------------------------
  * Level-set fields phi are FP64 device arrays of n[0] (1D) or n[0]*n[1] (2D)
  * values, x fastest, interior cells only; phi <= 0 inside the material.  State
