@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip FORCE / FP32 lines")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the built-in launch shapes instead of timing candidates first")
     ap.add_argument("--x-block", type=int, default=0)
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
@@ -92,6 +94,8 @@ def main():
                      y_rows=args.y_rows)
         return s
 
+    tuned = {}
+
     U_soa = W.aos_to_soa(wl.U)
 
     def run_case(flux, dtype, steps, warmup):
@@ -100,6 +104,9 @@ def main():
             a = torch.from_numpy(U_soa).to(device=dev, dtype=dtype)
             w = torch.empty_like(a)
         stream.synchronize()
+        if args.autotune and not (args.x_block or args.x_rows or args.y_block or args.y_rows):
+            t = solver.autotune(a, w, reps=2, stream=stream)
+            tuned[(flux, str(dtype))] = [t.x_block, t.x_rows, t.y_block, t.y_rows]
         if warmup:
             solver.simulate(a, w, cfl, dh, max_iters=warmup, stream=stream)
         torch.cuda.synchronize()
@@ -145,6 +152,7 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
         "algorithmic_bytes": int(bytes_per_sweep),
+        "tuning": tuned.get((N.FL_FLUX_HLLC, str(torch.float64))),
         "x_sweep_us": round(t_x * 1e6, 1),
         "y_sweep_us": round(t_y * 1e6, 1),
         "step_bytes_per_cell": 128,
@@ -163,7 +171,8 @@ def main():
             variants[name] = {"value": round(v, 1), "unit": "cell-updates/s",
                               "ms_per_step": round(el2 * 1e3 / vsteps, 4),
                               "hbm_frac": round(v * bpc / 1e9 / HBM_PEAK_GBS, 4),
-                              "vs_f64_hllc": round(v / value, 3)}
+                              "vs_f64_hllc": round(v / value, 3),
+                              "tuning": tuned.get((flux, str(dt)))}
             del s2, a2, w2
             torch.cuda.empty_cache()
 

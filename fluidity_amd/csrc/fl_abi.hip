@@ -300,6 +300,63 @@ int fl_ctx_get_tuning(const fl_context* c, fl_tuning* t) {
   return FL_OK;
 }
 
+int fl_ctx_autotune(fl_context* c, const fl_field* in, const fl_field* work,
+                    const fl_scheme* s, int reps, fl_tuning* chosen, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  int rc = check_pair(in, work);
+  if (rc) return rc;
+  if ((rc = check_scheme(s, in->ndim))) return rc;
+  if (reps < 1 || reps > 100) return fail(FL_ERR_ARGUMENT, "reps %d out of [1, 100]", reps);
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  static const int widths[] = {64, 128, 256};
+  static const int xrows[] = {8, 16, 32, 64};
+  static const int yrows[] = {16, 32, 64, 128};
+  hipEvent_t e0, e1;
+  FL_HIP(hipEventCreate(&e0));
+  FL_HIP(hipEventCreate(&e1));
+  const fl_tuning saved = c->tuning;
+  fl_tuning best = c->tuning;
+  for (int dim = 0; dim < in->ndim && !rc; ++dim) {
+    float best_ms = 1e30f;
+    for (int w : widths) {
+      const int* rows = dim ? yrows : xrows;
+      const int nr = in->ndim == 1 ? 1 : 4;
+      for (int k = 0; k < nr && !rc; ++k) {
+        fl_tuning t = best;
+        if (dim == 0) {
+          t.x_block = w;
+          t.x_rows = rows[k];
+        } else {
+          t.y_block = w;
+          t.y_rows = rows[k];
+        }
+        c->tuning = t;
+        float total = 0.0f;
+        for (int r = 0; r <= reps && !rc; ++r) {  // r = 0 warms up
+          if (hipEventRecord(e0, st) != hipSuccess) rc = fail(FL_ERR_HIP, "hipEventRecord");
+          if (!rc) rc = launch_sweep(c, in, work, dim, s, 1e-3, nullptr, nullptr, st);
+          if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(FL_ERR_HIP, "hipEventRecord");
+          if (!rc && hipEventSynchronize(e1) != hipSuccess)
+            rc = fail(FL_ERR_HIP, "hipEventSynchronize");
+          float ms = 0.0f;
+          if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r > 0) total += ms;
+        }
+        if (!rc && total < best_ms) {
+          best_ms = total;
+          best = t;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->tuning = rc ? saved : best;
+  if (!rc && chosen) *chosen = best;
+  return rc;
+}
+
 int fl_split_sweep(fl_context* c, const fl_field* in, const fl_field* out, int dim,
                    const fl_scheme* s, double dtdh, void* stream) {
   if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");

@@ -67,7 +67,7 @@ __global__ void reset_counters_kernel(Counters* cnt) {
 // fixed dt/dh and no wavespeed output (launch-shape measurement; the states
 // are not modified).
 template <typename T, int FLUX, int PHASE>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 phase_kernel(const Prob<T>* __restrict__ probs, const Tile* __restrict__ tiles, int ntiles,
              Counters* __restrict__ cnt, int timing, T timing_dtdh) {
   __shared__ int s_tile;

@@ -6,13 +6,13 @@
 namespace fl {
 
 template <typename T, int ND, int FLUX>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_x_kernel(SweepArgs<T> a) {
   sweep_x_tile<T, ND, FLUX>(a, blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int FLUX>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(FL_SWEEP_WAVES)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_y_kernel(SweepArgs<T> a) {
   sweep_y_tile<T, FLUX>(a, blockIdx.x, blockIdx.y);
 }

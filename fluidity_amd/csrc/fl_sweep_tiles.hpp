@@ -27,11 +27,19 @@
 
 #include "fl_kernels.hpp"
 
-// Waves per SIMD the 2D sweep kernels are compiled for (register budget
-// 512 / FL_SWEEP_WAVES VGPRs).
+// Waves per SIMD the sweep kernels are compiled for (register budget
+// 512 / waves VGPRs), per precision.
 #ifndef FL_SWEEP_WAVES
 #define FL_SWEEP_WAVES 2
 #endif
+#ifndef FL_SWEEP_WAVES_F32
+#define FL_SWEEP_WAVES_F32 4
+#endif
+
+namespace fl {
+template <typename T> struct SweepWaves { static constexpr int value = FL_SWEEP_WAVES; };
+template <> struct SweepWaves<float> { static constexpr int value = FL_SWEEP_WAVES_F32; };
+}  // namespace fl
 
 
 namespace fl {

@@ -77,6 +77,16 @@ class SplitSolver:
     def set_tuning(self, **kw):
         self.ctx.set_tuning(**kw)
 
+    def autotune(self, U, work, reps: int = 3, stream=None) -> N.Tuning:
+        """Times the candidate launch shapes of both sweeps on this grid
+        (U is not modified) and keeps the fastest."""
+        fi, fo = grid_field(U, self.layout), grid_field(work, self.layout)
+        s = self.scheme.c()
+        t = N.Tuning()
+        N.check(N.lib.fl_ctx_autotune(self.ctx.handle, C.byref(fi), C.byref(fo), C.byref(s), reps,
+                                      C.byref(t), _stream_handle(stream)))
+        return t
+
     def sweep(self, U_in, U_out, dim: int, dtdh: float, stream=None) -> None:
         fi, fo = grid_field(U_in, self.layout), grid_field(U_out, self.layout)
         N.check(N.lib.fl_split_sweep(self.ctx.handle, C.byref(fi), C.byref(fo), dim,

@@ -173,7 +173,11 @@ def _fd_check(A, E, prob, dirs, steps, eps):
 def test_finite_differences_with_flow(A, E):
     prob = W.adjoint_problem(1024, flow=True)
     dirs = W.smooth_directions(1024, 64, seed=1006)
-    errs = _fd_check(A, E, prob, dirs, 200, 1e-6)
+    # The scheme is piecewise smooth (limiter and HLLC branches): the central
+    # difference error grows linearly with the step while branches switch
+    # inside it (measured max over the 64 directions: 9.6e-6 at 1e-6,
+    # 1.6e-7 at 1e-7, 4.6e-8 at 1e-8), so the step is 1e-8.
+    errs = _fd_check(A, E, prob, dirs, 200, 1e-8)
     print("FD vs adjoint, 64 directions: max rel err %.3g, median %.3g" %
           (max(errs), float(np.median(errs))))
     assert max(errs) <= 1e-6
@@ -189,6 +193,6 @@ def test_finite_differences_static_rho_gamma(A, E):
         rho_only[..., 0] = d[..., 0]
         dirs.append((rho_only, 0.0))
     dirs.append((np.zeros_like(full[0][0]), 1.0))
-    errs = _fd_check(A, E, prob, dirs, 200, 1e-6)
+    errs = _fd_check(A, E, prob, dirs, 200, 1e-8)
     print("static workload, rho/gamma directions: rel errs", ["%.3g" % e for e in errs])
     assert max(errs[:4]) <= 1e-6
