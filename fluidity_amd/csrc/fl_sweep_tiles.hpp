@@ -36,9 +36,20 @@
 #define FL_SWEEP_WAVES_F32 4
 #endif
 
+// Rows each lane keeps in flight ahead of the row it computes, per
+// precision (the FP32 kernels run twice the waves in half the registers).
+#ifndef FL_PREFETCH
+#define FL_PREFETCH 2
+#endif
+#ifndef FL_PREFETCH_F32
+#define FL_PREFETCH_F32 2
+#endif
+
 namespace fl {
 template <typename T> struct SweepWaves { static constexpr int value = FL_SWEEP_WAVES; };
 template <> struct SweepWaves<float> { static constexpr int value = FL_SWEEP_WAVES_F32; };
+template <typename T> struct SweepPrefetch { static constexpr int value = FL_PREFETCH; };
+template <> struct SweepPrefetch<float> { static constexpr int value = FL_PREFETCH_F32; };
 }  // namespace fl
 
 
@@ -215,30 +226,31 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   const int64_t rbeg = (ND == 2) ? y0 - 3 : 0;
   const int64_t rlast = (ND == 2) ? yend + 2 : 0;
 
-  // Software pipelining with two row buffers in fixed registers (A for even,
-  // B for odd steps of the unrolled loop): the loads of row r+2 and of the
-  // halo cells of the output row two steps ahead are issued while row r is
-  // computed, without ever copying a register whose load is still in flight.
+  // Software pipelining with PF row buffers in fixed registers: the loads of
+  // row r+PF and of the halo cells of the output row PF steps ahead are issued
+  // while row r is computed, without ever copying a register whose load is
+  // still in flight.
   // All loads are unconditional (rows past the end are re-loaded, clamped), so
   // the compiler can count the loads issued after each one and wait only for
   // the one it needs.
   auto hrow = [&](int64_t r) { return (ND == 2) ? r - 3 : int64_t(0); };
-  Pending<T, S> QA, QB, HA, HB;
-  issue_load<T, ND>(a, c, rbeg, QA);
-  issue_load<T, ND>(a, c, min(rbeg + 1, rlast), QB);
-  issue_load<T, ND>(a, hcol, hrow(rbeg), HA);
-  issue_load<T, ND>(a, hcol, hrow(min(rbeg + 1, rlast)), HB);
-
-  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
-  // two-step loop is fully unrolled so that each copy names its buffers
-  // statically.
-  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
+  constexpr int PF = SweepPrefetch<T>::value;
+  Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
-  for (int step = 0; step < 2; ++step) {
+  for (int i = 0; i < PF; ++i) {
+    issue_load<T, ND>(a, c, min(rbeg + i, rlast), Qs[i]);
+    issue_load<T, ND>(a, hcol, hrow(min(rbeg + i, rlast)), Hs[i]);
+  }
+
+  // Step i of each group of PF rows owns buffer i.  The inner loop is fully
+  // unrolled so that each copy names its buffers statically.
+  for (int64_t r0 = rbeg; r0 <= rlast; r0 += PF) {
+#pragma unroll
+  for (int step = 0; step < PF; ++step) {
     const int64_t r = r0 + step;
     if (r > rlast) break;
-    Pending<T, S>& Q = step ? QB : QA;
-    Pending<T, S>& H = step ? HB : HA;
+    Pending<T, S>& Q = Qs[step];
+    Pending<T, S>& H = Hs[step];
     // ---- the newest row of this lane's column and its primitives
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -250,10 +262,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
     take<T, ND>(Q, Uw[3]);
-    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
+    issue_load<T, ND>(a, c, min(r + PF, rlast), Q);
     T uh[S];  // halo cell of output row hrow(r)
     take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, hrow(min(r + 2, rlast)), H);
+    issue_load<T, ND>(a, hcol, hrow(min(r + PF, rlast)), H);
     {
       const Prim<T, ND> w = prim_of<T, ND>(Uw[3], a.mat);
       pw[4] = w.p;
@@ -393,26 +405,26 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
   for (int k = 0; k < S; ++k) { Rprev[k] = T(0); Gprev[k] = T(0); }
 
   unsigned long long wsmax = 0ull;
-  // Software pipelining with two row buffers in fixed registers (see the x
-  // sweep): rows r+2 (and their halo cells) load while row r is computed.
+  // Software pipelining with PF row buffers in fixed registers (see the x
+  // sweep): rows r+PF (and their halo cells) load while row r is computed.
   const int64_t rbeg = y0 - 4, rlast = yend + 3;
   // All loads are unconditional (see the x sweep); the halo cells of row r are
   // loaded on the same schedule as the row itself.
-  Pending<T, S> QA, QB, HA, HB;
-  issue_load<T, ND>(a, c, rbeg, QA);
-  issue_load<T, ND>(a, c, rbeg + 1, QB);
-  issue_load<T, ND>(a, hcol, rbeg, HA);
-  issue_load<T, ND>(a, hcol, rbeg + 1, HB);
-  // Even steps (r - rbeg even) own buffers A, odd steps B.  The inner
-  // two-step loop is fully unrolled so that each copy names its buffers
-  // statically.
-  for (int64_t r0 = rbeg; r0 <= rlast; r0 += 2) {
+  constexpr int PF = SweepPrefetch<T>::value;
+  Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
-  for (int step = 0; step < 2; ++step) {
+  for (int i = 0; i < PF; ++i) {
+    issue_load<T, ND>(a, c, min(rbeg + i, rlast), Qs[i]);
+    issue_load<T, ND>(a, hcol, min(rbeg + i, rlast), Hs[i]);
+  }
+  // Step i of each group of PF rows owns buffer i (see the x sweep).
+  for (int64_t r0 = rbeg; r0 <= rlast; r0 += PF) {
+#pragma unroll
+  for (int step = 0; step < PF; ++step) {
     const int64_t r = r0 + step;
     if (r > rlast) break;
-    Pending<T, S>& Q = step ? QB : QA;
-    Pending<T, S>& H = step ? HB : HA;
+    Pending<T, S>& Q = Qs[step];
+    Pending<T, S>& H = Hs[step];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -425,10 +437,10 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       cxw[i] = cxw[i + 1];
     }
     take<T, ND>(Q, Uw[4]);
-    issue_load<T, ND>(a, c, min(r + 2, rlast), Q);
+    issue_load<T, ND>(a, c, min(r + PF, rlast), Q);
     T uh[S];
     take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, min(r + 2, rlast), H);
+    issue_load<T, ND>(a, hcol, min(r + PF, rlast), H);
     const Prim<T, ND> w = prim_of<T, ND>(Uw[4], a.mat);
     pw[4] = w.p;
     vyw[3] = w.vel[1];
