@@ -255,6 +255,8 @@ int fl_ctx_create(int device, fl_context** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->partial, sizeof(double) * kSumBlocks * 4);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_ctl, sizeof(fl::StepCtl), 0);
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_partial, sizeof(double) * kSumBlocks * 4, 0);
+  if (e == hipSuccess) e = hipStreamCreate(&c->cap_stream);
+  c->use_graphs = 1;
   if (e == hipSuccess) e = hipMalloc((void**)&c->eik, sizeof(fl::ls::EikCtl));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, sizeof(fl::ls::EikCtl), 0);
   if (e != hipSuccess) {
@@ -273,6 +275,9 @@ int fl_ctx_destroy(fl_context* c) {
   if (c->partial) (void)hipFree(c->partial);
   if (c->host_ctl) (void)hipHostFree(c->host_ctl);
   if (c->host_partial) (void)hipHostFree(c->host_partial);
+  if (c->sim_exec) (void)hipGraphExecDestroy(c->sim_exec);
+  if (c->sim_graph) (void)hipGraphDestroy(c->sim_graph);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->eik) (void)hipFree(c->eik);
   if (c->host_eik) (void)hipHostFree(c->host_eik);
   delete c;
@@ -411,6 +416,12 @@ int fl_max_wavespeed(fl_context* c, const fl_field* in, const fl_scheme* s, doub
   return FL_OK;
 }
 
+int fl_ctx_set_graphs(fl_context* c, int enable) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  c->use_graphs = enable ? 1 : 0;
+  return FL_OK;
+}
+
 int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme* s,
                 const fl_sim_params* p, fl_sim_result* result, void* stream) {
   if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
@@ -431,27 +442,70 @@ int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme*
   if (rc) return rc;
 
   // Steps are launched in batches; between batches the host reads the control
-  // block once to stop when the loop condition fails.
+  // block once to stop when the loop condition fails.  With graphs on, a pair
+  // of steps (both parities) is captured once and replayed; a step past the
+  // stop condition does nothing, so an odd batch may run one step more.
   const int64_t kBatch = 256;
   int64_t launched = 0;
   fl_field a = *state, b = *work;
+  auto one_step = [&](int parity, hipStream_t sx) -> int {
+    hipLaunchKernelGGL(fl::step_control_kernel, dim3(1), dim3(1), 0, sx, c->ctl, parity,
+                       p->cfl, p->dh, p->sim_time, (long long)p->max_iters);
+    unsigned long long* ws_next = &c->ctl->ws_bits[parity ^ 1];
+    if (state->ndim == 1) {
+      const fl_field* src = parity ? &b : &a;
+      const fl_field* dst = parity ? &a : &b;
+      return launch_sweep(c, src, dst, 0, s, 1.0, c->ctl, ws_next, sx);
+    }
+    int r = launch_sweep(c, &a, &b, 0, s, 1.0, c->ctl, nullptr, sx);
+    if (!r) r = launch_sweep(c, &b, &a, 1, s, 1.0, c->ctl, ws_next, sx);
+    return r;
+  };
+  if (c->use_graphs) {
+    SimGraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.a = state->data;
+    key.b = work->data;
+    key.dtype = state->dtype;
+    key.layout = state->layout;
+    key.ndim = state->ndim;
+    for (int d = 0; d < 3; ++d) key.n[d] = state->n[d];
+    key.scheme = *s;
+    key.cfl = p->cfl;
+    key.dh = p->dh;
+    key.sim_time = p->sim_time;
+    key.max_iters = p->max_iters;
+    key.tuning = c->tuning;
+    if (!c->sim_exec || memcmp(&key, &c->sim_key, sizeof(key)) != 0) {
+      if (c->sim_exec) (void)hipGraphExecDestroy(c->sim_exec);
+      if (c->sim_graph) (void)hipGraphDestroy(c->sim_graph);
+      c->sim_exec = nullptr;
+      c->sim_graph = nullptr;
+      FL_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+      rc = one_step(0, c->cap_stream);
+      if (!rc) rc = one_step(1, c->cap_stream);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->cap_stream, &gr);
+      if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+      }
+      if (e != hipSuccess)
+        return fail(FL_ERR_HIP, "step graph capture failed: %s", hipGetErrorString(e));
+      c->sim_graph = gr;
+      FL_HIP(hipGraphInstantiate(&c->sim_exec, c->sim_graph, nullptr, nullptr, 0));
+      c->sim_key = key;
+    }
+  }
   while (launched < p->max_iters) {
     const int64_t nb = (p->max_iters - launched) < kBatch ? (p->max_iters - launched) : kBatch;
-    for (int64_t k = 0; k < nb; ++k) {
-      const int64_t step = launched + k;
-      const int parity = (int)(step & 1);
-      hipLaunchKernelGGL(fl::step_control_kernel, dim3(1), dim3(1), 0, st, c->ctl, parity,
-                         p->cfl, p->dh, p->sim_time, (long long)p->max_iters);
-      unsigned long long* ws_next = &c->ctl->ws_bits[parity ^ 1];
-      if (state->ndim == 1) {
-        const fl_field* src = (step & 1) ? &b : &a;
-        const fl_field* dst = (step & 1) ? &a : &b;
-        rc = launch_sweep(c, src, dst, 0, s, 1.0, c->ctl, ws_next, st);
-      } else {
-        rc = launch_sweep(c, &a, &b, 0, s, 1.0, c->ctl, nullptr, st);
-        if (!rc) rc = launch_sweep(c, &b, &a, 1, s, 1.0, c->ctl, ws_next, st);
+    if (c->use_graphs) {
+      for (int64_t k = 0; k < nb; k += 2) FL_HIP(hipGraphLaunch(c->sim_exec, st));
+    } else {
+      for (int64_t k = 0; k < nb; ++k) {
+        rc = one_step((int)((launched + k) & 1), st);
+        if (rc) return rc;
       }
-      if (rc) return rc;
     }
     launched += nb;
     FL_HIP(hipMemcpyAsync(c->host_ctl, c->ctl, sizeof(fl::StepCtl), hipMemcpyDeviceToHost, st));

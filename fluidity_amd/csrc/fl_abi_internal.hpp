@@ -11,6 +11,18 @@
 #include "fl_kernels.hpp"
 #include "fl_levelset.hpp"
 
+// What a captured fl_simulate step pair depends on (compared bytewise).
+struct SimGraphKey {
+  void* a;
+  void* b;
+  int32_t dtype, layout, ndim, pad;
+  int64_t n[3];
+  fl_scheme scheme;
+  double cfl, dh, sim_time;
+  int64_t max_iters;
+  fl_tuning tuning;
+};
+
 struct fl_context {
   int device;
   fl::StepCtl* ctl;            // device control block of fl_simulate
@@ -21,6 +33,11 @@ struct fl_context {
   fl::ls::EikCtl* eik;         // device control words of the eikonal loop
   fl::ls::EikCtl* host_eik;    // pinned host mirror of eik
   fl_tuning tuning;
+  int use_graphs;              // fl_simulate replays a captured pair of steps
+  hipStream_t cap_stream;      // capture stream of that graph
+  hipGraph_t sim_graph;
+  hipGraphExec_t sim_exec;
+  SimGraphKey sim_key;
 };
 
 namespace fl {

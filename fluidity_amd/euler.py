@@ -77,6 +77,11 @@ class SplitSolver:
     def set_tuning(self, **kw):
         self.ctx.set_tuning(**kw)
 
+    def set_graphs(self, enable: bool) -> None:
+        """simulate() replays a hipGraph of a pair of steps (default) or
+        launches every kernel."""
+        N.check(N.lib.fl_ctx_set_graphs(self.ctx.handle, 1 if enable else 0))
+
     def autotune(self, U, work, reps: int = 3, stream=None) -> N.Tuning:
         """Times the candidate launch shapes of both sweeps on this grid
         (U is not modified) and keeps the fastest."""

@@ -26,7 +26,8 @@ FL_BC_TRANSMISSIVE, FL_BC_REFLECTIVE, FL_BC_PERIODIC = 0, 1, 2
 
 EXPORTED = [
     "fl_status_string", "fl_last_error", "fl_version", "fl_ctx_create", "fl_ctx_destroy",
-    "fl_ctx_set_tuning", "fl_ctx_get_tuning", "fl_ctx_autotune", "fl_split_sweep",
+    "fl_ctx_set_tuning", "fl_ctx_get_tuning", "fl_ctx_autotune", "fl_ctx_set_graphs",
+    "fl_split_sweep",
     "fl_split_solve",
     "fl_max_wavespeed", "fl_simulate", "fl_conservation_sums",
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
@@ -148,6 +149,7 @@ def _load():
     L.fl_ctx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.fl_ctx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.fl_ctx_autotune.argtypes = [vp, fp, fp, sp, C.c_int, C.POINTER(Tuning), vp]
+    L.fl_ctx_set_graphs.argtypes = [vp, C.c_int]
     L.fl_split_sweep.argtypes = [vp, fp, fp, C.c_int, sp, C.c_double, vp]
     L.fl_split_solve.argtypes = [vp, fp, fp, sp, C.c_double, vp]
     L.fl_max_wavespeed.argtypes = [vp, fp, sp, C.POINTER(C.c_double), vp]

@@ -146,6 +146,11 @@ int fl_max_wavespeed(fl_context* ctx, const fl_field* in, const fl_scheme* schem
 int fl_simulate(fl_context* ctx, fl_field* state, fl_field* work, const fl_scheme* scheme,
                 const fl_sim_params* params, fl_sim_result* result, void* stream);
 
+/* fl_simulate captures a pair of steps (control kernel + sweeps, both
+ * parities) in a hipGraph on first use for a given grid, scheme, parameters
+ * and tuning, and replays it (default); enable = 0 launches every kernel. */
+int fl_ctx_set_graphs(fl_context* ctx, int enable);
+
 /* FP64 sums of each conserved component over the grid (synchronises
  * `stream`); sums has 2+ndim entries.  Conservation diagnostics beside the
  * FP32 bulk path. */

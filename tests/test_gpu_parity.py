@@ -228,3 +228,20 @@ def test_vacuum_reports_numeric_error(E, N):
     with pytest.raises(N.FluidityError) as ei:
         solver.simulate(a, w, 0.9, 1.0 / n, max_iters=50)
     assert ei.value.status == N.FL_ERR_NUMERIC
+
+
+@pytest.mark.parametrize("nd,steps", [(1, 37), (2, 13)])
+def test_simulate_graph_replay_equals_launches(E, nd, steps):
+    """The hipGraph replay of step pairs gives the same state, step count and
+    run time as launching every kernel, also for an odd number of steps."""
+    U = cons_field("smooth", (97,) if nd == 1 else (40, 52), 0)
+    res = []
+    for graphs in (True, False):
+        solver = E.SplitSolver(E.Scheme(), layout=1)
+        solver.set_graphs(graphs)
+        a = _dev(U, layout=1)
+        r = solver.simulate(a, torch.empty_like(a), 0.9, 1.0 / U.shape[-2], max_iters=steps)
+        res.append((_host(a, layout=1), r.iters, r.run_time, r.last_dt))
+    assert res[0][1] == res[1][1] == steps
+    assert res[0][2] == res[1][2] and res[0][3] == res[1][3]
+    assert np.array_equal(res[0][0].view(np.uint64), res[1][0].view(np.uint64))
