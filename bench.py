@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
     ap.add_argument("--y-rows", type=int, default=0)
-    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost", "batch"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost", "batch", "adjoint"],
+                    default="bench",
                     help="sweeps: a plain device copy of one grid (HBM calibration for "
                          "counter runs) followed by --steps x sweeps and y sweeps")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
@@ -69,6 +70,10 @@ def main():
     from fluidity_amd import native as N
     from fluidity_amd import workloads as W
 
+    if args.mode == "adjoint":
+        stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+        print(json.dumps(adjoint_case(torch, N, W, stream, steps=args.steps)), flush=True)
+        return
     if args.mode == "batch":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
         print(json.dumps(batched_case(torch, N, W, stream, args.steps)), flush=True)
@@ -177,6 +182,7 @@ def main():
             torch.cuda.empty_cache()
 
         variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
+        variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
         bf = batched_case(torch, N, W, stream, steps=20)
         for k, v in bf.items():
             v["vs_f64_hllc"] = round(v["value"] / value, 3)
@@ -207,6 +213,44 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def adjoint_case(torch, N, W, stream, steps=200, every=20, n=1024):
+    """Gradient of the density misfit after 200 steps of the 1024^2 periodic
+    bump workload (seed 1004, target seed 1005) with respect to the initial
+    state and gamma: the checkpointed forward run (a checkpoint every 20
+    steps) and the hand-written adjoint with per-segment recompute, FP64.
+    `value` counts adjoint cell-steps (cells x steps) per second of the
+    backward pass, which includes the recompute."""
+    from fluidity_amd import adjoint as AJ
+    from fluidity_amd import euler as E
+    prob = W.adjoint_problem(n)
+    adj = AJ.Adjoint(E.Scheme(gamma=prob.gamma, bc=(2, 2, 0)), 0.9, prob.dh)
+    dev = torch.device("cuda:0")
+    with torch.cuda.stream(stream):
+        U = torch.from_numpy(np.ascontiguousarray(np.moveaxis(prob.U, -1, 0))).to(dev)
+        tgt = torch.from_numpy(prob.target).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = adj.forward(U, steps, every, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        grad = torch.zeros_like(U)
+        grad[0] = 2.0 * prob.dh * prob.dh * (U[0] - tgt)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    dgamma = adj.backward(grad, run, stream=stream)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    cells = n * n
+    return {"value": round(steps * cells / (t3 - t2), 1), "unit": "adjoint cell-steps/s",
+            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t3 - t2) * 1e3, 2),
+            "steps": steps, "checkpoint_every": every,
+            "checkpoint_bytes": int(run["checkpoints"].numel()),
+            "dloss_dgamma": dgamma,
+            "workload": f"2D Euler {n}x{n} periodic, 16 density bumps (seed 1004), u=v=0, p=1, "
+                        "HLLC + van Leer, CFL 0.9, FP64"}
 
 
 def batched_case(torch, N, W, stream, steps):

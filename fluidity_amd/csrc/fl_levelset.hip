@@ -495,21 +495,35 @@ __global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* 
   has_star[i] = 1;
 }
 
+__device__ __forceinline__ bool in_band(double p, double upper) { return p > 0.0 && p <= upper; }
+
 // Extrapolation source field, in both Jacobi buffers: the material's own state
-// (AoS scratch), with the star state at interfacial cells.
+// (AoS scratch), with the star state at interfacial cells.  Only the cells an
+// extrapolation sweep can read are written: the band cells and their
+// neighbours (a band cell reads itself and its upwind neighbours).
 template <int ND>
-__global__ void gfm_seed_kernel(StateView um, const double* __restrict__ star,
+__global__ void gfm_seed_kernel(const double* __restrict__ phi_m, double upper, StateView um,
+                                const double* __restrict__ star,
                                 const uint8_t* __restrict__ has_star, Grid g,
                                 double* __restrict__ A, double* __restrict__ B) {
   constexpr int S = 2 + ND;
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  bool needed = in_band(phi_m[i], upper);
+  if (x > 0) needed |= in_band(phi_m[i - 1], upper);
+  if (x < g.nx - 1) needed |= in_band(phi_m[i + 1], upper);
+  if (ND > 1) {
+    if (y > 0) needed |= in_band(phi_m[i - g.nx], upper);
+    if (y < g.ny - 1) needed |= in_band(phi_m[i + g.nx], upper);
+  }
+  if (!needed) return;
   double u[S];
   if (has_star[i]) {
 #pragma unroll
     for (int k = 0; k < S; ++k) u[k] = star[i * S + k];
   } else {
-    load_state<S>(um, g, i % g.nx, i / g.nx, u);
+    load_state<S>(um, g, x, y, u);
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -518,16 +532,28 @@ __global__ void gfm_seed_kernel(StateView um, const double* __restrict__ star,
   }
 }
 
+// Indices of the band cells 0 < phi <= upper (in no particular order: each
+// band cell's update is independent of the others within a sweep).
+__global__ void gfm_band_kernel(const double* __restrict__ phi_m, Grid g, double upper,
+                                int* __restrict__ list, int* __restrict__ count) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  if (in_band(phi_m[i], upper)) list[atomicAdd(count, 1)] = (int)i;
+}
+
+__global__ void count_clear_kernel(int* count) { *count = 0; }
+
 // GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202),
 // one Jacobi sweep over the ghost band 0 < phi <= 6 dh / sqrt(2).
 template <int ND>
 __global__ void gfm_extrap_kernel(const double* __restrict__ phi_m, const double* __restrict__ A,
-                                  Grid g, double dh, double upper, double* __restrict__ B) {
+                                  Grid g, double dh, const int* __restrict__ list,
+                                  const int* __restrict__ count, double* __restrict__ B) {
   constexpr int S = 2 + ND;
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
+  const int n = *count;
+  for (int k = (int)gid(); k < n; k += gridDim.x * blockDim.x) {
+  const int64_t i = list[k];
   const double p = phi_m[i];
-  if (!(p > 0.0 && p <= upper)) return;  // both buffers hold the seed here
   const int64_t x = i % g.nx, y = i / g.nx;
   double nrm[2] = {0.0, 0.0};
   norm_at(phi_m, g, x, y, dh, nrm);
@@ -554,16 +580,18 @@ __global__ void gfm_extrap_kernel(const double* __restrict__ phi_m, const double
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) B[i * S + k] = (l1 > 1e-2) ? acc[k] / l1 : acc[k];
+  }
 }
 
 template <int ND>
-__global__ void gfm_writeback_kernel(const double* __restrict__ phi_m, const double* __restrict__ A,
-                                     Grid g, double upper, StateView um) {
+__global__ void gfm_writeback_kernel(const double* __restrict__ A, const int* __restrict__ list,
+                                     const int* __restrict__ count, StateView um) {
   constexpr int S = 2 + ND;
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
-  const double p = phi_m[i];
-  if (p > 0.0 && p <= upper) store_state<S>(um, i, A + i * S);
+  const int n = *count;
+  for (int k = (int)gid(); k < n; k += gridDim.x * blockDim.x) {
+    const int64_t i = list[k];
+    store_state<S>(um, i, A + i * S);
+  }
 }
 
 // combine_into_first (material/cuda/combine_materials.cuh:39-63): the first
@@ -586,12 +614,13 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
                                                uint8_t*);                                   \
-  template __global__ void gfm_seed_kernel<ND>(StateView, const double*, const uint8_t*,     \
-                                               Grid, double*, double*);                      \
+  template __global__ void gfm_seed_kernel<ND>(const double*, double, StateView,            \
+                                               const double*, const uint8_t*, Grid, double*, \
+                                               double*);                                    \
   template __global__ void gfm_extrap_kernel<ND>(const double*, const double*, Grid, double,  \
-                                                 double, double*);                           \
-  template __global__ void gfm_writeback_kernel<ND>(const double*, const double*, Grid,      \
-                                                    double, StateView);                      \
+                                                 const int*, const int*, double*);           \
+  template __global__ void gfm_writeback_kernel<ND>(const double*, const int*, const int*,   \
+                                                    StateView);                              \
   template __global__ void combine_kernel<ND>(const double*, StateView, StateView, Grid,     \
                                               StateView);
 

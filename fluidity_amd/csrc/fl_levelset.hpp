@@ -54,14 +54,18 @@ __global__ void gfm_star_kernel(const double* phi_m, const double* phi_o, StateV
                                 StateView uo, Eos em, Eos eo, Grid g, double dh, double* star,
                                 uint8_t* has_star);
 template <int ND>
-__global__ void gfm_seed_kernel(StateView um, const double* star, const uint8_t* has_star,
-                                Grid g, double* A, double* B);
+__global__ void gfm_seed_kernel(const double* phi_m, double upper, StateView um,
+                                const double* star, const uint8_t* has_star, Grid g, double* A,
+                                double* B);
+__global__ void gfm_band_kernel(const double* phi_m, Grid g, double upper, int* list,
+                                int* count);
+__global__ void count_clear_kernel(int* count);
 template <int ND>
 __global__ void gfm_extrap_kernel(const double* phi_m, const double* A, Grid g, double dh,
-                                  double upper, double* B);
+                                  const int* list, const int* count, double* B);
 template <int ND>
-__global__ void gfm_writeback_kernel(const double* phi_m, const double* A, Grid g,
-                                     double upper, StateView um);
+__global__ void gfm_writeback_kernel(const double* A, const int* list, const int* count,
+                                     StateView um);
 template <int ND>
 __global__ void combine_kernel(const double* phi0, StateView u0, StateView u1, Grid g,
                                StateView out);

@@ -174,26 +174,42 @@ int launch_star(const double* phi_m, const double* phi_o, const fl_field* sm,
   return FL_OK;
 }
 
+// Extrapolation scratch: two AoS Jacobi buffers, the band list and its count.
+int64_t extrap_work_bytes(const ls::Grid& g) {
+  const int64_t cells = g.nx * g.ny;
+  return 2 * align256(cells * (2 + g.nd) * 8) + align256(cells * 4) + 256;
+}
+
 template <int ND>
 void extrapolate_nd(const double* phi_m, ls::StateView um, const double* star,
-                    const uint8_t* has, const ls::Grid& g, double dh, int sweeps, double* work,
+                    const uint8_t* has, const ls::Grid& g, double dh, int sweeps, void* work,
                     hipStream_t st) {
   const int64_t cells = g.nx * g.ny;
   const dim3 grid = grid_for(cells);
-  double* buf[2] = {work, work + cells * (2 + ND)};
+  char* w = (char*)work;
+  const int64_t bb = align256(cells * (2 + ND) * 8);
+  double* buf[2] = {(double*)w, (double*)(w + bb)};
+  int* list = (int*)(w + 2 * bb);
+  int* count = (int*)(w + 2 * bb + align256(cells * 4));
   const double upper = 6.0 * dh * (1.0 / sqrt(2.0));
-  hipLaunchKernelGGL(ls::gfm_seed_kernel<ND>, grid, dim3(kBlock), 0, st, um, star, has, g,
-                     buf[0], buf[1]);
+  // band cells are a thin ring: a fixed grid walks the list
+  const dim3 lgrid(256);
+  hipLaunchKernelGGL(ls::count_clear_kernel, dim3(1), dim3(1), 0, st, count);
+  hipLaunchKernelGGL(ls::gfm_band_kernel, grid, dim3(kBlock), 0, st, phi_m, g, upper, list,
+                     count);
+  hipLaunchKernelGGL(ls::gfm_seed_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m, upper, um, star,
+                     has, g, buf[0], buf[1]);
   for (int k = 0; k < sweeps; ++k)
-    hipLaunchKernelGGL(ls::gfm_extrap_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m,
-                       (const double*)buf[k & 1], g, dh, upper, buf[(k + 1) & 1]);
-  hipLaunchKernelGGL(ls::gfm_writeback_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m,
-                     (const double*)buf[sweeps & 1], g, upper, um);
+    hipLaunchKernelGGL(ls::gfm_extrap_kernel<ND>, lgrid, dim3(kBlock), 0, st, phi_m,
+                       (const double*)buf[k & 1], g, dh, (const int*)list, (const int*)count,
+                       buf[(k + 1) & 1]);
+  hipLaunchKernelGGL(ls::gfm_writeback_kernel<ND>, lgrid, dim3(kBlock), 0, st,
+                     (const double*)buf[sweeps & 1], (const int*)list, (const int*)count, um);
 }
 
 int launch_extrapolate(const double* phi_m, const fl_field* sm, const double* star,
                        const uint8_t* has, const ls::Grid& g, double dh, int sweeps,
-                       double* work, hipStream_t st) {
+                       void* work, hipStream_t st) {
   if (g.nd == 1)
     extrapolate_nd<1>(phi_m, view_of(sm), star, has, g, dh, sweeps, work, st);
   else
@@ -218,7 +234,7 @@ int launch_combine(const double* phi0, const fl_field* s0, const fl_field* s1,
 // Scratch layout of fl_mm_simulate.
 struct MmWork {
   double* star[2];
-  double* extrap;   // 2 * cells * S
+  void* extrap;     // extrap_work_bytes
   double* V[2];
   double* adv;      // 2 * cells
   double* tphi[2];
@@ -240,7 +256,7 @@ MmWork mm_layout(void* base, const ls::Grid& g) {
   MmWork w;
   w.star[0] = (double*)take(cells * S * 8);
   w.star[1] = (double*)take(cells * S * 8);
-  w.extrap = (double*)take(2 * cells * S * 8);
+  w.extrap = take(extrap_work_bytes(g));
   w.V[0] = (double*)take(cells * 8);
   w.V[1] = (double*)take(cells * 8);
   w.adv = (double*)take(2 * cells * 8);
@@ -344,8 +360,14 @@ int fl_gfm_star(const double* phi_m, const double* phi_o, const fl_field* state_
                      (hipStream_t)stream);
 }
 
+int64_t fl_gfm_extrapolate_work_bytes(int ndim, const int64_t* n) {
+  ls::Grid g;
+  if (check_shape(ndim, n, &g)) return -1;
+  return extrap_work_bytes(g);
+}
+
 int fl_gfm_extrapolate(const double* phi_m, fl_field* state_m, const double* star,
-                       const uint8_t* has_star, double dh, int sweeps, double* work,
+                       const uint8_t* has_star, double dh, int sweeps, void* work,
                        void* stream) {
   if (!phi_m || !state_m || !star || !has_star || !work)
     return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
@@ -355,6 +377,7 @@ int fl_gfm_extrapolate(const double* phi_m, fl_field* state_m, const double* sta
   if ((rc = check_state(state_m, g, "state_m"))) return rc;
   if ((rc = check_dh(dh))) return rc;
   if (sweeps < 0 || sweeps > 4096) return fail(FL_ERR_ARGUMENT, "sweeps %d out of range", sweeps);
+  if ((uintptr_t)work & 255) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
   return launch_extrapolate(phi_m, state_m, star, has_star, g, dh, sweeps, work,
                             (hipStream_t)stream);
 }

@@ -124,8 +124,8 @@ def riemann_gfm_star(phi_m, phi_o, state_m, state_o, scheme_m: Scheme, scheme_o:
 def extrapolate(phi_m, state_m, star, has, dh: float, sweeps: int = 8,
                 layout: int = N.FL_LAYOUT_AOS, stream=None) -> None:
     """Fills the ghost band of state_m in place."""
-    S = star.shape[-1]
-    work = torch.empty(2 * phi_m.numel() * S, dtype=torch.float64, device=phi_m.device)
+    nd, n = _shape(phi_m)
+    work = _work(N.lib.fl_gfm_extrapolate_work_bytes(nd, n), phi_m.device)
     fm = grid_field(state_m, layout)
     N.check(N.lib.fl_gfm_extrapolate(phi_m.data_ptr(), C.byref(fm), star.data_ptr(),
                                      has.data_ptr(), dh, sweeps, work.data_ptr(),

@@ -31,7 +31,8 @@ EXPORTED = [
     "fl_split_solve",
     "fl_max_wavespeed", "fl_simulate", "fl_conservation_sums",
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
-    "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate",
+    "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate_work_bytes",
+    "fl_gfm_extrapolate",
     "fl_mm_combine", "fl_mm_work_bytes", "fl_mm_simulate",
     "fl_batch_create", "fl_batch_destroy", "fl_batch_set_tuning", "fl_batch_get_tuning",
     "fl_batch_autotune", "fl_batch_reset", "fl_batch_step", "fl_batch_finish",
@@ -42,6 +43,7 @@ EXPORTED = [
 
 # entry points returning a size rather than a status
 _SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes", "fl_adj_checkpoint_bytes",
+                   "fl_gfm_extrapolate_work_bytes",
                    "fl_adj_work_bytes")
 
 
@@ -163,6 +165,7 @@ def _load():
     L.fl_ls_reinit_work_bytes.argtypes = [C.c_int, i64p]
     L.fl_ls_reinit.argtypes = [vp, vp, C.c_int, i64p, C.c_double, C.c_int, vp, vp, vp, i64p, vp]
     L.fl_gfm_star.argtypes = [vp, vp, fp, fp, sp, sp, C.c_double, vp, vp, vp]
+    L.fl_gfm_extrapolate_work_bytes.argtypes = [C.c_int, i64p]
     L.fl_gfm_extrapolate.argtypes = [vp, fp, vp, vp, C.c_double, C.c_int, vp, vp]
     L.fl_mm_combine.argtypes = [vp, fp, fp, fp, vp]
     L.fl_mm_work_bytes.argtypes = [C.c_int, i64p]

@@ -305,10 +305,12 @@ int fl_gfm_star(const double* phi_m, const double* phi_o, const fl_field* state_
 
 /* Fills the ghost band 0 < phi_m <= 6 dh / sqrt(2) of state_m by `sweeps`
  * Jacobi sweeps of the normal-weighted upwind average, seeded with the star
- * states.  work: 2 * cells * (2+ndim) doubles.  Replaces
+ * states (the sweeps walk a list of the band cells).  work:
+ * fl_gfm_extrapolate_work_bytes() bytes, 256-byte aligned.  Replaces
  * GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202). */
+int64_t fl_gfm_extrapolate_work_bytes(int ndim, const int64_t* n);
 int fl_gfm_extrapolate(const double* phi_m, fl_field* state_m, const double* star,
-                       const uint8_t* has_star, double dh, int sweeps, double* work,
+                       const uint8_t* has_star, double dh, int sweeps, void* work,
                        void* stream);
 
 /* out = state0 where phi0 <= 0, else state1.  Replaces combine_into_first
