@@ -167,16 +167,29 @@ __device__ __forceinline__ double weno_diff(double d1, double d2, double d3, dou
   return (alpha_1 * phi_1 + alpha_2 * phi_2 + alpha_3 * phi_3) / w_denom;
 }
 
+// Mirrored index within a line of n cells (32-bit; n < 2^30).
+__device__ __forceinline__ int mirror32(int i, int n) {
+  if (i < 0) i = -i - 1;
+  if (i >= n) i = 2 * n - 1 - i;
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
 // L = V * H(phi) (levelset_hamiltonian.hpp:63-98, godunov_upwind.hpp:94-166).
-__device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int64_t x, int64_t y,
-                         double v, double dh) {
+__device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, int y, double v,
+                         double dh) {
   double result = 0.0;
   const double h1 = 1.0 / dh;
+  const int nx = (int)g.nx, ny = (int)g.ny;
   for (int d = 0; d < g.nd; ++d) {
-    const int64_t dx = (d == 0), dy = (d == 1);
     double f[7];
+    if (d == 0) {
+      const double* row = phi + (int64_t)y * nx;
 #pragma unroll
-    for (int k = -3; k <= 3; ++k) f[k + 3] = phi_at(phi, g, x + k * dx, y + k * dy);
+      for (int k = -3; k <= 3; ++k) f[k + 3] = row[mirror32(x + k, nx)];
+    } else {
+#pragma unroll
+      for (int k = -3; k <= 3; ++k) f[k + 3] = phi[(int64_t)mirror32(y + k, ny) * nx + x];
+    }
     const double bd = weno_diff(h1 * (f[1] - f[0]), h1 * (f[2] - f[1]), h1 * (f[3] - f[2]),
                                 h1 * (f[4] - f[3]), h1 * (f[5] - f[4]));
     const double fd = weno_diff(h1 * (f[6] - f[5]), h1 * (f[5] - f[4]), h1 * (f[4] - f[3]),
@@ -198,22 +211,26 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int64_t 
 //  stage 0: out = phi_n - dt L(phi_n)
 //  stage 1: out = 0.75 phi_n + 0.25 (s1 - dt L(s1))
 //  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
+// Launched on a (ceil(nx / blockDim.x), min(ny, 65535)) grid; rows stride by
+// gridDim.y.
 __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
                                  const double* __restrict__ src, const double* __restrict__ V,
                                  Grid g, double dt, double dh, double* __restrict__ out) {
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
-  const int64_t x = i % g.nx, y = i / g.nx;
-  const double l = ls_rhs(src, g, x, y, V[i], dh);
-  if (stage == 0) {
-    out[i] = phi_n[i] - dt * l;
-  } else if (stage == 1) {
-    out[i] = 0.75 * phi_n[i] + 0.25 * (src[i] - dt * l);
-  } else {
-    const double fact_13 = 1.0 / 3.0;
-    const double fact_23 = 2.0 * fact_13;
-    const double s3 = src[i] - dt * l;
-    out[i] = (fact_13 * phi_n[i]) + (fact_23 * s3);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= g.nx) return;
+  for (int y = blockIdx.y; y < g.ny; y += gridDim.y) {
+    const int64_t i = (int64_t)y * g.nx + x;
+    const double l = ls_rhs(src, g, x, y, V[i], dh);
+    if (stage == 0) {
+      out[i] = phi_n[i] - dt * l;
+    } else if (stage == 1) {
+      out[i] = 0.75 * phi_n[i] + 0.25 * (src[i] - dt * l);
+    } else {
+      const double fact_13 = 1.0 / 3.0;
+      const double fact_23 = 2.0 * fact_13;
+      const double s3 = src[i] - dt * l;
+      out[i] = (fact_13 * phi_n[i]) + (fact_23 * s3);
+    }
   }
 }
 

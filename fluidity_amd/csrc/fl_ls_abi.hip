@@ -96,7 +96,8 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
   const int64_t cells = g.nx * g.ny;
   double* s1 = work;
   double* s2 = work + cells;
-  const dim3 grid = grid_for(cells);
+  const dim3 grid((unsigned)((g.nx + kBlock - 1) / kBlock),
+                  (unsigned)(g.ny < 65535 ? g.ny : 65535));
   hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, dh,
                      s1);
   hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1,
