@@ -3,18 +3,45 @@
 // standalone wavespeed max, the device step control and FP64 sums.
 #include "fl_sweep_tiles.hpp"
 
+#ifndef FL_XCD_REMAP
+#define FL_XCD_REMAP 0
+#endif
+
 namespace fl {
+
+// Tile of this workgroup.  With FL_XCD_REMAP, consecutive workgroups of one
+// XCD (dispatch is round-robin over the 8 XCDs) take consecutive tiles down a
+// column band, so the rows that vertically adjacent tiles both read meet in
+// that XCD's L2.
+__device__ __forceinline__ void tile_of_block(int64_t& bx, int64_t& by) {
+#if FL_XCD_REMAP
+  const int64_t gx = gridDim.x, gy = gridDim.y, T = gx * gy;
+  const int64_t p = blockIdx.x + gx * (int64_t)blockIdx.y;
+  const int64_t xcd = p % 8, k = p / 8;
+  const int64_t base = T / 8, extra = T % 8;
+  const int64_t tile = xcd * base + (xcd < extra ? xcd : extra) + k;
+  bx = tile / gy;
+  by = tile % gy;
+#else
+  bx = blockIdx.x;
+  by = blockIdx.y;
+#endif
+}
 
 template <typename T, int ND, int FLUX>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_x_kernel(SweepArgs<T> a) {
-  sweep_x_tile<T, ND, FLUX>(a, blockIdx.x, blockIdx.y);
+  int64_t bx, by;
+  tile_of_block(bx, by);
+  sweep_x_tile<T, ND, FLUX>(a, bx, by);
 }
 
 template <typename T, int FLUX>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_y_kernel(SweepArgs<T> a) {
-  sweep_y_tile<T, FLUX>(a, blockIdx.x, blockIdx.y);
+  int64_t bx, by;
+  tile_of_block(bx, by);
+  sweep_y_tile<T, FLUX>(a, bx, by);
 }
 
 // ---------------------------------------------------------------------------
