@@ -15,7 +15,7 @@ import ctypes as C
 import torch
 
 from . import native as N
-from .euler import Scheme, _stream_handle, grid_field
+from .euler import Scheme, _stream_handle, autotune_context, grid_field
 
 
 def _bytes(n: int, device) -> torch.Tensor:
@@ -32,9 +32,18 @@ class Adjoint:
         self.dh = dh
         self.layout = layout
         self.ctx = N.Context(device)
+        self._tuned = None
+
+    def tune(self, U: torch.Tensor, stream=None):
+        """Launch shapes for this grid shape (once per shape)."""
+        if self._tuned != tuple(U.shape):
+            autotune_context(self.ctx, U, torch.empty_like(U), self.scheme, self.layout,
+                             stream=stream)
+            self._tuned = tuple(U.shape)
 
     def forward(self, U: torch.Tensor, steps: int, checkpoint_every: int = 20, stream=None):
         """Advances U in place by `steps` steps; returns the checkpoints."""
+        self.tune(U, stream)
         f = grid_field(U, self.layout)
         cb = N.lib.fl_adj_checkpoint_bytes(C.byref(f), steps, checkpoint_every)
         wb = N.lib.fl_adj_work_bytes(C.byref(f), checkpoint_every)
@@ -63,6 +72,7 @@ class Adjoint:
 
     def step_vjp(self, U: torch.Tensor, lam_next: torch.Tensor, stream=None):
         """(dU_{n+1}/dU_n)^T lam_next and lam_next . dU_{n+1}/dgamma."""
+        self.tune(U, stream)
         fu = grid_field(U, self.layout)
         fn = grid_field(lam_next, self.layout)
         lam = torch.empty_like(lam_next)

@@ -135,11 +135,25 @@ fl::SweepArgs<T> make_args(const fl_context* ctx, const fl_field* in, const fl_f
   return a;
 }
 
+// Rows per workgroup, halved (down to 4) while the launch would have fewer
+// than kMinGroups workgroups: small grids keep the machine filled whatever
+// the tuning says.
+constexpr int64_t kMinGroups = 1024;
+int fit_rows(int rows, int64_t bands, int64_t ny) {
+  while (rows > 4 && bands * ((ny + rows - 1) / rows) < kMinGroups) rows /= 2;
+  return rows;
+}
+
 template <typename T>
 int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
                    const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
                    unsigned long long* ws_out, hipStream_t stream) {
-  const fl_tuning& tu = ctx->tuning;
+  fl_tuning tu = ctx->tuning;
+  if (in->ndim == 2) {
+    const int64_t nx = in->n[0], ny = in->n[1];
+    tu.x_rows = fit_rows(tu.x_rows, (nx + tu.x_block - 3) / (tu.x_block - 2), ny);
+    tu.y_rows = fit_rows(tu.y_rows, (nx + tu.y_block - 1) / tu.y_block, ny);
+  }
   if (in->ndim == 1) {
     const int B = tu.x_block;
     fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, 1);

@@ -65,6 +65,10 @@ struct AdjLayout {
   double* lam[3];    // adjoint ping-pong + adjoint after the y sweep
   double* contrib;   // faces * 4 * S
   double* face_g;    // faces * 2
+  double* P;         // (nx+8)(ny+8): pressure within 4 ghost cells
+  double* V0;        // same: x velocity
+  double* V1;        // same: y velocity
+  double* chi;       // (nx+2)(ny+2): flattening factor within 1 ghost cell
   double* partials;  // 2 sweeps * 2 * kPartialBlocks
   aj::StepRec* rec;  // K
   double* acc;       // 2
@@ -89,6 +93,12 @@ AdjLayout adj_layout(void* base, const fl_field* f, int K) {
   for (int i = 0; i < 3; ++i) L.lam[i] = (double*)take(gb);
   L.contrib = (double*)take(nf * 4 * S * 8);
   L.face_g = (double*)take(nf * 2 * 8);
+  const int64_t nx = f->n[0], ny = f->ndim > 1 ? f->n[1] : 1;
+  const int64_t padded = (nx + 8) * (f->ndim > 1 ? ny + 8 : 1);
+  L.P = (double*)take(padded * 8);
+  L.V0 = (double*)take(padded * 8);
+  L.V1 = (double*)take(padded * 8);
+  L.chi = (double*)take((nx + 2) * (f->ndim > 1 ? ny + 2 : 1) * 8);
   L.partials = (double*)take(2 * 2 * aj::kPartialBlocks * 8);
   L.rec = (aj::StepRec*)take(sizeof(aj::StepRec) * K);
   L.acc = (double*)take(2 * 8);
@@ -173,8 +183,15 @@ void sweep_adj(const fl_field* f, const fl_scheme* s, const double* Uin, const d
   const int64_t n = D == 0 ? g.nx : g.ny;
   const int64_t lines = D == 0 ? g.ny : g.nx;
   const int64_t nf = (n + 1) * lines;
+  const int64_t padded = (g.nx + 8) * (ND > 1 ? g.ny + 8 : 1);
+  const int64_t cpad = (g.nx + 2) * (ND > 1 ? g.ny + 2 : 1);
+  hipLaunchKernelGGL(aj::prim_pad_kernel<ND>, dim3((unsigned)ceil_div(padded, 256)), dim3(256), 0,
+                     st, g, m, L.P, L.V0, L.V1);
+  hipLaunchKernelGGL(aj::chi_pad_kernel<ND>, dim3((unsigned)ceil_div(cpad, 256)), dim3(256), 0,
+                     st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1, L.chi);
   hipLaunchKernelGGL((aj::face_adj_kernel<ND, D>), dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0,
-                     st, g, lam_out, m, s->limiter, rec, L.contrib, L.face_g);
+                     st, g, lam_out, m, s->limiter, rec, (const double*)L.chi, L.contrib,
+                     L.face_g);
   hipLaunchKernelGGL(aj::partial_sum_kernel, dim3(aj::kPartialBlocks), dim3(256), 0, st,
                      (const double*)L.face_g, nf, partials);
   hipLaunchKernelGGL((aj::gather_adj_kernel<ND, D>), dim3((unsigned)ceil_div(g.nx * g.ny, 256)),

@@ -48,54 +48,64 @@ __device__ __forceinline__ void load_state(const AGrid& g, int64_t x, int64_t y,
   if (ND > 1 && fy) reflect<double, ND>(u, 1);
 }
 
+// Pressure and velocities of every position within 4 ghost cells of the grid
+// (ghost rule and reflective flip applied), on the padded grid P (x fastest,
+// pad 4 in each dimension of the grid).
 template <int ND>
-__device__ __forceinline__ double pressure_at(const AGrid& g, const Material<double>& m,
-                                              int64_t x, int64_t y) {
+__global__ void __launch_bounds__(256) prim_pad_kernel(AGrid g, Material<double> m,
+                                                       double* __restrict__ P,
+                                                       double* __restrict__ V0,
+                                                       double* __restrict__ V1) {
+  const int64_t px = g.nx + 8, py = ND > 1 ? g.ny + 8 : 1;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= px * py) return;
+  const int64_t x = q % px - 4, y = ND > 1 ? q / px - 4 : 0;
   double u[2 + ND];
   load_state<ND>(g, x, y, u);
-  return prim_of<double, ND>(u, m).p;
+  const Prim<double, ND> w = prim_of<double, ND>(u, m);
+  P[q] = w.p;
+  V0[q] = w.vel[0];
+  if (ND > 1) V1[q] = w.vel[ND - 1];
 }
 
+// chi_v of MHReconstructor::evolve (reconstruction/muscl_reconstructor.hpp:59-87,
+// 106-118) -- the minimum over dimensions of chi about the cell and about its
+// neighbour on the higher-pressure side -- for every position within one
+// ghost cell of the grid, from the padded primitives.
 template <int ND>
-__device__ __forceinline__ double velocity_at(const AGrid& g, const Material<double>& m,
-                                              int64_t x, int64_t y, int d) {
-  double u[2 + ND];
-  load_state<ND>(g, x, y, u);
-  return prim_of<double, ND>(u, m).vel[d];
-}
-
-// MHReconstructor::chi (reconstruction/muscl_reconstructor.hpp:59-87) and the
-// minimum over dimensions of evolve (:106-118), as the forward sweeps use it.
-template <int ND>
-__device__ double chi_at(const AGrid& g, const Material<double>& m, int64_t x, int64_t y,
-                         int d) {
-  const int64_t dx = (d == 0), dy = (d == 1);
-  const double p_next = pressure_at<ND>(g, m, x + dx, y + dy);
-  const double p_prev = pressure_at<ND>(g, m, x - dx, y - dy);
-  const double p_nnext = pressure_at<ND>(g, m, x + 2 * dx, y + 2 * dy);
-  const double p_pprev = pressure_at<ND>(g, m, x - 2 * dx, y - 2 * dy);
-  const double v_prev = velocity_at<ND>(g, m, x - dx, y - dy, d);
-  const double v_next = velocity_at<ND>(g, m, x + dx, y + dy, d);
-  return chi_factor<double>(p_pprev, p_prev, p_next, p_nnext, v_prev, v_next);
-}
-
-template <int ND>
-__device__ double chi_min(const AGrid& g, const Material<double>& m, int64_t x, int64_t y) {
-  double chi = DBL_MAX;
+__global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __restrict__ P,
+                                                      const double* __restrict__ V0,
+                                                      const double* __restrict__ V1,
+                                                      double* __restrict__ chi) {
+  const int64_t px = g.nx + 8;
+  const int64_t cx = g.nx + 2, cy = ND > 1 ? g.ny + 2 : 1;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cx * cy) return;
+  const int64_t x = q % cx - 1, y = ND > 1 ? q / cx - 1 : 0;
+  auto at = [&](const double* A, int64_t xx, int64_t yy) {
+    return A[(ND > 1 ? (yy + 4) * px : 0) + xx + 4];
+  };
+  auto chi_d = [&](int64_t xx, int64_t yy, int d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const double* V = d == 0 ? V0 : V1;
+    return chi_factor<double>(at(P, xx - 2 * dx, yy - 2 * dy), at(P, xx - dx, yy - dy),
+                              at(P, xx + dx, yy + dy), at(P, xx + 2 * dx, yy + 2 * dy),
+                              at(V, xx - dx, yy - dy), at(V, xx + dx, yy + dy));
+  };
+  double c = DBL_MAX;
   for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
-    const int sign = signum(pressure_at<ND>(g, m, x + dx, y + dy) -
-                            pressure_at<ND>(g, m, x - dx, y - dy));
-    chi = smin(chi, smin(chi_at<ND>(g, m, x, y, d),
-                         chi_at<ND>(g, m, x + sign * dx, y + sign * dy, d)));
+    const int sign = signum(at(P, x + dx, y + dy) - at(P, x - dx, y - dy));
+    c = smin(c, smin(chi_d(x, y, d), chi_d(x + sign * dx, y + sign * dy, d)));
   }
-  return chi;
+  chi[q] = c;
 }
 
 template <int ND, int D>
 __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __restrict__ lam,
                                                        Material<double> m, int limiter,
                                                        const StepRec* __restrict__ rec,
+                                                       const double* __restrict__ chi_pad,
                                                        double* __restrict__ contrib,
                                                        double* __restrict__ face_g) {
   constexpr int S = 2 + ND;
@@ -131,8 +141,9 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
   int64_t xa, ya, xb, yb;
   pos(i, xa, ya);
   pos(i + 1, xb, yb);
-  const double chi_a = chi_min<ND>(g, m, xa, ya);
-  const double chi_b = chi_min<ND>(g, m, xb, yb);
+  const int64_t cxw = g.nx + 2;
+  const double chi_a = chi_pad[(ND > 1 ? (ya + 1) * cxw : 0) + xa + 1];
+  const double chi_b = chi_pad[(ND > 1 ? (yb + 1) * cxw : 0) + xb + 1];
   double R[S], L[S], tmp[S], G[S];
   evolve_faces<double, ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, kFormCons, m, R, tmp);
   evolve_faces<double, ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, kFormCons, m, tmp, L);
@@ -179,11 +190,13 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
                     g_half, gm);
   evolve_adj<ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, m, zero, gL, c[1], c[2], c[3],
                     g_half, gm);
-  double* out = contrib + f * (4 * S);
+  // contributions as planes (slot, component) x faces: coalesced both here and
+  // in the gather
+  const int64_t nf = (n + 1) * lines;
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int k = 0; k < S; ++k) out[s * S + k] = c[s][k];
+    for (int k = 0; k < S; ++k) contrib[(s * S + k) * nf + f] = c[s][k];
   face_g[2 * f] = g_dtdh + 0.5 * g_half;
   face_g[2 * f + 1] = gm.gamma;
 }
@@ -227,11 +240,11 @@ __global__ void __launch_bounds__(256) gather_adj_kernel(AGrid g, const double* 
       const int64_t i = v + 1 - s;
       if (i < -1 || i > n - 1) continue;
       const int64_t f = D == 0 ? line * (n + 1) + (i + 1) : (i + 1) * lines + line;
-      const double* c = contrib + f * (4 * S) + s * S;
+      const int64_t nf = (n + 1) * lines;
 #pragma unroll
       for (int k = 0; k < S; ++k) {
-        const double ck = (flips[a] && k == 2 + D) ? -c[k] : c[k];
-        acc[k] += ck;
+        const double ck = contrib[(s * S + k) * nf + f];
+        acc[k] += (flips[a] && k == 2 + D) ? -ck : ck;
       }
     }
   }
@@ -354,13 +367,20 @@ __global__ void acc_clear_kernel(double* acc) {
 
 #define FL_ADJ_INST(ND, D)                                                                  \
   template __global__ void face_adj_kernel<ND, D>(AGrid, const double*, Material<double>,    \
-                                                  int, const StepRec*, double*, double*);    \
+                                                  int, const StepRec*, const double*,        \
+                                                  double*, double*);                         \
   template __global__ void gather_adj_kernel<ND, D>(AGrid, const double*, double*,           \
                                                     const double*);
 
 FL_ADJ_INST(1, 0)
 FL_ADJ_INST(2, 0)
 FL_ADJ_INST(2, 1)
+template __global__ void prim_pad_kernel<1>(AGrid, Material<double>, double*, double*, double*);
+template __global__ void prim_pad_kernel<2>(AGrid, Material<double>, double*, double*, double*);
+template __global__ void chi_pad_kernel<1>(AGrid, const double*, const double*, const double*,
+                                           double*);
+template __global__ void chi_pad_kernel<2>(AGrid, const double*, const double*, const double*,
+                                           double*);
 template __global__ void argmax_kernel<1>(AGrid, Material<double>, StepRec*);
 template __global__ void argmax_kernel<2>(AGrid, Material<double>, StepRec*);
 template __global__ void finalize_kernel<1>(AGrid, Material<double>, const StepRec*,

@@ -25,9 +25,15 @@ struct StepRec {
 
 constexpr int kPartialBlocks = 256;
 
+template <int ND>
+__global__ void prim_pad_kernel(AGrid g, Material<double> m, double* P, double* V0, double* V1);
+template <int ND>
+__global__ void chi_pad_kernel(AGrid g, const double* P, const double* V0, const double* V1,
+                               double* chi);
 template <int ND, int D>
 __global__ void face_adj_kernel(AGrid g, const double* lam, Material<double> m, int limiter,
-                                const StepRec* rec, double* contrib, double* face_g);
+                                const StepRec* rec, const double* chi_pad, double* contrib,
+                                double* face_g);
 template <int ND, int D>
 __global__ void gather_adj_kernel(AGrid g, const double* lam_out, double* lam_in,
                                   const double* contrib);

@@ -65,6 +65,18 @@ class Scheme:
                         self.bc)
 
 
+def autotune_context(ctx: N.Context, U, work, scheme: "Scheme", layout: int, reps: int = 2,
+                     stream=None) -> N.Tuning:
+    """Launch-shape autotuning of a context on the grid U (U is not
+    modified; `work` is scratch of the same shape)."""
+    fi, fo = grid_field(U, layout), grid_field(work, layout)
+    s = scheme.c()
+    t = N.Tuning()
+    N.check(N.lib.fl_ctx_autotune(ctx.handle, C.byref(fi), C.byref(fo), C.byref(s), reps,
+                                  C.byref(t), _stream_handle(stream)))
+    return t
+
+
 class SplitSolver:
     """Dimensionally split MUSCL-Hancock solver on one device."""
 
@@ -85,12 +97,7 @@ class SplitSolver:
     def autotune(self, U, work, reps: int = 3, stream=None) -> N.Tuning:
         """Times the candidate launch shapes of both sweeps on this grid
         (U is not modified) and keeps the fastest."""
-        fi, fo = grid_field(U, self.layout), grid_field(work, self.layout)
-        s = self.scheme.c()
-        t = N.Tuning()
-        N.check(N.lib.fl_ctx_autotune(self.ctx.handle, C.byref(fi), C.byref(fo), C.byref(s), reps,
-                                      C.byref(t), _stream_handle(stream)))
-        return t
+        return autotune_context(self.ctx, U, work, self.scheme, self.layout, reps, stream)
 
     def sweep(self, U_in, U_out, dim: int, dtdh: float, stream=None) -> None:
         fi, fo = grid_field(U_in, self.layout), grid_field(U_out, self.layout)

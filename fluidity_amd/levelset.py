@@ -22,7 +22,7 @@ import ctypes as C
 import torch
 
 from . import native as N
-from .euler import Scheme, _stream_handle, grid_field
+from .euler import Scheme, _stream_handle, autotune_context, grid_field
 
 INSIDE, OUTSIDE, ON_BOUNDARY, INSIDE_INTERFACIAL, OUTSIDE_INTERFACIAL = 1, 2, 4, 8, 16
 
@@ -159,6 +159,7 @@ class MultimaterialSimulator:
         self.layout = layout
         self.ctx = N.Context(device)
         self.materials = []
+        self._tuned = False
 
     def add_material(self, state: torch.Tensor, phi: torch.Tensor, scheme: Scheme):
         if len(self.materials) == 2:
@@ -176,6 +177,10 @@ class MultimaterialSimulator:
             mats[m].work = grid_field(work, self.layout)
             mats[m].phi = phi.data_ptr()
             mats[m].scheme = scheme.c()
+        if not self._tuned:
+            state, work, _, scheme = self.materials[0]
+            autotune_context(self.ctx, state, work, scheme, self.layout, stream=stream)
+            self._tuned = True
         nd, n = _shape(self.materials[0][2])
         wbytes = N.lib.fl_mm_work_bytes(nd, n)
         work = _work(wbytes, self.materials[0][0].device)
