@@ -43,6 +43,10 @@ namespace {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Segments of B-2 cells a 1D tile marches (loads of the next segments overlap
+// the current one).
+constexpr int kSegments1D = 8;
+
 bool valid_width(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0; }
 
 // Tile lists for a tuning: phase 0 holds the x tiles of the 2D problems
@@ -67,7 +71,7 @@ void build_tiles(const fl_batch* b, const fl_tuning& t, std::vector<bt::Tile>& t
     const fl_field& f = b->problems[p].state;
     const int64_t nx = f.n[0];
     if (f.ndim == 1) {
-      for (int64_t bx = 0; bx < ceil_div(nx, t.x_block - 2); ++bx)
+      for (int64_t bx = 0; bx < ceil_div(ceil_div(nx, t.x_block - 2), kSegments1D); ++bx)
         t0.push_back({p, (int)bx, 0, 0});
     } else {
       const int64_t ny = f.n[1];
@@ -111,7 +115,7 @@ std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
     a.form = q.scheme.form;
     a.bcx = q.scheme.bc[0];
     a.bcy = q.state.ndim > 1 ? q.scheme.bc[1] : 0;
-    a.rows_per_block = q.state.ndim > 1 ? t.x_rows : 1;
+    a.rows_per_block = q.state.ndim > 1 ? t.x_rows : kSegments1D;
     a.ctl = b->ctls + p;
     a.ws_out = nullptr;
     P.ctl = b->ctls + p;

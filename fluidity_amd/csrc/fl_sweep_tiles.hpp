@@ -176,8 +176,10 @@ __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
 // ---------------------------------------------------------------------------
 
 // One x-sweep tile: columns [bx (B-2), (bx+1) (B-2)) of rows
-// [by rows_per_block, (by+1) rows_per_block), B = blockDim.x.  Every thread of
-// the workgroup must call it with the same arguments.
+// [by rows_per_block, (by+1) rows_per_block), B = blockDim.x.  In 1D a tile
+// is rows_per_block consecutive segments of B-2 cells, starting at segment
+// bx rows_per_block, marched like rows.  Every thread of the workgroup must
+// call it with the same arguments.
 template <typename T, int ND, int FLUX>
 __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int S = 2 + ND;
@@ -197,10 +199,14 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   const int B = blockDim.x;
   const int t = threadIdx.x;
   const int s = t + kHalo;                       // LDS slot of this lane's column
-  const int64_t x0 = bx * (B - 2);
+  const int64_t seg0 = (ND == 2) ? bx : bx * a.rows_per_block;   // first column band
+  const int64_t x0 = seg0 * (B - 2);
   const int64_t c = x0 - 1 + t;                  // column whose face states this lane builds
   const int64_t y0 = (ND == 2) ? by * a.rows_per_block : 0;
   const int64_t yend = (ND == 2) ? min(y0 + (int64_t)a.rows_per_block, a.ny) : 1;
+  // 1D: segments of this tile, at most rows_per_block and not past the grid
+  const int64_t nseg1 = (a.nx + B - 3) / (B - 2) - seg0;
+  const int64_t segs = (ND == 2) ? 1 : min((int64_t)a.rows_per_block, nseg1);
 
   // Halo lanes: t < 6 stage columns x0-4..x0-2 (slots 0..2) and
   // x0+B-1..x0+B+1 (slots B+3..B+5) of the output row.
@@ -209,6 +215,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   const int hl = t % (2 * kHalo);
   const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - 1 - kHalo + hslot;
+  // column of this lane and its halo cell for row (1D: segment) r
+  const int64_t cstep = (ND == 2) ? 0 : B - 2;
 
   // Register windows along y (2D): rows r-3..r of U, u_x; rows r-4..r of p;
   // rows r-3..r of u_y; K_y of rows r-4..r-2.
@@ -224,7 +232,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 
   unsigned long long wsmax = 0ull;
   const int64_t rbeg = (ND == 2) ? y0 - 3 : 0;
-  const int64_t rlast = (ND == 2) ? yend + 2 : 0;
+  const int64_t rlast = (ND == 2) ? yend + 2 : segs - 1;
 
   // Software pipelining with PF row buffers in fixed registers: the loads of
   // row r+PF and of the halo cells of the output row PF steps ahead are issued
@@ -238,8 +246,9 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
-    issue_load<T, ND>(a, c, min(rbeg + i, rlast), Qs[i]);
-    issue_load<T, ND>(a, hcol, hrow(min(rbeg + i, rlast)), Hs[i]);
+    const int64_t ri = min(rbeg + i, rlast);
+    issue_load<T, ND>(a, c + ri * cstep, ri, Qs[i]);
+    issue_load<T, ND>(a, hcol + ri * cstep, hrow(ri), Hs[i]);
   }
 
   // Step i of each group of PF rows owns buffer i.  The inner loop is fully
@@ -262,10 +271,11 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
     take<T, ND>(Q, Uw[3]);
-    issue_load<T, ND>(a, c, min(r + PF, rlast), Q);
+    const int64_t rn = min(r + PF, rlast);
+    issue_load<T, ND>(a, c + rn * cstep, rn, Q);
     T uh[S];  // halo cell of output row hrow(r)
     take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, hrow(min(r + PF, rlast)), H);
+    issue_load<T, ND>(a, hcol + rn * cstep, hrow(rn), H);
     {
       const Prim<T, ND> w = prim_of<T, ND>(Uw[3], a.mat);
       pw[4] = w.p;
@@ -341,11 +351,12 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     __syncthreads();
 
     // ---- phase E: conservative update U + dtdh (F_{c-1/2} - F_{c+1/2})
-    if (t >= 1 && t <= B - 2 && c < a.nx) {
+    const int64_t cr = c + r * cstep;
+    if (t >= 1 && t <= B - 2 && cr < a.nx) {
       T o[S];
 #pragma unroll
       for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k][t - 1] - G[k]) * dtdh;
-      store_cell<T, S>(a.out, jo * a.nx + c, a.es, a.cs, o);
+      store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
       if (a.ws_out) {
         const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(o, a.mat64));
         wsmax = b > wsmax ? b : wsmax;
