@@ -198,9 +198,10 @@ int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst
   strides(f, es, cs);
   const int64_t n = cell_count(f);
   const fl::Material<double> m64 = make_material<double>(s);
-  const int B = 256;
+  const int B = fl::kMaxBlock;
   int64_t blocks = (n + B - 1) / B;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > fl::kMaxWsBlocks) blocks = fl::kMaxWsBlocks;
+  if (blocks < 1) blocks = 1;
   dim3 grid((unsigned)blocks);
   if (f->dtype == FL_F64) {
     if (f->ndim == 1)

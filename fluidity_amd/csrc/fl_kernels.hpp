@@ -46,6 +46,11 @@ struct SweepArgs {
 constexpr int kMaxBlock = 256;
 constexpr int kHalo = 3;  // extra cells each side staged in LDS for chi
 
+// Standalone wavespeed max: at most kMaxWsBlocks workgroups of kMaxBlock
+// threads, each thread loading kMaxWsUnroll cells per iteration.
+constexpr int kMaxWsBlocks = 512;
+constexpr int kMaxWsUnroll = 4;
+
 }  // namespace fl
 
 // Kernels defined in fl_sweep.hip (explicitly instantiated there).

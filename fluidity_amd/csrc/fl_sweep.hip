@@ -48,15 +48,31 @@ sweep_y_kernel(SweepArgs<T> a) {
 // standalone wavespeed max, step control
 // ---------------------------------------------------------------------------
 
+// Grid-stride max over the cells; each thread issues the loads of
+// kMaxWsUnroll cells before computing any of them.  The launch keeps the
+// grid small (kMaxWsBlocks) because every workgroup ends in one atomicMax on
+// the same address, and those serialise in L2.
 template <typename T, int ND>
 __global__ void __launch_bounds__(kMaxBlock) max_wavespeed_kernel(const T* __restrict__ in,
                                                                    int64_t es, int64_t cs,
                                                                    int64_t ncells,
                                                                    Material<double> m64,
                                                                    unsigned long long* dst) {
+  constexpr int U = kMaxWsUnroll;
   unsigned long long v = 0ull;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < ncells; i += U * stride) {
+    T u[U][2 + ND];
+#pragma unroll
+    for (int k = 0; k < U; ++k) load_cell<T, 2 + ND>(in, i + k * stride, es, cs, u[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], m64));
+      v = b > v ? b : v;
+    }
+  }
+  for (; i < ncells; i += stride) {
     T u[2 + ND];
     load_cell<T, 2 + ND>(in, i, es, cs, u);
     const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u, m64));
