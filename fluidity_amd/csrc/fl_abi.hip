@@ -85,17 +85,6 @@ int check_scheme(const fl_scheme* s, int ndim) {
   return FL_OK;
 }
 
-template <typename T>
-fl::Material<T> make_material(const fl_scheme* s) {
-  fl::Material<T> m;
-  m.gamma = (T)s->gamma;
-  m.p_inf = (T)s->p_inf;
-  m.gm1 = m.gamma - T(1);
-  m.q = (m.gamma + T(1)) / (T(2) * m.gamma);
-  m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
-  return m;
-}
-
 void strides(const fl_field* f, int64_t& es, int64_t& cs) {
   const int S = 2 + f->ndim;
   const int64_t cells = f->n[0] * (f->ndim > 1 ? f->n[1] : 1);

@@ -67,6 +67,19 @@ struct DeviceGuard {
   }
 };
 
+// Material constants of a scheme, derived on the host with the expressions the
+// reference evaluates per call (gamma - 1, (gamma + 1) / (2 gamma)).
+template <typename T>
+inline fl::Material<T> make_material(const fl_scheme* s) {
+  fl::Material<T> m;
+  m.gamma = (T)s->gamma;
+  m.p_inf = (T)s->p_inf;
+  m.gm1 = m.gamma - T(1);
+  m.q = (m.gamma + T(1)) / (T(2) * m.gamma);
+  m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
+  return m;
+}
+
 double bits_to_double(unsigned long long b);
 int check_field(const fl_field* f, const char* name);
 int check_scheme(const fl_scheme* s, int ndim);

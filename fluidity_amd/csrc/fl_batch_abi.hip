@@ -86,17 +86,6 @@ void build_tiles(const fl_batch* b, const fl_tuning& t, std::vector<bt::Tile>& t
 }
 
 template <typename T>
-fl::Material<T> material_of(const fl_scheme* s) {
-  fl::Material<T> m;
-  m.gamma = (T)s->gamma;
-  m.p_inf = (T)s->p_inf;
-  m.gm1 = m.gamma - T(1);
-  m.q = (m.gamma + T(1)) / (T(2) * m.gamma);
-  m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
-  return m;
-}
-
-template <typename T>
 std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
   std::vector<bt::Prob<T>> v(b->nprob);
   for (int p = 0; p < b->nprob; ++p) {
@@ -109,8 +98,8 @@ std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
     strides(&q.state, a.es, a.cs);
     a.nx = q.state.n[0];
     a.ny = q.state.ndim > 1 ? q.state.n[1] : 1;
-    a.mat = material_of<T>(&q.scheme);
-    a.mat64 = material_of<double>(&q.scheme);
+    a.mat = make_material<T>(&q.scheme);
+    a.mat64 = make_material<double>(&q.scheme);
     a.limiter = q.scheme.limiter;
     a.form = q.scheme.form;
     a.bcx = q.scheme.bc[0];

@@ -39,6 +39,8 @@ EXPORTED = [
     "fl_batch_conservation_sums",
     "fl_adj_checkpoint_bytes", "fl_adj_work_bytes", "fl_adj_forward", "fl_adj_backward",
     "fl_split_step_vjp",
+    "fl_primitive_planes", "fl_from_primitive", "fl_set_states", "fl_write_results",
+    "fl_write_results_separate_raw",
 ]
 
 # entry points returning a size rather than a status
@@ -188,6 +190,11 @@ def _load():
                                   C.c_int64, vp, C.c_int64, C.POINTER(C.c_double), vp]
     L.fl_split_step_vjp.argtypes = [vp, fp, fp, fp, sp, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), vp, C.c_int64, vp]
+    L.fl_primitive_planes.argtypes = [vp, fp, sp, vp, vp]
+    L.fl_from_primitive.argtypes = [vp, vp, sp, fp, vp]
+    L.fl_set_states.argtypes = [vp, vp, C.POINTER(C.c_double), sp, fp, vp]
+    L.fl_write_results.argtypes = [vp, fp, sp, vp, C.c_double, C.c_double, C.c_char_p, vp]
+    L.fl_write_results_separate_raw.argtypes = [vp, fp, sp, C.c_double, C.c_char_p, vp]
     for name in EXPORTED:
         if name in _SIZE_RETURNING:
             getattr(L, name).restype = C.c_int64

@@ -156,6 +156,55 @@ int fl_ctx_set_graphs(fl_context* ctx, int enable);
  * FP32 bulk path. */
 int fl_conservation_sums(fl_context* ctx, const fl_field* in, double* sums, void* stream);
 
+/* ---- initial conditions and result output -----------------------------------
+ * The device half of SimulatorImpl::fill_data / MultimaterialSimulator's
+ * set_state_data, and the reference's result formats. */
+
+/* Primitive planes of a grid, for output: `out` (device, FP64) receives 3+ndim
+ * planes of cell_count doubles, x fastest: rho, p, v_x, (v_y), and the
+ * internal energy e = eos(primitive state).  Replaces the per-cell
+ * state.primitive(material) and material.eos(state) of stream_output_ascii
+ * (simulator/simulator_impl.hpp:478-490, state/state_impl.hpp:339-357,
+ * material/ideal_gas.hpp:60-63, material/stiff_fluid.hpp:77-80). */
+int fl_primitive_planes(fl_context* ctx, const fl_field* state, const fl_scheme* scheme,
+                        double* out, void* stream);
+
+/* Conservative grid from primitive planes (device, FP64: rho, p, v_x, (v_y),
+ * x fastest): fill_state.conservative(material) of SimulatorImpl::fill_data
+ * (simulator/simulator_impl.hpp:329-331, state/state_impl.hpp:135-142,
+ * 388-406).  The host evaluates the fillers. */
+int fl_from_primitive(fl_context* ctx, const double* prim, const fl_scheme* scheme,
+                      fl_field* state, void* stream);
+
+/* Cells inside the level set `phi` (device, one double per cell, phi <= 0)
+ * take the conservative form of the primitive state `prim_state` (host,
+ * 2+ndim doubles: rho, p, v...); others are untouched.  Replaces set_states
+ * (state/cuda/state_initialization.cuh:72-118) as called by
+ * MultimaterialSimulationData::set_state_data
+ * (simulator/multimaterial_simulation_data.hpp:166-189). */
+int fl_set_states(fl_context* ctx, const double* phi, const double* prim_state,
+                  const fl_scheme* scheme, fl_field* state, void* stream);
+
+/* The ASCII result file of SimulatorImpl::write_results / print_results
+ * (simulator/simulator_impl.hpp:343-357, 435-508): a "# t = <run_time>"
+ * header, column names, then one line per cell (x fastest) of coordinates
+ * ((i + 0.5) * resolution), rho, p, v..., e, each as "%-12.8f ".  path NULL
+ * writes to stdout.  With `phi` (device, nullable) the file is the
+ * two-material one of MultimaterialSimulator::stream_output_ascii
+ * (simulator/multimaterial_simulator.hpp:882-971): one level-set column per
+ * dimension after the coordinates.  Synchronises `stream`. */
+int fl_write_results(fl_context* ctx, const fl_field* state, const fl_scheme* scheme,
+                     const double* phi, double run_time, double resolution, const char* path,
+                     void* stream);
+
+/* SimulatorImpl::write_results_separate_raw (simulator/simulator_impl.hpp:
+ * 360-420): one file per primitive component, named
+ * <prefix>_<rho|p|v_x|v_y>_<sim_time as %f>.dat, holding the values as "%g "
+ * one grid row per line (1D: one more newline).  Synchronises `stream`. */
+int fl_write_results_separate_raw(fl_context* ctx, const fl_field* state,
+                                  const fl_scheme* scheme, double sim_time, const char* prefix,
+                                  void* stream);
+
 /* ---- batched problems -------------------------------------------------------
  * Many independent grids of uneven size (1D and 2D, each with its own
  * material, CFL dt and stop condition) advanced together: one step is a

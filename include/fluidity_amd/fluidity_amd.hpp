@@ -284,10 +284,41 @@ class Simulator {
     const fl_scheme s = Solver::scheme(mat_, setter_);
     fl_sim_result r{};
     amd::check(fl_simulate(amd::Context::instance().get(), &fs, &fw, &s, &p_, &r, stream));
+    run_time_ += r.run_time;
     return r;
   }
 
+  // print_results / write_results / write_results_separate_raw
+  // (simulator/simulator_impl.hpp:343-420): the reference's ASCII formats of
+  // `state`, stamped with the accumulated run time (sim_time for the raw
+  // files) and placed with the configured resolution.
+  template <typename It>
+  void print_results(const It& state, void* stream = nullptr) const {
+    write(state, nullptr, stream);
+  }
+  template <typename It>
+  void write_results(const It& state, const std::string& prefix, void* stream = nullptr) const {
+    write(state, (prefix + ".dat").c_str(), stream);
+  }
+  template <typename It>
+  void write_results_separate_raw(const It& state, const std::string& prefix,
+                                  void* stream = nullptr) const {
+    fl_field fs = state.field();
+    const fl_scheme s = Solver::scheme(mat_, setter_);
+    amd::check(fl_write_results_separate_raw(amd::Context::instance().get(), &fs, &s,
+                                             p_.sim_time, prefix.c_str(), stream));
+  }
+
  private:
+  template <typename It>
+  void write(const It& state, const char* path, void* stream) const {
+    fl_field fs = state.field();
+    const fl_scheme s = Solver::scheme(mat_, setter_);
+    amd::check(fl_write_results(amd::Context::instance().get(), &fs, &s, nullptr, run_time_,
+                                p_.dh, path, stream));
+  }
+
+  double run_time_ = 0.0;
   fl_sim_params p_{0.9, 0.0, 1e300, int64_t(1) << 40};
   eos_t mat_{};
   solver::BoundarySetter setter_{};

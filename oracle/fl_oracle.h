@@ -103,6 +103,14 @@ int  orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const i
                      int reinit_every, int nthreads, double* run_time, int64_t* iters,
                      double* dt_hist);
 
+/* ---- initial conditions and result files (fl_oracle_io.cpp) ---- */
+int  orc_write_ascii(const double* U, const int64_t* n, int nd, const orc_scheme* s,
+                     const double* phi, double run_time, double resolution, const char* path);
+int  orc_write_separate_raw(const double* U, const int64_t* n, int nd, const orc_scheme* s,
+                            double sim_time, const char* prefix);
+void orc_set_states(const double* phi, const double* W, const orc_scheme* s, double* U,
+                    int64_t cells, int nd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -119,6 +119,13 @@ def lib():
                                       C.c_double, C.c_double, C.c_int64, C.c_int, C.c_int,
                                       C.c_int, C.c_int, dp, i64p, dp]
         L.orc_mm_simulate.restype = C.c_int
+        L.orc_write_ascii.argtypes = [dp, i64p, C.c_int, sp, dp, C.c_double, C.c_double,
+                                      C.c_char_p]
+        L.orc_write_ascii.restype = C.c_int
+        L.orc_write_separate_raw.argtypes = [dp, i64p, C.c_int, sp, C.c_double, C.c_char_p]
+        L.orc_write_separate_raw.restype = C.c_int
+        L.orc_set_states.argtypes = [dp, dp, sp, dp, C.c_int64, C.c_int]
+        L.orc_set_states.restype = None
         _lib = L
     return _lib
 
@@ -383,3 +390,70 @@ def mm_simulate(U0, U1, phi0, phi1, s0, s1, cfl, dh, sim_time=float("inf"), max_
     if rc:
         raise RuntimeError(f"orc_mm_simulate failed: {rc}")
     return U0, U1, phi0, phi1, rt.value, it.value, dts[: it.value]
+
+
+# ---- initial conditions and result files (fl_oracle_io.cpp) ---------------
+
+def fill_positions(n):
+    """The positions SimulatorImpl::fill_data hands its fillers
+    (simulator/simulator_impl.hpp:318-326): pos[d] = float(index_d) / size_d in
+    single precision, index_d = (i / stride_d) % size_d
+    (dimension/dimension_info.hpp flattened_index).  n = sizes, x first;
+    returns an (cells, 3) float32 array, x fastest, unused dimensions 0."""
+    n = [int(v) for v in n]
+    cells = int(np.prod(n))
+    pos = np.zeros((cells, 3), dtype=np.float32)
+    stride = 1
+    i = np.arange(cells, dtype=np.int64)
+    for d, size in enumerate(n):
+        idx = (i // stride) % size
+        pos[:, d] = idx.astype(np.float32) / np.float32(size)
+        stride *= size
+    return pos
+
+
+def fill_data(n, fillers, s):
+    """SimulatorImpl::fill_data (simulator/simulator_impl.hpp:292-340): each
+    filler (name, f) maps the float32 positions to float32 values of the
+    primitive element `name`; every cell becomes conservative(fill_state).
+    Elements no filler names keep the default 0 (the fill state is
+    value-initialised once and reused).  Returns the AoS conservative grid."""
+    n = [int(v) for v in n]
+    nd = len(n)
+    names = ["rho", "p", "v_x", "v_y"][:2 + nd]
+    pos = fill_positions(n)
+    W = np.zeros((pos.shape[0], 2 + nd), dtype=np.float64)
+    for name, f in fillers:
+        if name not in names:
+            raise RuntimeError("State does not have a data element named: \n\t" + name)
+        W[:, names.index(name)] = np.asarray(f(pos), dtype=np.float32)
+    U = np.zeros_like(W)
+    for i in range(W.shape[0]):
+        lib().orc_prim_to_cons(_dp(W[i]), nd, 0, C.byref(s), _dp(U[i]))
+    return U.reshape((n[0], 2 + nd) if nd == 1 else (n[1], n[0], 2 + nd))
+
+
+def write_ascii(U, s, path, run_time, resolution, phi=None):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    nd, n = _dims(U)
+    ph = None if phi is None else _dp(np.ascontiguousarray(phi, dtype=np.float64))
+    if lib().orc_write_ascii(_dp(U), _n_ptr(n), nd, C.byref(s), ph, run_time, resolution,
+                             str(path).encode()):
+        raise OSError(f"cannot write {path}")
+
+
+def write_separate_raw(U, s, sim_time, prefix):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    nd, n = _dims(U)
+    if lib().orc_write_separate_raw(_dp(U), _n_ptr(n), nd, C.byref(s), sim_time,
+                                    str(prefix).encode()):
+        raise OSError(f"cannot write {prefix}_*")
+
+
+def set_states(phi, W, s, U):
+    """In place on the AoS grid U: cells with phi <= 0 become conservative(W)."""
+    nd = U.shape[-1] - 2
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    assert U.dtype == np.float64 and U.flags.c_contiguous
+    lib().orc_set_states(_dp(phi), _dp(W), C.byref(s), _dp(U), phi.size, nd)

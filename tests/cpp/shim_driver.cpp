@@ -3,6 +3,10 @@
 //
 //   shim_driver solve    <combo> <nd> <nx> <ny> <dtdh>  <in.bin> <out.bin>
 //   shim_driver simulate <combo> <nd> <nx> <ny> <steps> <in.bin> <out.bin>
+//   shim_driver write    <combo> <nd> <nx> <ny> <steps> <in.bin> <prefix>
+//
+// `write` runs `steps` steps with sim_time 10, then writes <prefix>.dat
+// (write_results) and <prefix>_<name>_10.000000.dat (write_results_separate_raw).
 //
 // Grids are raw little-endian float64 in the reference's AoS layout.  combo 0:
 // MHReconstructor<VanLeer<cons>> + Hllc + IdealGas; combo 1: MHReconstructor<
@@ -77,8 +81,16 @@ static int run(const std::string& mode, int64_t nx, int64_t ny, double arg, cons
       simr.configure_cfl(0.9).configure_resolution(1.0 / nx).configure_max_iterations(
           static_cast<int64_t>(arg));
       simr.configure_material(mat);
+      if (mode == "write") simr.configure_sim_time(10.0);
       const fl_sim_result r = simr.simulate(in, out);
       std::printf("run_time %.17g iters %lld\n", r.run_time, (long long)r.iters);
+      if (mode == "write") {
+        simr.write_results(in, out_path);
+        simr.write_results_separate_raw(in, out_path);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return 0;
+      }
       (void)hipMemcpy(host.data(), in.data, n * sizeof(double), hipMemcpyDeviceToHost);
     }
   } catch (const amd::Error& e) {
@@ -93,7 +105,8 @@ static int run(const std::string& mode, int64_t nx, int64_t ny, double arg, cons
 
 int main(int argc, char** argv) {
   if (argc != 9) {
-    std::fprintf(stderr, "usage: %s solve|simulate combo nd nx ny arg in.bin out.bin\n", argv[0]);
+    std::fprintf(stderr, "usage: %s solve|simulate|write combo nd nx ny arg in.bin out\n",
+                 argv[0]);
     return 1;
   }
   const std::string mode = argv[1];

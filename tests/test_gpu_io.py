@@ -1,0 +1,233 @@
+"""Initial conditions and result files on the GPU against the oracle.
+
+* fl_primitive_planes / fl_from_primitive / fl_set_states: every FP64 value
+  bit-identical to the oracle's primitive / conservative conversions and
+  setter (1D and 2D, AoS and SoA, ideal and stiffened gas); FP32 within
+  1e-6 relative.
+* Simulator (configure_*, fill_data, simulate, write_results,
+  write_results_separate_raw) and MultimaterialSimulator (add_material,
+  simulate_mm, write_results): the files are byte-identical to the ones the
+  oracle writes after running the same steps from the same fillers.
+* The reference-named C++ Simulator writes the same bytes.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fluidity_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def S():
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm device")
+    from fluidity_amd import simulator
+    return simulator
+
+
+def _state(nd, seed=0, shape=(24, 20)):
+    rng = np.random.default_rng(seed)
+    if nd == 1:
+        n = shape[0] * shape[1]
+        Wp = np.stack([0.5 + rng.random(n), 0.1 + rng.random(n), rng.random(n) - 0.5], -1)
+    else:
+        ny, nx = shape
+        Wp = np.stack([0.5 + rng.random((ny, nx)), 0.1 + rng.random((ny, nx)),
+                       rng.random((ny, nx)) - 0.5, rng.random((ny, nx)) - 0.5], -1)
+    return Wp
+
+
+def _dev(U, layout, dtype=torch.float64):
+    from fluidity_amd import native as N
+    a = np.moveaxis(U, -1, 0) if layout == N.FL_LAYOUT_SOA else U
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dtype)
+
+
+def _host(T, layout):
+    from fluidity_amd import native as N
+    a = T.double().cpu().numpy()
+    return np.ascontiguousarray(np.moveaxis(a, 0, -1)) if layout == N.FL_LAYOUT_SOA else a
+
+
+def _schemes(oracle, stiff):
+    from fluidity_amd.euler import Scheme
+    from fluidity_amd import native as N
+    if stiff:
+        return (Scheme(eos=N.FL_EOS_STIFF, gamma=4.4, p_inf=6e8 / 1e9),
+                oracle.scheme(gamma=4.4, eos=oracle.EOS_STIFF, p_inf=6e8 / 1e9))
+    return Scheme(gamma=1.4), oracle.scheme(gamma=1.4)
+
+
+@pytest.mark.parametrize("nd", [1, 2])
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("stiff", [False, True])
+def test_conversions_bit_exact(S, oracle, nd, layout, stiff):
+    sch, osch = _schemes(oracle, stiff)
+    Wp = _state(nd, seed=nd + 2 * layout)
+    U = W.prim_to_cons_eos(Wp, sch.gamma, sch.p_inf, stiff)
+    flat_U = U.reshape(-1, 2 + nd)
+    # primitive planes
+    planes = S.primitive_planes(_dev(U, layout), sch, layout).cpu().numpy()
+    for i in range(0, flat_U.shape[0], 7):
+        w = oracle.cons_to_prim(flat_U[i], osch)
+        num = w[1] + (osch.gamma * osch.p_inf if stiff else 0.0)
+        e = num / ((osch.gamma - 1.0) * w[0])
+        assert np.array_equal(planes[:, i].view(np.uint64),
+                              np.array(list(w) + [e]).view(np.uint64)), i
+    # conservative from primitive planes
+    prim = torch.from_numpy(np.ascontiguousarray(Wp.reshape(-1, 2 + nd).T)).to("cuda")
+    out = torch.zeros_like(_dev(U, layout))
+    S.from_primitive(prim, sch, out, layout)
+    got = _host(out, layout).reshape(-1, 2 + nd)
+    flat_W = Wp.reshape(-1, 2 + nd)
+    for i in range(0, flat_W.shape[0], 5):
+        assert np.array_equal(got[i].view(np.uint64),
+                              oracle.prim_to_cons(flat_W[i], osch).view(np.uint64)), i
+
+
+@pytest.mark.parametrize("nd", [1, 2])
+def test_from_primitive_f32(S, oracle, nd):
+    sch, osch = _schemes(oracle, False)
+    Wp = _state(nd, seed=9)
+    prim = torch.from_numpy(np.ascontiguousarray(Wp.reshape(-1, 2 + nd).T)).to("cuda")
+    out = torch.zeros((2 + nd,) + Wp.shape[:-1], dtype=torch.float32, device="cuda")
+    S.from_primitive(prim, sch, out, 1)
+    got = _host(out, 1).reshape(-1, 2 + nd)
+    ref = np.array([oracle.prim_to_cons(w, osch) for w in Wp.reshape(-1, 2 + nd)])
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)) < 1e-6
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_set_states_bit_exact(S, oracle, layout):
+    sch, osch = _schemes(oracle, True)
+    Wp = _state(2, seed=4)
+    U = W.prim_to_cons_eos(Wp, sch.gamma, sch.p_inf, True)
+    phi = np.random.default_rng(5).standard_normal(U.shape[:-1])
+    phi[0, :4] = [0.0, -0.0, np.nan, 1e-310]
+    state = np.array([1.2, 3.4, -0.5, 0.25])
+    Ud = _dev(U, layout)
+    S.set_states(torch.from_numpy(phi).to("cuda"), state, sch, Ud, layout)
+    ref = U.copy()
+    oracle.set_states(phi, state, osch, ref)
+    assert np.array_equal(_host(Ud, layout).view(np.uint64), ref.view(np.uint64))
+
+
+def _fillers_1d():
+    return [("rho", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.125)),
+            ("p", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.1)),
+            ("v_x", lambda p: 0.25 * np.sin(6.0 * p[:, 0]))]
+
+
+def _fillers_2d():
+    return [("rho", lambda p: 1.0 + 0.5 * (p[:, 0] < 0.4) + 0.25 * (p[:, 1] < 0.6)),
+            ("p", lambda p: np.where(p[:, 0] + p[:, 1] < 1.0, 1.0, 0.4)),
+            ("v_x", lambda p: 0.2 * p[:, 1]),
+            ("v_y", lambda p: -0.1 * p[:, 0])]
+
+
+@pytest.mark.parametrize("nd,steps", [(1, 40), (2, 12)])
+def test_simulator_files_match_oracle(S, oracle, tmp_path, nd, steps):
+    from fluidity_amd import native as N
+    sim = S.Simulator(dimensions=nd)
+    sim.configure_resolution(1.0 / 64)
+    sim.configure_dimension(0, 0.0, 1.0)
+    if nd == 2:
+        sim.configure_dimension(1, 0.0, 0.75)
+    sim.configure_cfl(0.9)
+    sim.configure_sim_time(10.0)
+    sim.configure_max_iterations(steps)
+    fillers = _fillers_1d() if nd == 1 else _fillers_2d()
+    sim.fill_data([S.FillInfo(n, f) for n, f in fillers])
+    r = sim.simulate()
+    assert r.iters == steps
+    prefix = str(tmp_path / "gpu")
+    sim.write_results(prefix)
+    sim.write_results_separate_raw(prefix)
+
+    osch = oracle.scheme(gamma=1.4)
+    n = sim.sizes()
+    U0 = oracle.fill_data(n, fillers, osch)
+    U, rt, it, _ = oracle.simulate(U0, osch, 0.9, 1.0 / 64, sim_time=10.0, max_iters=steps)
+    assert it == steps and rt == sim.params.run_time
+    oprefix = str(tmp_path / "orc")
+    oracle.write_ascii(U, osch, oprefix + ".dat", rt, 1.0 / 64)
+    oracle.write_separate_raw(U, osch, 10.0, oprefix)
+    assert open(prefix + ".dat").read() == open(oprefix + ".dat").read()
+    for nm in ["rho", "p", "v_x", "v_y"][:2 + nd]:
+        assert (open(f"{prefix}_{nm}_10.000000.dat").read() ==
+                open(f"{oprefix}_{nm}_10.000000.dat").read()), nm
+    assert N.FL_LAYOUT_SOA == sim.layout
+
+
+def test_multimaterial_files_match_oracle(S, oracle, tmp_path):
+    sim = S.MultimaterialSimulator(dimensions=1, reinit_every=1)
+    sim.configure_resolution(1.0 / 200)
+    sim.configure_dimension(0, 0.0, 1.0)
+    sim.configure_sim_time(10.0)
+    sim.configure_max_iterations(6)
+    membrane = 0.5
+    left = lambda p: p[:, 0].astype(np.float64) - membrane   # noqa: E731
+    right = lambda p: membrane - p[:, 0].astype(np.float64)  # noqa: E731
+    sim.add_material(S.IdealGas(1.4), left, rho=1.0, p=1.0, v_x=0.0)
+    sim.add_material(S.IdealGas(1.667), right, rho=0.125, p=0.1, v_x=0.0)
+    r = sim.simulate_mm()
+    assert r.iters == 6
+    prefix = "mm_gpu"
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        sim.write_results(prefix)
+    finally:
+        os.chdir(cwd)
+
+    n = sim.sizes()
+    pos = oracle.fill_positions(n)
+    s0, s1 = oracle.scheme(gamma=1.4), oracle.scheme(gamma=1.667)
+    phis, Us = [], []
+    for fn, st, sc in ((left, [1.0, 1.0, 0.0], s0), (right, [0.125, 0.1, 0.0], s1)):
+        phi = np.asarray(fn(pos), dtype=np.float64)
+        U = np.zeros((n[0], 3))
+        oracle.set_states(-np.ones(n[0]), np.array(st), sc, U)
+        oracle.set_states(phi, np.array(st), sc, U)
+        phis.append(phi)
+        Us.append(U)
+    U0, U1, p0, p1, rt, it, _ = oracle.mm_simulate(Us[0], Us[1], phis[0], phis[1], s0, s1, 0.9,
+                                                   1.0 / 200, sim_time=10.0, max_iters=6,
+                                                   band=6, extrap_sweeps=8, reinit_every=1)
+    assert it == 6 and rt == sim.params.run_time
+    for m, (U, phi, sc) in enumerate(((U0, p0, s0), (U1, p1, s1))):
+        ref = tmp_path / f"orc_{m}.dat"
+        oracle.write_ascii(U, sc, ref, rt, 1.0 / 200, phi=phi)
+        got = (tmp_path / f"material_{m + 1}_{prefix}.dat").read_text()
+        assert got == ref.read_text(), m
+
+
+def test_shim_simulator_writes_match_oracle(S, oracle, tmp_path):
+    driver = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "shim_driver")
+    if not os.path.exists(driver):
+        pytest.fail("tests/cpp/shim_driver is not built")
+    Wp = _state(2, seed=11, shape=(16, 24))
+    U = W.prim_to_cons(Wp, 1.4)
+    fin = tmp_path / "in.bin"
+    np.ascontiguousarray(U).tofile(fin)
+    prefix = str(tmp_path / "shim")
+    res = subprocess.run([driver, "write", "0", "2", "24", "16", "5", str(fin), prefix],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    osch = oracle.scheme(gamma=1.4)
+    Uo, rt, it, _ = oracle.simulate(U, osch, 0.9, 1.0 / 24, sim_time=10.0, max_iters=5)
+    oprefix = str(tmp_path / "orc")
+    oracle.write_ascii(Uo, osch, oprefix + ".dat", rt, 1.0 / 24)
+    oracle.write_separate_raw(Uo, osch, 10.0, oprefix)
+    assert open(prefix + ".dat").read() == open(oprefix + ".dat").read()
+    for nm in ("rho", "p", "v_x", "v_y"):
+        assert (open(f"{prefix}_{nm}_10.000000.dat").read() ==
+                open(f"{oprefix}_{nm}_10.000000.dat").read()), nm
