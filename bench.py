@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--x-rows", type=int, default=0)
     ap.add_argument("--y-block", type=int, default=0)
     ap.add_argument("--y-rows", type=int, default=0)
-    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost", "batch", "adjoint"],
+    ap.add_argument("--mode", choices=["bench", "sweeps", "ghost", "batch", "adjoint", "io"],
                     default="bench",
                     help="sweeps: a plain device copy of one grid (HBM calibration for "
                          "counter runs) followed by --steps x sweeps and y sweeps")
@@ -77,6 +77,10 @@ def main():
     if args.mode == "batch":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
         print(json.dumps(batched_case(torch, N, W, stream, args.steps)), flush=True)
+        return
+    if args.mode == "io":
+        stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+        print(json.dumps(io_case(torch, N, W, stream)), flush=True)
         return
     if args.mode == "ghost":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
@@ -183,6 +187,7 @@ def main():
 
         variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
         variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
+        variants["io_f64"] = io_case(torch, N, W, stream)
         bf = batched_case(torch, N, W, stream, steps=20)
         for k, v in bf.items():
             v["vs_f64_hllc"] = round(v["value"] / value, 3)
@@ -296,6 +301,65 @@ def batched_case(torch, N, W, stream, steps):
         del b, probs
         torch.cuda.empty_cache()
     return out
+
+
+def io_case(torch, N, W, stream, n=4096, reps=20, text_n=512):
+    """The device halves of the initial-condition and output paths on the
+    4096^2 FP64 grid: conservative -> primitive planes with internal energy
+    (read 32 B, write 40 B per cell) and primitive planes -> conservative
+    (read 32 B, write 32 B per cell), each timed with HIP events on the launch
+    stream over `reps` launches; then the reference's ASCII result file of a
+    512^2 grid written to a temporary directory (host formatting)."""
+    import tempfile
+    from fluidity_amd import simulator as SM
+    from fluidity_amd.euler import Scheme
+    wl = W.riemann_2d(n=n, seed=1001)
+    dev = torch.device("cuda:0")
+    sch = Scheme()
+    with torch.cuda.stream(stream):
+        U = torch.from_numpy(W.aos_to_soa(wl.U)).to(dev)
+        prim = SM.primitive_planes(U, sch, N.FL_LAYOUT_SOA, stream=stream)
+        out = torch.empty_like(U)
+    stream.synchronize()
+    cells = n * n
+    res = {"workload": f"2D {n}x{n} FP64 SoA (seed 1001); ASCII file of a {text_n}^2 grid"}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    with torch.cuda.stream(stream):  # the planes are allocated on the launch stream
+        prim4 = prim[:4].contiguous()
+        SM.from_primitive(prim4, sch, out, N.FL_LAYOUT_SOA, stream=stream)
+        ev[0].record(stream)
+        for _ in range(reps):
+            SM.primitive_planes(U, sch, N.FL_LAYOUT_SOA, stream=stream)
+        ev[1].record(stream)
+        for _ in range(reps):
+            SM.from_primitive(prim4, sch, out, N.FL_LAYOUT_SOA, stream=stream)
+        ev[2].record(stream)
+    stream.synchronize()
+    t_prim = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    t_cons = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    res["primitive_planes_us"] = round(t_prim * 1e6, 1)
+    res["primitive_planes_GBps"] = round(72 * cells / t_prim / 1e9, 1)
+    res["primitive_planes_hbm_frac"] = round(72 * cells / t_prim / 1e9 / HBM_PEAK_GBS, 4)
+    res["from_primitive_us"] = round(t_cons * 1e6, 1)
+    res["from_primitive_GBps"] = round(64 * cells / t_cons / 1e9, 1)
+    res["from_primitive_hbm_frac"] = round(64 * cells / t_cons / 1e9 / HBM_PEAK_GBS, 4)
+    small = W.riemann_2d(n=text_n, seed=1001)
+    sim = SM.Simulator(dimensions=2)
+    sim.configure_resolution(1.0 / text_n)
+    sim.configure_dimension(0, 0.0, 1.0)
+    sim.configure_dimension(1, 0.0, 1.0)
+    sim.state.copy_(torch.from_numpy(W.aos_to_soa(small.U)).to(dev))
+    torch.cuda.synchronize()
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        sim.write_results(os.path.join(d, "io"))
+        el = time.perf_counter() - t0
+        size = os.path.getsize(os.path.join(d, "io.dat"))
+    res["write_results_cells_per_s"] = round(text_n * text_n / el, 1)
+    res["write_results_MBps"] = round(size / el / 1e6, 1)
+    del U, prim, prim4, out
+    torch.cuda.empty_cache()
+    return res
 
 
 def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024):

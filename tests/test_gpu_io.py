@@ -231,3 +231,34 @@ def test_shim_simulator_writes_match_oracle(S, oracle, tmp_path):
     for nm in ("rho", "p", "v_x", "v_y"):
         assert (open(f"{prefix}_{nm}_10.000000.dat").read() ==
                 open(f"{oprefix}_{nm}_10.000000.dat").read()), nm
+
+
+def test_option_manager_creates_simulator(S, oracle, tmp_path):
+    """The reference's validation settings file through the option manager:
+    a 2D HLLC simulator on its domain (at a coarser resolution) that steps
+    like the oracle."""
+    from fluidity_amd.settings import OptionManager, Settings
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "settings_2d_shock.txt")
+    m = OptionManager()
+    m.configure_options(Settings.from_file(fixture))
+    m.set("dimensions", "two").set("flux_method", "hllc")
+    m.params.resolution = 0.02
+    sim = m.create()
+    assert sim.sizes() == [80, 50]
+    amb = m.params.ambient
+    sim.fill_data([("rho", lambda p: np.where(p[:, 0] < 0.25, 2.0, amb["rho"])),
+                   ("p", lambda p: np.where(p[:, 0] < 0.25, 3.0, amb["p"])),
+                   ("v_x", lambda p: np.full(p.shape[0], amb["v_x"])),
+                   ("v_y", lambda p: np.full(p.shape[0], amb["v_y"]))])
+    sim.configure_sim_time(1.0)
+    sim.configure_max_iterations(4)
+    sim.simulate()
+    prefix = str(tmp_path / "opt")
+    sim.write_results(prefix)
+    osch = oracle.scheme(gamma=1.4, flux=oracle.FLUX_HLLC)
+    U0 = oracle.fill_data([80, 50], [("rho", lambda p: np.where(p[:, 0] < 0.25, 2.0, 1.0)),
+                                     ("p", lambda p: np.where(p[:, 0] < 0.25, 3.0, 1.0))], osch)
+    U, rt, it, _ = oracle.simulate(U0, osch, m.params.cfl, 0.02, sim_time=1.0, max_iters=4)
+    oracle.write_ascii(U, osch, str(tmp_path / "orc.dat"), rt, 0.02)
+    assert open(prefix + ".dat").read() == open(tmp_path / "orc.dat").read()
