@@ -245,3 +245,47 @@ def test_simulate_graph_replay_equals_launches(E, nd, steps):
     assert res[0][1] == res[1][1] == steps
     assert res[0][2] == res[1][2] and res[0][3] == res[1][3]
     assert np.array_equal(res[0][0].view(np.uint64), res[1][0].view(np.uint64))
+
+
+def test_headline_workload_full_size_bitwise(E, oracle):
+    """The 4096^2 FP64 HLLC four-quadrant workload itself (seed 1001), after
+    the launch-shape autotuner: 3 steps of the device loop equal 3 steps of
+    the oracle (16 host threads) bit for bit, with the same run time."""
+    import os
+    from fluidity_amd import workloads as W
+    wl = W.riemann_2d(n=4096, seed=1001)
+    os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
+    ref, rt, it, _ = oracle.simulate(wl.U, os_, 0.9, wl.dh, max_iters=3,
+                                     nthreads=min(os.cpu_count() or 1, 16))
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+    a = _dev(wl.U, layout=1)
+    w = torch.empty_like(a)
+    solver.autotune(a, w, reps=1)
+    r = solver.simulate(a, w, 0.9, wl.dh, max_iters=3)
+    got = _host(a, 1)
+    del a, w
+    torch.cuda.empty_cache()
+    assert r.iters == it == 3 and r.run_time == rt
+    assert _bitwise(got, ref), _report(got, ref)
+
+
+def test_fp32_full_size_against_fp64(E):
+    """FP32 bulk on the 4096^2 workload, 100 steps, against the FP64 device
+    loop (bit-identical to the oracle): relative L1 density error <= 1e-5."""
+    from fluidity_amd import workloads as W
+    wl = W.riemann_2d(n=4096, seed=1001)
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+        a = _dev(wl.U, dtype=dt, layout=1)
+        w = torch.empty_like(a)
+        r = solver.simulate(a, w, 0.9, wl.dh, max_iters=100)
+        assert r.iters == 100
+        out[dt] = a[0].double()
+        del a, w
+    rel_l1 = float((out[torch.float32] - out[torch.float64]).abs().sum() /
+                   out[torch.float64].abs().sum())
+    print(f"4096^2 fp32 rel L1 density after 100 steps {rel_l1:.3e}", file=sys.stderr)
+    assert rel_l1 <= 1e-5
+    del out
+    torch.cuda.empty_cache()
