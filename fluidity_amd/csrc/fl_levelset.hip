@@ -175,10 +175,10 @@ __device__ __forceinline__ int mirror32(int i, int n) {
 }
 
 // L = V * H(phi) (levelset_hamiltonian.hpp:63-98, godunov_upwind.hpp:94-166).
+// h1 = 1 / dh, computed once by the caller (the same IEEE quotient).
 __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, int y, double v,
-                         double dh) {
+                         double h1) {
   double result = 0.0;
-  const double h1 = 1.0 / dh;
   const int nx = (int)g.nx, ny = (int)g.ny;
   for (int d = 0; d < g.nd; ++d) {
     double f[7];
@@ -215,12 +215,12 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
 // gridDim.y.
 __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
                                  const double* __restrict__ src, const double* __restrict__ V,
-                                 Grid g, double dt, double dh, double* __restrict__ out) {
+                                 Grid g, double dt, double h1, double* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= g.nx) return;
   for (int y = blockIdx.y; y < g.ny; y += gridDim.y) {
     const int64_t i = (int64_t)y * g.nx + x;
-    const double l = ls_rhs(src, g, x, y, V[i], dh);
+    const double l = ls_rhs(src, g, x, y, V[i], h1);
     if (stage == 0) {
       out[i] = phi_n[i] - dt * l;
     } else if (stage == 1) {
@@ -242,9 +242,10 @@ __device__ __forceinline__ int signum(double x) { return (0.0 < x) - (x < 0.0); 
 
 // Sources (fast_iterative_init, scheme/cuda/fast_iterative_method.cuh:48-85):
 // a cell beside a sign change with the smaller |phi|; distance |phi| there.
-// info = source | (signum(phi) + 1) << 1.
+// info = source | (signum(phi) + 1) << 1.  Both Jacobi buffers start equal.
 __global__ void eik_init_kernel(const double* __restrict__ phi, Grid g,
-                                double* __restrict__ d, uint8_t* __restrict__ info) {
+                                double* __restrict__ d, double* __restrict__ d1,
+                                uint8_t* __restrict__ info) {
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
   const int64_t x = i % g.nx, y = i / g.nx;
@@ -260,7 +261,49 @@ __global__ void eik_init_kernel(const double* __restrict__ phi, Grid g,
     }
   }
   info[i] = (uint8_t)(source | ((signum(c) + 1) << 1));
-  d[i] = source ? fabs(c) : DBL_MAX;
+  const double v = source ? fabs(c) : DBL_MAX;
+  d[i] = v;
+  d1[i] = v;
+}
+
+// Tiles of the eikonal sweeps: kEikTileW cells x kEikTileH rows, one per
+// workgroup of kEikTileW * kEikTileH threads.  src[t] = the tile holds a
+// source.  Launched on (tiles per row, min(tile rows, 65535)) workgroups; tile
+// rows stride by gridDim.y.
+__device__ __forceinline__ void eik_cell_of(int64_t tr, int64_t& x, int64_t& y) {
+  x = (int64_t)blockIdx.x * kEikTileW + (threadIdx.x % kEikTileW);
+  y = tr * kEikTileH + (threadIdx.x / kEikTileW);
+}
+
+__global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_tile_src_kernel(
+    const uint8_t* __restrict__ info, Grid g, int64_t trows, uint8_t* __restrict__ src) {
+  const int64_t segs = gridDim.x;
+  for (int64_t tr = blockIdx.y; tr < trows; tr += gridDim.y) {
+    int64_t x, y;
+    eik_cell_of(tr, x, y);
+    const int has = (x < g.nx && y < g.ny) ? (info[y * g.nx + x] & 1) : 0;
+    const int any = __syncthreads_or(has);
+    if (threadIdx.x == 0) src[tr * segs + blockIdx.x] = (uint8_t)(any != 0);
+  }
+}
+
+// act[t] = some tile within `tr_r` tile rows and `segs_r` tiles of t holds a
+// source.  A cell's distance grows by at least dh/sqrt(2) per axis step from
+// the nearest source (the Godunov update q >= min(a, b) + dh/sqrt(2)), so a
+// cell farther than band * sqrt(2) steps from every source never enters the
+// band and tiles without such cells can be skipped by every sweep.
+__global__ void eik_tile_act_kernel(const uint8_t* __restrict__ src, int64_t trows, int64_t segs,
+                                    int tr_r, int segs_r, uint8_t* __restrict__ act) {
+  const int64_t t = gid();
+  if (t >= trows * segs) return;
+  const int64_t y = t / segs, sx = t % segs;
+  const int64_t y0 = y - tr_r < 0 ? 0 : y - tr_r, y1 = y + tr_r >= trows ? trows - 1 : y + tr_r;
+  const int64_t s0 = sx - segs_r < 0 ? 0 : sx - segs_r;
+  const int64_t s1 = sx + segs_r >= segs ? segs - 1 : sx + segs_r;
+  uint8_t a = 0;
+  for (int64_t yy = y0; yy <= y1 && !a; ++yy)
+    for (int64_t ss = s0; ss <= s1; ++ss) a |= src[yy * segs + ss];
+  act[t] = a;
 }
 
 // Neighbour distance seen from a cell with sign bits sc: negated across the
@@ -283,47 +326,60 @@ __global__ void eik_ctl_init_kernel(EikCtl* c) {
 // band: a cell takes q when q < d and q <= limit.  A sweep after one that
 // changed nothing marks the loop done; later sweeps return at once, so a
 // batch of sweeps can be launched ahead of the convergence test.  Every
-// active sweep writes every cell, so once converged both buffers agree.
-__global__ void eik_sweep_kernel(const double* __restrict__ d, const uint8_t* __restrict__ info,
-                                 Grid g, double dh, double limit, double* __restrict__ e,
-                                 EikCtl* __restrict__ ctl, int k) {
-  const int64_t i = gid();
+// active sweep writes every cell of the active tiles (the others hold their
+// initial value in both buffers), so once converged both buffers agree.
+// Launched like eik_tile_src_kernel.
+__global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
+    const double* __restrict__ d, const uint8_t* __restrict__ info, Grid g, int64_t trows,
+    double dh, double limit, double* __restrict__ e, EikCtl* __restrict__ ctl, int k,
+    const uint8_t* __restrict__ act) {
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
   if (ctl->done) return;
   if (k > 0 && ctl->flags[(k - 1) % 3] == 0) {
-    if (i == 0) ctl->done = 1;
+    if (first) ctl->done = 1;
     return;
   }
-  if (i == 0) {
+  if (first) {
     ctl->flags[(k + 1) % 3] = 0;
     ctl->iters = k + 1;
   }
-  if (i >= g.nx * g.ny) return;
-  const double cur = d[i];
-  double nv = cur;
-  const uint8_t me = info[i];
-  if (!(me & 1)) {
-    const uint8_t sc = me & 6;
-    const int64_t x = i % g.nx, y = i / g.nx;
-    const double fh = 1.0 * dh;
-    const double xm = x > 0 ? eik_nbr(d, info, i - 1, sc) : DBL_MAX;
-    const double xp = x < g.nx - 1 ? eik_nbr(d, info, i + 1, sc) : DBL_MAX;
-    const double a = smin(xm, xp);
-    double q;
-    if (g.nd == 1) {
-      q = a + fh;
-    } else {
-      const double ym = y > 0 ? eik_nbr(d, info, i - g.nx, sc) : DBL_MAX;
-      const double yp = y < g.ny - 1 ? eik_nbr(d, info, i + g.nx, sc) : DBL_MAX;
-      const double b = smin(ym, yp);
-      const double amb = a - b;
-      q = fabs(amb) >= fh ? smin(a, b) + fh : 0.5 * (a + b + sqrt(2.0 * fh * fh - (amb * amb)));
+  const int64_t segs = gridDim.x;
+  bool changed = false;
+  for (int64_t tr = blockIdx.y; tr < trows; tr += gridDim.y) {
+    int64_t x, y;
+    eik_cell_of(tr, x, y);
+    if (!act[tr * segs + blockIdx.x] || x >= g.nx || y >= g.ny) continue;
+    const int64_t i = y * g.nx + x;
+    const double cur = d[i];
+    double nv = cur;
+    const uint8_t me = info[i];
+    if (!(me & 1)) {
+      const uint8_t sc = me & 6;
+      const double fh = 1.0 * dh;
+      const double xm = x > 0 ? eik_nbr(d, info, i - 1, sc) : DBL_MAX;
+      const double xp = x < g.nx - 1 ? eik_nbr(d, info, i + 1, sc) : DBL_MAX;
+      const double a = smin(xm, xp);
+      double q;
+      if (g.nd == 1) {
+        q = a + fh;
+      } else {
+        const double ym = y > 0 ? eik_nbr(d, info, i - g.nx, sc) : DBL_MAX;
+        const double yp = y < g.ny - 1 ? eik_nbr(d, info, i + g.nx, sc) : DBL_MAX;
+        const double b = smin(ym, yp);
+        const double amb = a - b;
+        q = fabs(amb) >= fh ? smin(a, b) + fh
+                            : 0.5 * (a + b + sqrt(2.0 * fh * fh - (amb * amb)));
+      }
+      if (q < cur && q <= limit) {
+        nv = q;
+        changed = true;
+      }
     }
-    if (q < cur && q <= limit) {
-      nv = q;
-      ctl->flags[k % 3] = 1;
-    }
+    e[i] = nv;
   }
-  e[i] = nv;
+  // one store of the change flag per wave
+  const unsigned long long m = __ballot(changed);
+  if (m && (int)(threadIdx.x % 64) == __ffsll((long long)m) - 1) ctl->flags[k % 3] = 1;
 }
 
 __global__ void eik_finish_kernel(const double* __restrict__ phi, const double* __restrict__ d,
@@ -455,10 +511,13 @@ __device__ void interp_state(const StateView& v, const Grid& g, int64_t x, int64
 template <int ND>
 __global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* __restrict__ phi_o,
                                 StateView um, StateView uo, Eos em, Eos eo, Grid g, double dh,
-                                double* __restrict__ star, uint8_t* __restrict__ has_star) {
+                                double* __restrict__ star, uint8_t* __restrict__ has_star,
+                                int* __restrict__ count_reset) {
   constexpr int S = 2 + ND;
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
+  // zeroes the band-list count the extrapolation that follows appends to
+  if (count_reset && i == 0) *count_reset = 0;
   const int64_t x = i % g.nx, y = i / g.nx;
   has_star[i] = 0;
   const double p = phi_m[i];
@@ -514,20 +573,25 @@ __global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* 
 
 __device__ __forceinline__ bool in_band(double p, double upper) { return p > 0.0 && p <= upper; }
 
-// Extrapolation source field, in both Jacobi buffers: the material's own state
-// (AoS scratch), with the star state at interfacial cells.  Only the cells an
-// extrapolation sweep can read are written: the band cells and their
-// neighbours (a band cell reads itself and its upwind neighbours).
+// Extrapolation source field, in both Jacobi buffers, and the band list, in
+// one pass: the material's own state (AoS scratch), with the star state at
+// interfacial cells.  Only the cells an extrapolation sweep can read are
+// written: the band cells and their neighbours (a band cell reads itself and
+// its upwind neighbours).  Band cells 0 < phi <= upper are appended to `list`
+// in no particular order: each band cell's update is independent of the
+// others within a sweep.
 template <int ND>
-__global__ void gfm_seed_kernel(const double* __restrict__ phi_m, double upper, StateView um,
-                                const double* __restrict__ star,
-                                const uint8_t* __restrict__ has_star, Grid g,
-                                double* __restrict__ A, double* __restrict__ B) {
+__global__ void gfm_band_seed_kernel(const double* __restrict__ phi_m, double upper,
+                                     StateView um, const double* __restrict__ star,
+                                     const uint8_t* __restrict__ has_star, Grid g,
+                                     double* __restrict__ A, double* __restrict__ B,
+                                     int* __restrict__ list, int* __restrict__ count) {
   constexpr int S = 2 + ND;
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
   const int64_t x = i % g.nx, y = i / g.nx;
-  bool needed = in_band(phi_m[i], upper);
+  const bool band = in_band(phi_m[i], upper);
+  bool needed = band;
   if (x > 0) needed |= in_band(phi_m[i - 1], upper);
   if (x < g.nx - 1) needed |= in_band(phi_m[i + 1], upper);
   if (ND > 1) {
@@ -535,6 +599,7 @@ __global__ void gfm_seed_kernel(const double* __restrict__ phi_m, double upper, 
     if (y < g.ny - 1) needed |= in_band(phi_m[i + g.nx], upper);
   }
   if (!needed) return;
+  if (band) list[atomicAdd(count, 1)] = (int)i;
   double u[S];
   if (has_star[i]) {
 #pragma unroll
@@ -549,27 +614,15 @@ __global__ void gfm_seed_kernel(const double* __restrict__ phi_m, double upper, 
   }
 }
 
-// Indices of the band cells 0 < phi <= upper (in no particular order: each
-// band cell's update is independent of the others within a sweep).
-__global__ void gfm_band_kernel(const double* __restrict__ phi_m, Grid g, double upper,
-                                int* __restrict__ list, int* __restrict__ count) {
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
-  if (in_band(phi_m[i], upper)) list[atomicAdd(count, 1)] = (int)i;
-}
-
 __global__ void count_clear_kernel(int* count) { *count = 0; }
 
-// GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202),
-// one Jacobi sweep over the ghost band 0 < phi <= 6 dh / sqrt(2).
+// GhostCellExtrapolator::invoke (ghost_fluid/ghost_fluid_extrapolator.hpp:54-202):
+// the Jacobi update of band cell i from buffer A into buffer B.
 template <int ND>
-__global__ void gfm_extrap_kernel(const double* __restrict__ phi_m, const double* __restrict__ A,
-                                  Grid g, double dh, const int* __restrict__ list,
-                                  const int* __restrict__ count, double* __restrict__ B) {
+__device__ __forceinline__ void extrap_cell(const double* __restrict__ phi_m,
+                                            const double* __restrict__ A, Grid g, double dh,
+                                            int64_t i, double* __restrict__ B) {
   constexpr int S = 2 + ND;
-  const int n = *count;
-  for (int k = (int)gid(); k < n; k += gridDim.x * blockDim.x) {
-  const int64_t i = list[k];
   const double p = phi_m[i];
   const int64_t x = i % g.nx, y = i / g.nx;
   double nrm[2] = {0.0, 0.0};
@@ -597,17 +650,35 @@ __global__ void gfm_extrap_kernel(const double* __restrict__ phi_m, const double
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) B[i * S + k] = (l1 > 1e-2) ? acc[k] / l1 : acc[k];
-  }
 }
 
+// One extrapolation sweep of up to two materials in one launch: the first
+// kExtrapBlocks workgroups walk job 0's band list, the next kExtrapBlocks job
+// 1's.  Sweep k reads buffer A (k even) or B (k odd) and writes the other; the
+// last sweep also writes the band cells into the material's state (the
+// write-back).  With sweep < 0 the seeded buffer A is written back as it is
+// (no sweeps).
 template <int ND>
-__global__ void gfm_writeback_kernel(const double* __restrict__ A, const int* __restrict__ list,
-                                     const int* __restrict__ count, StateView um) {
+__global__ void __launch_bounds__(256) gfm_extrap_kernel(ExtrapJob j0, ExtrapJob j1, Grid g,
+                                                         double dh, int sweep, int last) {
   constexpr int S = 2 + ND;
-  const int n = *count;
-  for (int k = (int)gid(); k < n; k += gridDim.x * blockDim.x) {
-    const int64_t i = list[k];
-    store_state<S>(um, i, A + i * S);
+  const ExtrapJob j = blockIdx.x < kExtrapBlocks ? j0 : j1;
+  const int n = *j.count;
+  const int first = (int)(blockIdx.x % kExtrapBlocks) * blockDim.x + threadIdx.x;
+  const int stride = kExtrapBlocks * blockDim.x;
+  if (sweep < 0) {
+    for (int k = first; k < n; k += stride) {
+      const int64_t i = j.list[k];
+      store_state<S>(j.um, i, j.A + i * S);
+    }
+    return;
+  }
+  const double* src = (sweep & 1) ? j.B : j.A;
+  double* dst = (sweep & 1) ? j.A : j.B;
+  for (int k = first; k < n; k += stride) {
+    const int64_t i = j.list[k];
+    extrap_cell<ND>(j.phi, src, g, dh, i, dst);
+    if (last) store_state<S>(j.um, i, dst + i * S);
   }
 }
 
@@ -630,14 +701,12 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
                                                StateView, int, Grid, double, int, double*); \
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
-                                               uint8_t*);                                   \
-  template __global__ void gfm_seed_kernel<ND>(const double*, double, StateView,            \
-                                               const double*, const uint8_t*, Grid, double*, \
-                                               double*);                                    \
-  template __global__ void gfm_extrap_kernel<ND>(const double*, const double*, Grid, double,  \
-                                                 const int*, const int*, double*);           \
-  template __global__ void gfm_writeback_kernel<ND>(const double*, const int*, const int*,   \
-                                                    StateView);                              \
+                                               uint8_t*, int*);                             \
+  template __global__ void gfm_band_seed_kernel<ND>(const double*, double, StateView,       \
+                                                    const double*, const uint8_t*, Grid,      \
+                                                    double*, double*, int*, int*);            \
+  template __global__ void gfm_extrap_kernel<ND>(ExtrapJob, ExtrapJob, Grid, double, int,   \
+                                                 int);                                        \
   template __global__ void combine_kernel<ND>(const double*, StateView, StateView, Grid,     \
                                               StateView);
 

@@ -42,30 +42,44 @@ __global__ void velocity_kernel(const double* phi0, const double* phi1, StateVie
                                 StateView u1, int nmat, Grid g, double dh, int target, double* V);
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
-                                 const double* V, Grid g, double dt, double dh, double* out);
-__global__ void eik_init_kernel(const double* phi, Grid g, double* d, uint8_t* info);
+                                 const double* V, Grid g, double dt, double h1, double* out);
+__global__ void eik_init_kernel(const double* phi, Grid g, double* d, double* d1, uint8_t* info);
+// Eikonal tiles: kEikTileW cells x kEikTileH rows (one workgroup each).
+constexpr int kEikTileW = 64;
+constexpr int kEikTileH = 4;
+__global__ void eik_tile_src_kernel(const uint8_t* info, Grid g, int64_t trows, uint8_t* src);
+__global__ void eik_tile_act_kernel(const uint8_t* src, int64_t trows, int64_t segs, int tr_r,
+                                    int segs_r, uint8_t* act);
 __global__ void eik_ctl_init_kernel(EikCtl* c);
-__global__ void eik_sweep_kernel(const double* d, const uint8_t* info, Grid g, double dh,
-                                 double limit, double* e, EikCtl* ctl, int k);
+__global__ void eik_sweep_kernel(const double* d, const uint8_t* info, Grid g, int64_t trows,
+                                 double dh, double limit, double* e, EikCtl* ctl, int k,
+                                 const uint8_t* act);
 __global__ void eik_finish_kernel(const double* phi, const double* d, Grid g, double limit,
                                   double clamp, double* out, uint8_t* band_mask);
 template <int ND>
 __global__ void gfm_star_kernel(const double* phi_m, const double* phi_o, StateView um,
                                 StateView uo, Eos em, Eos eo, Grid g, double dh, double* star,
-                                uint8_t* has_star);
+                                uint8_t* has_star, int* count_reset);
+// One extrapolation job: a material's level set and state, its two Jacobi
+// buffers (AoS), and its band list with the list's count.
+struct ExtrapJob {
+  const double* phi;
+  StateView um;
+  double* A;
+  double* B;
+  const int* list;
+  int* count;
+};
+constexpr int kExtrapBlocks = 128;  // workgroups per job of one sweep launch
+
 template <int ND>
-__global__ void gfm_seed_kernel(const double* phi_m, double upper, StateView um,
-                                const double* star, const uint8_t* has_star, Grid g, double* A,
-                                double* B);
-__global__ void gfm_band_kernel(const double* phi_m, Grid g, double upper, int* list,
-                                int* count);
+__global__ void gfm_band_seed_kernel(const double* phi_m, double upper, StateView um,
+                                     const double* star, const uint8_t* has_star, Grid g,
+                                     double* A, double* B, int* list, int* count);
 __global__ void count_clear_kernel(int* count);
 template <int ND>
-__global__ void gfm_extrap_kernel(const double* phi_m, const double* A, Grid g, double dh,
-                                  const int* list, const int* count, double* B);
-template <int ND>
-__global__ void gfm_writeback_kernel(const double* A, const int* list, const int* count,
-                                     StateView um);
+__global__ void gfm_extrap_kernel(ExtrapJob j0, ExtrapJob j1, Grid g, double dh, int sweep,
+                                  int last);
 template <int ND>
 __global__ void combine_kernel(const double* phi0, StateView u0, StateView u1, Grid g,
                                StateView out);

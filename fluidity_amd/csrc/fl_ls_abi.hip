@@ -98,12 +98,13 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
   double* s2 = work + cells;
   const dim3 grid((unsigned)((g.nx + kBlock - 1) / kBlock),
                   (unsigned)(g.ny < 65535 ? g.ny : 65535));
-  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, dh,
+  const double h1 = 1.0 / dh;
+  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, h1,
                      s1);
   hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1,
-                     V, g, dt, dh, s2);
+                     V, g, dt, h1, s2);
   hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2,
-                     V, g, dt, dh, out);
+                     V, g, dt, h1, out);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -115,9 +116,14 @@ int launch_project(double* phi0, double* phi1, int64_t cells, hipStream_t st) {
   return FL_OK;
 }
 
+int64_t eik_segs(const ls::Grid& g) { return (g.nx + ls::kEikTileW - 1) / ls::kEikTileW; }
+int64_t eik_trows(const ls::Grid& g) { return (g.ny + ls::kEikTileH - 1) / ls::kEikTileH; }
+
+// Two distance buffers, the per-cell info bytes and two tile-flag arrays.
 int64_t reinit_work_bytes(const ls::Grid& g) {
   const int64_t cells = g.nx * g.ny;
-  return 2 * align256(cells * 8) + align256(cells);
+  const int64_t tiles = eik_segs(g) * eik_trows(g);
+  return 2 * align256(cells * 8) + align256(cells) + 2 * align256(tiles);
 }
 
 int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, int band,
@@ -127,10 +133,24 @@ int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, i
   char* w = (char*)work;
   double* buf[2] = {(double*)w, (double*)(w + align256(cells * 8))};
   uint8_t* info = (uint8_t*)(w + 2 * align256(cells * 8));
+  const int64_t segs = eik_segs(g), trows = eik_trows(g), tiles = segs * trows;
+  uint8_t* src = info + align256(cells);
+  uint8_t* act = src + align256(tiles);
   const double limit = band * dh;
   const double clamp = (band + 1) * dh;
   const dim3 grid = grid_for(cells);
-  hipLaunchKernelGGL(ls::eik_init_kernel, grid, dim3(kBlock), 0, st, phi, g, buf[0], info);
+  const dim3 tgrid((unsigned)segs, (unsigned)(trows < 65535 ? trows : 65535));
+  const dim3 tblock(ls::kEikTileW * ls::kEikTileH);
+  // cells within which a source can pull a cell into the band, in tiles
+  const int reach = (int)ceil(band * sqrt(2.0)) + 2;
+  const int segs_r = (reach + ls::kEikTileW - 1) / ls::kEikTileW;
+  const int tr_r = g.nd > 1 ? (reach + ls::kEikTileH - 1) / ls::kEikTileH : 0;
+  hipLaunchKernelGGL(ls::eik_init_kernel, grid, dim3(kBlock), 0, st, phi, g, buf[0], buf[1],
+                     info);
+  hipLaunchKernelGGL(ls::eik_tile_src_kernel, tgrid, tblock, 0, st, (const uint8_t*)info, g,
+                     trows, src);
+  hipLaunchKernelGGL(ls::eik_tile_act_kernel, grid_for(tiles), dim3(kBlock), 0, st,
+                     (const uint8_t*)src, trows, segs, tr_r, segs_r, act);
   hipLaunchKernelGGL(ls::eik_ctl_init_kernel, dim3(1), dim3(1), 0, st, c->eik);
   FL_HIP(hipGetLastError());
   // Every sweep strictly lowers some cell, so the loop ends; the bound only
@@ -140,9 +160,9 @@ int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, i
   for (;;) {
     for (int b = 0; b < kEikBatch; ++b, ++launched) {
       const int k = (int)launched;
-      hipLaunchKernelGGL(ls::eik_sweep_kernel, grid, dim3(kBlock), 0, st,
-                         (const double*)buf[k & 1], (const uint8_t*)info, g, dh, limit,
-                         buf[(k + 1) & 1], c->eik, k);
+      hipLaunchKernelGGL(ls::eik_sweep_kernel, tgrid, tblock, 0, st,
+                         (const double*)buf[k & 1], (const uint8_t*)info, g, trows, dh, limit,
+                         buf[(k + 1) & 1], c->eik, k, (const uint8_t*)act);
     }
     FL_HIP(hipGetLastError());
     FL_HIP(hipMemcpyAsync(c->host_eik, c->eik, sizeof(ls::EikCtl), hipMemcpyDeviceToHost, st));
@@ -161,16 +181,21 @@ int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, i
   return FL_OK;
 }
 
+// count_reset (nullable): a band-list count to zero for the extrapolation
+// that follows.
 int launch_star(const double* phi_m, const double* phi_o, const fl_field* sm,
                 const fl_field* so, const fl_scheme* em, const fl_scheme* eo, const ls::Grid& g,
-                double dh, double* star, uint8_t* has, hipStream_t st) {
+                double dh, double* star, uint8_t* has, hipStream_t st,
+                int* count_reset = nullptr) {
   const dim3 grid = grid_for(g.nx * g.ny);
   if (g.nd == 1)
     hipLaunchKernelGGL(ls::gfm_star_kernel<1>, grid, dim3(kBlock), 0, st, phi_m, phi_o,
-                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has);
+                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has,
+                       count_reset);
   else
     hipLaunchKernelGGL(ls::gfm_star_kernel<2>, grid, dim3(kBlock), 0, st, phi_m, phi_o,
-                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has);
+                       view_of(sm), view_of(so), eos_of(em), eos_of(eo), g, dh, star, has,
+                       count_reset);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -181,40 +206,61 @@ int64_t extrap_work_bytes(const ls::Grid& g) {
   return 2 * align256(cells * (2 + g.nd) * 8) + align256(cells * 4) + 256;
 }
 
-template <int ND>
-void extrapolate_nd(const double* phi_m, ls::StateView um, const double* star,
-                    const uint8_t* has, const ls::Grid& g, double dh, int sweeps, void* work,
-                    hipStream_t st) {
+// The pieces of one material's extrapolation scratch (extrap_work_bytes).
+ls::ExtrapJob extrap_job(const double* phi_m, ls::StateView um, const ls::Grid& g, void* work) {
   const int64_t cells = g.nx * g.ny;
-  const dim3 grid = grid_for(cells);
   char* w = (char*)work;
-  const int64_t bb = align256(cells * (2 + ND) * 8);
-  double* buf[2] = {(double*)w, (double*)(w + bb)};
-  int* list = (int*)(w + 2 * bb);
-  int* count = (int*)(w + 2 * bb + align256(cells * 4));
-  const double upper = 6.0 * dh * (1.0 / sqrt(2.0));
-  // band cells are a thin ring: a fixed grid walks the list
-  const dim3 lgrid(256);
-  hipLaunchKernelGGL(ls::count_clear_kernel, dim3(1), dim3(1), 0, st, count);
-  hipLaunchKernelGGL(ls::gfm_band_kernel, grid, dim3(kBlock), 0, st, phi_m, g, upper, list,
-                     count);
-  hipLaunchKernelGGL(ls::gfm_seed_kernel<ND>, grid, dim3(kBlock), 0, st, phi_m, upper, um, star,
-                     has, g, buf[0], buf[1]);
-  for (int k = 0; k < sweeps; ++k)
-    hipLaunchKernelGGL(ls::gfm_extrap_kernel<ND>, lgrid, dim3(kBlock), 0, st, phi_m,
-                       (const double*)buf[k & 1], g, dh, (const int*)list, (const int*)count,
-                       buf[(k + 1) & 1]);
-  hipLaunchKernelGGL(ls::gfm_writeback_kernel<ND>, lgrid, dim3(kBlock), 0, st,
-                     (const double*)buf[sweeps & 1], (const int*)list, (const int*)count, um);
+  const int64_t bb = align256(cells * (2 + g.nd) * 8);
+  ls::ExtrapJob j;
+  j.phi = phi_m;
+  j.um = um;
+  j.A = (double*)w;
+  j.B = (double*)(w + bb);
+  j.list = (const int*)(w + 2 * bb);
+  j.count = (int*)(w + 2 * bb + align256(cells * 4));
+  return j;
+}
+
+double extrap_upper(double dh) { return 6.0 * dh * (1.0 / sqrt(2.0)); }
+
+// Seeds the Jacobi buffers and lists the band of one material (the job's
+// count must be zero).
+void band_seed(const ls::ExtrapJob& j, const double* star, const uint8_t* has,
+               const ls::Grid& g, double dh, hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::gfm_band_seed_kernel<1>, grid, dim3(kBlock), 0, st, j.phi,
+                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, (int*)j.list, j.count);
+  else
+    hipLaunchKernelGGL(ls::gfm_band_seed_kernel<2>, grid, dim3(kBlock), 0, st, j.phi,
+                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, (int*)j.list, j.count);
+}
+
+// The sweeps and write-back of `njobs` (1 or 2) materials: one launch per
+// sweep covers every job; the last sweep writes the band into the states.
+void extrap_sweeps(const ls::ExtrapJob& j0, const ls::ExtrapJob& j1, int njobs,
+                   const ls::Grid& g, double dh, int sweeps, hipStream_t st) {
+  const dim3 grid(ls::kExtrapBlocks * njobs);
+  const int launches = sweeps > 0 ? sweeps : 1;
+  for (int k = 0; k < launches; ++k) {
+    const int sweep = sweeps > 0 ? k : -1;
+    const int last = k == launches - 1;
+    if (g.nd == 1)
+      hipLaunchKernelGGL(ls::gfm_extrap_kernel<1>, grid, dim3(kBlock), 0, st, j0, j1, g, dh,
+                         sweep, last);
+    else
+      hipLaunchKernelGGL(ls::gfm_extrap_kernel<2>, grid, dim3(kBlock), 0, st, j0, j1, g, dh,
+                         sweep, last);
+  }
 }
 
 int launch_extrapolate(const double* phi_m, const fl_field* sm, const double* star,
                        const uint8_t* has, const ls::Grid& g, double dh, int sweeps,
                        void* work, hipStream_t st) {
-  if (g.nd == 1)
-    extrapolate_nd<1>(phi_m, view_of(sm), star, has, g, dh, sweeps, work, st);
-  else
-    extrapolate_nd<2>(phi_m, view_of(sm), star, has, g, dh, sweeps, work, st);
+  const ls::ExtrapJob j = extrap_job(phi_m, view_of(sm), g, work);
+  hipLaunchKernelGGL(ls::count_clear_kernel, dim3(1), dim3(1), 0, st, j.count);
+  band_seed(j, star, has, g, dh, st);
+  extrap_sweeps(j, j, 1, g, dh, sweeps, st);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -235,7 +281,7 @@ int launch_combine(const double* phi0, const fl_field* s0, const fl_field* s1,
 // Scratch layout of fl_mm_simulate.
 struct MmWork {
   double* star[2];
-  void* extrap;     // extrap_work_bytes
+  void* extrap[2];  // extrap_work_bytes each
   double* V[2];
   double* adv;      // 2 * cells
   double* tphi[2];
@@ -257,7 +303,8 @@ MmWork mm_layout(void* base, const ls::Grid& g) {
   MmWork w;
   w.star[0] = (double*)take(cells * S * 8);
   w.star[1] = (double*)take(cells * S * 8);
-  w.extrap = take(extrap_work_bytes(g));
+  w.extrap[0] = take(extrap_work_bytes(g));
+  w.extrap[1] = take(extrap_work_bytes(g));
   w.V[0] = (double*)take(cells * 8);
   w.V[1] = (double*)take(cells * 8);
   w.adv = (double*)take(2 * cells * 8);
@@ -443,16 +490,19 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
   while (t < p->sim_time && it < p->max_iters) {
     // ghost cells: star states of both materials from the same data, then
     // extrapolation into each material's ghost band
+    ls::ExtrapJob jobs[2];
+    for (int m = 0; m < 2; ++m)
+      jobs[m] = extrap_job(mats[m].phi, view_of(&mats[m].state), g, w.extrap[m]);
     for (int m = 0; m < 2; ++m) {
       const int o = 1 - m;
       if ((rc = launch_star(mats[m].phi, mats[o].phi, &mats[m].state, &mats[o].state,
-                            &mats[m].scheme, &mats[o].scheme, g, dh, w.star[m], w.has[m], st)))
+                            &mats[m].scheme, &mats[o].scheme, g, dh, w.star[m], w.has[m], st,
+                            jobs[m].count)))
         return rc;
     }
-    for (int m = 0; m < 2; ++m)
-      if ((rc = launch_extrapolate(mats[m].phi, &mats[m].state, w.star[m], w.has[m], g, dh,
-                                   sweeps, w.extrap, st)))
-        return rc;
+    for (int m = 0; m < 2; ++m) band_seed(jobs[m], w.star[m], w.has[m], g, dh, st);
+    extrap_sweeps(jobs[0], jobs[1], 2, g, dh, sweeps, st);
+    FL_HIP(hipGetLastError());
     // dt from the maximum wavespeed over both grids
     hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
     for (int m = 0; m < 2; ++m)
