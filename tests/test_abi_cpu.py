@@ -121,3 +121,20 @@ def test_simulate_rejects_bad_params(N):
 def test_tuning_validation(N):
     t = N.Tuning(100, 0, 0, 0)
     assert N.lib.fl_ctx_set_tuning(None, C.byref(t)) == N.FL_ERR_ARGUMENT
+
+
+def test_io_entry_points_reject_bad_arguments(N):
+    """The initial-condition and output entry points fail with
+    FL_ERR_ARGUMENT (and a message) before any device work."""
+    f = N.field(0x1000, 2, (8, 8))
+    s = N.scheme()
+    lib = N.lib
+    assert lib.fl_primitive_planes(None, C.byref(f), C.byref(s), 0x2000, None) == N.FL_ERR_ARGUMENT
+    assert lib.fl_from_primitive(None, 0x2000, C.byref(s), C.byref(f), None) == N.FL_ERR_ARGUMENT
+    w = (C.c_double * 4)(1.0, 1.0, 0.0, 0.0)
+    assert lib.fl_set_states(None, 0x2000, w, C.byref(s), C.byref(f), None) == N.FL_ERR_ARGUMENT
+    assert lib.fl_write_results(None, C.byref(f), C.byref(s), None, 0.0, 0.1, b"x.dat",
+                                None) == N.FL_ERR_ARGUMENT
+    assert lib.fl_write_results_separate_raw(None, C.byref(f), C.byref(s), 0.0, b"x",
+                                             None) == N.FL_ERR_ARGUMENT
+    assert b"ctx" in lib.fl_last_error()
