@@ -289,3 +289,32 @@ def test_fp32_full_size_against_fp64(E):
     assert rel_l1 <= 1e-5
     del out
     torch.cuda.empty_cache()
+
+
+def test_large_grid_constant_in_y_equals_1d(E, oracle):
+    """Maximum-size indexing: a 65536 x 8192 FP64 SoA grid (537 M cells, each
+    component plane 4.3 GB, element offsets beyond 2^31) holding a 1D Riemann
+    problem constant in y with v = 0.  The y fluxes cancel exactly and the
+    flattening factors across y are 1, so after 2 steps every row equals the
+    oracle's 1D run of the same line bit for bit, with the same run time."""
+    from fluidity_amd import workloads as W
+    if torch.cuda.get_device_properties(0).total_memory < 64 * 2**30:
+        pytest.skip("needs 64 GiB of device memory")
+    nx, ny, steps = 65536, 8192, 2
+    r1 = W.riemann_1d(n=nx, seed=1000)
+    os1 = _oscheme(oracle, 0, 0, 0, (0,))
+    ref, rt, it, _ = oracle.simulate(r1.U, os1, 0.9, r1.dh, max_iters=steps)
+    row = torch.zeros((4, 1, nx), dtype=torch.float64, device="cuda")
+    row[:3, 0, :] = torch.from_numpy(np.ascontiguousarray(r1.U.T)).to("cuda")
+    a = row.expand(4, ny, nx).contiguous()
+    w = torch.empty_like(a)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+    r = solver.simulate(a, w, 0.9, r1.dh, max_iters=steps)
+    del w
+    assert r.iters == it == steps and r.run_time == rt
+    assert torch.equal(a, a[:, :1, :].expand(4, ny, nx))
+    got = a[:, 0, :].cpu().numpy()
+    del a, row
+    torch.cuda.empty_cache()
+    assert np.all(got[3] == 0.0)
+    assert _bitwise(np.ascontiguousarray(got[:3].T), ref), _report(got[:3].T, ref)
