@@ -72,7 +72,7 @@ int check_scheme(const fl_scheme* s, int ndim) {
     return fail(FL_ERR_ARGUMENT, "scheme->limiter %d invalid", s->limiter);
   if (s->form != FL_FORM_CONSERVATIVE && s->form != FL_FORM_PRIMITIVE)
     return fail(FL_ERR_ARGUMENT, "scheme->form %d invalid", s->form);
-  if (s->flux != FL_FLUX_HLLC && s->flux != FL_FLUX_FORCE)
+  if (s->flux < FL_FLUX_HLLC || s->flux > FL_FLUX_RICHTMYER)
     return fail(FL_ERR_ARGUMENT, "scheme->flux %d invalid", s->flux);
   if (s->eos != FL_EOS_IDEAL && s->eos != FL_EOS_STIFF)
     return fail(FL_ERR_ARGUMENT, "scheme->eos %d invalid", s->eos);
@@ -133,6 +133,18 @@ int fit_rows(int rows, int64_t bands, int64_t ny) {
   return rows;
 }
 
+// One sweep kernel of flux FLUX: the 1D sweep, or the x or y sweep of a 2D grid.
+template <typename T, int FLUX>
+void launch_flux(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
+                 hipStream_t stream) {
+  if (ndim == 1)
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX>), grid, dim3(B), 0, stream, a);
+  else if (dim == 0)
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX>), grid, dim3(B), 0, stream, a);
+  else
+    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX>), grid, dim3(B), 0, stream, a);
+}
+
 template <typename T>
 int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
                    const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
@@ -143,31 +155,30 @@ int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* ou
     tu.x_rows = fit_rows(tu.x_rows, (nx + tu.x_block - 3) / (tu.x_block - 2), ny);
     tu.y_rows = fit_rows(tu.y_rows, (nx + tu.y_block - 1) / tu.y_block, ny);
   }
+  dim3 grid;
+  int B;
+  fl::SweepArgs<T> a;
   if (in->ndim == 1) {
-    const int B = tu.x_block;
-    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, 1);
-    dim3 grid((unsigned)((a.nx + (B - 3)) / (B - 2)), 1, 1);
-    if (s->flux == FL_FLUX_HLLC)
-      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
-    else
-      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+    B = tu.x_block;
+    a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, 1);
+    grid = dim3((unsigned)((a.nx + (B - 3)) / (B - 2)), 1, 1);
   } else if (dim == 0) {
-    const int B = tu.x_block;
-    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.x_rows);
-    dim3 grid((unsigned)((a.nx + (B - 3)) / (B - 2)),
-              (unsigned)((a.ny + tu.x_rows - 1) / tu.x_rows), 1);
-    if (s->flux == FL_FLUX_HLLC)
-      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
-    else
-      hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+    B = tu.x_block;
+    a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.x_rows);
+    grid = dim3((unsigned)((a.nx + (B - 3)) / (B - 2)),
+                (unsigned)((a.ny + tu.x_rows - 1) / tu.x_rows), 1);
   } else {
-    const int B = tu.y_block;
-    fl::SweepArgs<T> a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.y_rows);
-    dim3 grid((unsigned)((a.nx + B - 1) / B), (unsigned)((a.ny + tu.y_rows - 1) / tu.y_rows), 1);
-    if (s->flux == FL_FLUX_HLLC)
-      hipLaunchKernelGGL((fl::sweep_y_kernel<T, fl::kFluxHllc>), grid, dim3(B), 0, stream, a);
-    else
-      hipLaunchKernelGGL((fl::sweep_y_kernel<T, fl::kFluxForce>), grid, dim3(B), 0, stream, a);
+    B = tu.y_block;
+    a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.y_rows);
+    grid = dim3((unsigned)((a.nx + B - 1) / B), (unsigned)((a.ny + tu.y_rows - 1) / tu.y_rows), 1);
+  }
+  switch (s->flux) {
+    case FL_FLUX_HLLC: launch_flux<T, fl::kFluxHllc>(in->ndim, dim, grid, B, a, stream); break;
+    case FL_FLUX_FORCE: launch_flux<T, fl::kFluxForce>(in->ndim, dim, grid, B, a, stream); break;
+    case FL_FLUX_LAX_FRIEDRICHS:
+      launch_flux<T, fl::kFluxLaxFriedrichs>(in->ndim, dim, grid, B, a, stream);
+      break;
+    default: launch_flux<T, fl::kFluxRichtmyer>(in->ndim, dim, grid, B, a, stream); break;
   }
   FL_HIP(hipGetLastError());
   return FL_OK;

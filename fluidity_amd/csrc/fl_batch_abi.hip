@@ -251,6 +251,8 @@ int fl_batch_create(fl_context* c, const fl_batch_problem* problems, int nprob, 
   if (nprob < 1 || nprob > (1 << 20)) return fail(FL_ERR_ARGUMENT, "nprob %d out of range", nprob);
   const int dtype = problems[0].state.dtype;
   const int flux = problems[0].scheme.flux;
+  if (flux != FL_FLUX_HLLC && flux != FL_FLUX_FORCE)
+    return fail(FL_ERR_UNSUPPORTED, "batches run HLLC or FORCE (flux %d)", flux);
   for (int p = 0; p < nprob; ++p) {
     const fl_batch_problem& q = problems[p];
     int rc = check_field(&q.state, "problem state");

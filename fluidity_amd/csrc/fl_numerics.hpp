@@ -20,7 +20,7 @@ namespace fl {
 
 enum : int { kLimVanLeer = 0, kLimSuperbee = 1 };
 enum : int { kFormCons = 0, kFormPrim = 1 };
-enum : int { kFluxHllc = 0, kFluxForce = 1 };
+enum : int { kFluxHllc = 0, kFluxForce = 1, kFluxLaxFriedrichs = 2, kFluxRichtmyer = 3 };
 enum : int { kEosIdeal = 0, kEosStiff = 1 };
 enum : int { kBcTransmissive = 0, kBcReflective = 1, kBcPeriodic = 2 };
 
@@ -272,11 +272,45 @@ __device__ __forceinline__ void force_flux(const T* ul, const T* ur, T dtdh, T d
   }
 }
 
+// LaxFriedrichs::Impl::operator() (flux_method/flux_lax_friedrichs.hpp:62-78).
+template <typename T, int ND, int D>
+__device__ __forceinline__ void lax_friedrichs_flux(const T* ul, const T* ur, T dhdt,
+                                                    const Material<T>& m, T* F) {
+  constexpr int S = 2 + ND;
+  const Prim<T, ND> wl = prim_of<T, ND>(ul, m);
+  const Prim<T, ND> wr = prim_of<T, ND>(ur, m);
+  T fl[S], fr[S];
+  euler_flux<T, ND, D>(ul, wl, fl);
+  euler_flux<T, ND, D>(ur, wr, fr);
+#pragma unroll
+  for (int k = 0; k < S; ++k) F[k] = (fl[k] + fr[k] + (ul[k] - ur[k]) * dhdt) * T(0.5);
+}
+
+// Richtmyer::Impl::operator() (flux_method/flux_richtmyer.hpp:78-88).
+template <typename T, int ND, int D>
+__device__ __forceinline__ void richtmyer_flux(const T* ul, const T* ur, T dtdh,
+                                               const Material<T>& m, T* F) {
+  constexpr int S = 2 + ND;
+  const Prim<T, ND> wl = prim_of<T, ND>(ul, m);
+  const Prim<T, ND> wr = prim_of<T, ND>(ur, m);
+  T fl[S], fr[S], uri[S];
+  euler_flux<T, ND, D>(ul, wl, fl);
+  euler_flux<T, ND, D>(ur, wr, fr);
+#pragma unroll
+  for (int k = 0; k < S; ++k) uri[k] = (ul[k] + ur[k] + (fl[k] - fr[k]) * dtdh) * T(0.5);
+  const Prim<T, ND> wri = prim_of<T, ND>(uri, m);
+  euler_flux<T, ND, D>(uri, wri, F);
+}
+
 template <typename T, int ND, int D, int FLUX>
 __device__ __forceinline__ void numerical_flux(const T* ul, const T* ur, T dtdh, T dhdt,
                                                const Material<T>& m, T* F) {
   if constexpr (FLUX == kFluxHllc) {
     hllc_flux<T, ND, D>(ul, ur, m, F);
+  } else if constexpr (FLUX == kFluxLaxFriedrichs) {
+    lax_friedrichs_flux<T, ND, D>(ul, ur, dhdt, m, F);
+  } else if constexpr (FLUX == kFluxRichtmyer) {
+    richtmyer_flux<T, ND, D>(ul, ur, dtdh, m, F);
   } else {
     force_flux<T, ND, D>(ul, ur, dtdh, dhdt, m, F);
   }

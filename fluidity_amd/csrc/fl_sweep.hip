@@ -166,6 +166,12 @@ __global__ void __launch_bounds__(kMaxBlock) component_sums_kernel(const T* __re
   template __global__ void sweep_x_kernel<T, 2, kFluxForce>(SweepArgs<T>);     \
   template __global__ void sweep_y_kernel<T, kFluxHllc>(SweepArgs<T>);         \
   template __global__ void sweep_y_kernel<T, kFluxForce>(SweepArgs<T>);        \
+  template __global__ void sweep_x_kernel<T, 1, kFluxLaxFriedrichs>(SweepArgs<T>); \
+  template __global__ void sweep_x_kernel<T, 2, kFluxLaxFriedrichs>(SweepArgs<T>); \
+  template __global__ void sweep_y_kernel<T, kFluxLaxFriedrichs>(SweepArgs<T>);    \
+  template __global__ void sweep_x_kernel<T, 1, kFluxRichtmyer>(SweepArgs<T>);     \
+  template __global__ void sweep_x_kernel<T, 2, kFluxRichtmyer>(SweepArgs<T>);     \
+  template __global__ void sweep_y_kernel<T, kFluxRichtmyer>(SweepArgs<T>);        \
   template __global__ void max_wavespeed_kernel<T, 1>(const T*, int64_t, int64_t, int64_t, \
                                                       Material<double>, unsigned long long*); \
   template __global__ void max_wavespeed_kernel<T, 2>(const T*, int64_t, int64_t, int64_t, \

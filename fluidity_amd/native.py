@@ -20,7 +20,7 @@ FL_F64, FL_F32 = 0, 1
 FL_LAYOUT_AOS, FL_LAYOUT_SOA = 0, 1
 FL_LIMITER_VAN_LEER, FL_LIMITER_SUPERBEE = 0, 1
 FL_FORM_CONSERVATIVE, FL_FORM_PRIMITIVE = 0, 1
-FL_FLUX_HLLC, FL_FLUX_FORCE = 0, 1
+FL_FLUX_HLLC, FL_FLUX_FORCE, FL_FLUX_LAX_FRIEDRICHS, FL_FLUX_RICHTMYER = 0, 1, 2, 3
 FL_EOS_IDEAL, FL_EOS_STIFF = 0, 1
 FL_BC_TRANSMISSIVE, FL_BC_REFLECTIVE, FL_BC_PERIODIC = 0, 1, 2
 

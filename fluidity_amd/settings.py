@@ -106,7 +106,7 @@ _RUNS = {
     "data_type": {"double", "float"},
     "dimensions": {"one", "two"},
     "execution": {"gpu"},
-    "flux_method": {"force", "hllc"},
+    "flux_method": {"force", "hllc", "lax-friedrichs", "richtmyer"},
     "limit_form": {"conservative", "primitive"},
     "limiter": {"van-leer", "superbee"},
     "material": {"ideal-gas"},
@@ -218,7 +218,9 @@ class OptionManager:
             else N.FL_LIMITER_VAN_LEER,
             form=N.FL_FORM_PRIMITIVE if self.values["limit_form"] == "primitive"
             else N.FL_FORM_CONSERVATIVE,
-            flux=N.FL_FLUX_FORCE if self.values["flux_method"] == "force" else N.FL_FLUX_HLLC)
+            flux={"force": N.FL_FLUX_FORCE, "hllc": N.FL_FLUX_HLLC,
+                  "lax-friedrichs": N.FL_FLUX_LAX_FRIEDRICHS,
+                  "richtmyer": N.FL_FLUX_RICHTMYER}[self.values["flux_method"]])
 
     def dimensions(self) -> int:
         return {"one": 1, "two": 2, "three": 3}[self.values["dimensions"]]

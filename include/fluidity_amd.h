@@ -38,7 +38,12 @@ typedef enum fl_layout { FL_LAYOUT_AOS = 0, FL_LAYOUT_SOA = 1 } fl_layout;
 
 typedef enum fl_limiter { FL_LIMITER_VAN_LEER = 0, FL_LIMITER_SUPERBEE = 1 } fl_limiter;
 typedef enum fl_limit_form { FL_FORM_CONSERVATIVE = 0, FL_FORM_PRIMITIVE = 1 } fl_limit_form;
-typedef enum fl_flux { FL_FLUX_HLLC = 0, FL_FLUX_FORCE = 1 } fl_flux;
+typedef enum fl_flux {
+  FL_FLUX_HLLC = 0,            /* flux_method/flux_hllc.hpp */
+  FL_FLUX_FORCE = 1,           /* flux_method/flux_force.hpp */
+  FL_FLUX_LAX_FRIEDRICHS = 2,  /* flux_method/flux_lax_friedrichs.hpp (single grids) */
+  FL_FLUX_RICHTMYER = 3        /* flux_method/flux_richtmyer.hpp (single grids) */
+} fl_flux;
 typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1 } fl_eos;
 typedef enum fl_boundary {
   FL_BC_TRANSMISSIVE = 0,  /* solver/boundary_loader.hpp:29-32 */

@@ -132,6 +132,8 @@ template <typename Limiter> struct MHReconstructor {
 namespace flux {
 struct Hllc { static constexpr int id = FL_FLUX_HLLC; };
 struct Force { static constexpr int id = FL_FLUX_FORCE; };
+struct LaxFriedrichs { static constexpr int id = FL_FLUX_LAX_FRIEDRICHS; };
+struct Richtmyer { static constexpr int id = FL_FLUX_RICHTMYER; };
 }  // namespace flux
 
 // ---- materials (material/ideal_gas.hpp, material/stiff_fluid.hpp) ----------

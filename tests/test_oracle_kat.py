@@ -99,3 +99,29 @@ def test_flux_delta_kat(oracle, flux):
     U = np.array([oracle.prim_to_cons(w, s) for w in W])
     got = oracle.flux_delta_at(U, s, kat["dtdh"], 0, kat["cell"])
     assert [float(x) for x in got] == kat["expect"][flux]
+
+
+def test_lax_friedrichs_and_richtmyer_hand_values(oracle):
+    """The standalone Lax-Friedrichs and Richtmyer fluxes
+    (flux_method/flux_lax_friedrichs.hpp:62-78, flux_richtmyer.hpp:78-88) on a
+    Sod-like pair at rest, gamma 1.4, dtdh 0.1, against values worked by hand:
+      U_L = (1, 2.5, 0), U_R = (0.125, 0.25, 0) -> p_L = 1, p_R = 0.1,
+      F_L = (0, 0, 1), F_R = (0, 0, 0.1);
+      LF = 1/2 (F_L + F_R + 10 (U_L - U_R)) = (4.375, 11.25, 0.55);
+      U* = 1/2 (U_L + U_R + 0.1 (F_L - F_R)) = (0.5625, 1.375, 0.045),
+      u* = 0.08, p* = 0.4 (1.375 - 0.5 * 0.5625 * 0.0064) = 0.54928,
+      RI = F(U*) = (0.045, 0.08 (1.375 + 0.54928), 0.045 * 0.08 + 0.54928).
+    FORCE is their mean bit for bit, and each is consistent (F(U, U) = F(U))."""
+    ul = np.array([1.0, 2.5, 0.0])
+    ur = np.array([0.125, 0.25, 0.0])
+    lf = oracle.numerical_flux(ul, ur, oracle.scheme(flux=oracle.FLUX_LF), 0.1, 0)
+    ri = oracle.numerical_flux(ul, ur, oracle.scheme(flux=oracle.FLUX_RICHTMYER), 0.1, 0)
+    fo = oracle.numerical_flux(ul, ur, oracle.scheme(flux=oracle.FLUX_FORCE), 0.1, 0)
+    np.testing.assert_allclose(lf, [4.375, 11.25, 0.55], rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(ri, [0.045, 0.08 * (1.375 + 0.54928), 0.0036 + 0.54928],
+                               rtol=1e-14, atol=1e-15)
+    assert np.array_equal(fo, (lf + ri) * 0.5)
+    u = np.array([0.7, 3.1, 0.4])
+    F = oracle.flux(u, oracle.scheme(), 0)
+    for fl in (oracle.FLUX_LF, oracle.FLUX_RICHTMYER):
+        assert np.array_equal(oracle.numerical_flux(u, u, oracle.scheme(flux=fl), 0.3, 0), F)

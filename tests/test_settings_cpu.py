@@ -77,8 +77,9 @@ def test_options_mapping_and_errors():
         m.set("flux_method", "roe")
     assert not m.try_set_option("no_such_option", "x")
     m.set("flux_method", "richtmyer")
-    with pytest.raises(UnsupportedOption, match="flux_method"):
-        m.scheme()
+    assert m.scheme().flux == N.FL_FLUX_RICHTMYER
+    m.set("flux_method", "lax-friedrichs")
+    assert m.scheme().flux == N.FL_FLUX_LAX_FRIEDRICHS
     m.set("flux_method", "force").set("solve_method", "unsplit")
     with pytest.raises(UnsupportedOption, match="solve_method"):
         m.scheme()
