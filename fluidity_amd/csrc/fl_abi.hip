@@ -411,6 +411,37 @@ int fl_split_solve(fl_context* c, fl_field* in, fl_field* out, const fl_scheme* 
   return FL_OK;
 }
 
+int fl_split_step(fl_context* c, fl_field* in, fl_field* out, const fl_scheme* s, double dtdh,
+                  double* max_wavespeed, void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  int rc = check_pair(in, out);
+  if (rc) return rc;
+  rc = check_scheme(s, in->ndim);
+  if (rc) return rc;
+  if (!(dtdh > 0.0) || !isfinite(dtdh)) return fail(FL_ERR_ARGUMENT, "dtdh %g must be > 0", dtdh);
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  if (max_wavespeed) hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+  for (int d = 0; d < in->ndim; ++d) {
+    // the last sweep writes the final states and reduces their wavespeeds
+    unsigned long long* ws = (max_wavespeed && d == in->ndim - 1) ? c->ws_tmp : nullptr;
+    rc = launch_sweep(c, in, out, d, s, dtdh, nullptr, ws, st);
+    if (rc) return rc;
+    if (in->ndim != 1 && d < in->ndim - 1) {  // solver/cuda/split_solver.cuh:75-78
+      void* tmp = in->data;
+      in->data = out->data;
+      out->data = tmp;
+    }
+  }
+  if (max_wavespeed) {
+    FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    *max_wavespeed = bits_to_double(c->host_ctl->ws_bits[0]);
+  }
+  return FL_OK;
+}
+
 int fl_max_wavespeed(fl_context* c, const fl_field* in, const fl_scheme* s, double* result,
                      void* stream) {
   if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");

@@ -114,6 +114,18 @@ class SplitSolver:
             return U_out, U_in
         return U_in, U_out
 
+    def step(self, U_in, U_out, dtdh: float, stream=None):
+        """All sweeps with the CFL reduction fused into the last one; returns
+        (in, out, max wavespeed of the new states) after the reference's swap."""
+        fi, fo = grid_field(U_in, self.layout), grid_field(U_out, self.layout)
+        ws = C.c_double()
+        N.check(N.lib.fl_split_step(self.ctx.handle, C.byref(fi), C.byref(fo),
+                                    C.byref(self.scheme.c()), dtdh, C.byref(ws),
+                                    _stream_handle(stream)))
+        if fo.data == U_in.data_ptr():
+            return U_out, U_in, ws.value
+        return U_in, U_out, ws.value
+
     def max_wavespeed(self, U, stream=None) -> float:
         f = grid_field(U, self.layout)
         out = C.c_double()

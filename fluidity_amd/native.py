@@ -28,7 +28,7 @@ EXPORTED = [
     "fl_status_string", "fl_last_error", "fl_version", "fl_ctx_create", "fl_ctx_destroy",
     "fl_ctx_set_tuning", "fl_ctx_get_tuning", "fl_ctx_autotune", "fl_ctx_set_graphs",
     "fl_split_sweep",
-    "fl_split_solve",
+    "fl_split_solve", "fl_split_step",
     "fl_max_wavespeed", "fl_simulate", "fl_conservation_sums",
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
     "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate_work_bytes",
@@ -156,6 +156,7 @@ def _load():
     L.fl_ctx_set_graphs.argtypes = [vp, C.c_int]
     L.fl_split_sweep.argtypes = [vp, fp, fp, C.c_int, sp, C.c_double, vp]
     L.fl_split_solve.argtypes = [vp, fp, fp, sp, C.c_double, vp]
+    L.fl_split_step.argtypes = [vp, fp, fp, sp, C.c_double, C.POINTER(C.c_double), vp]
     L.fl_max_wavespeed.argtypes = [vp, fp, sp, C.POINTER(C.c_double), vp]
     L.fl_simulate.argtypes = [vp, fp, fp, sp, C.POINTER(SimParams), C.POINTER(SimResult), vp]
     L.fl_conservation_sums.argtypes = [vp, fp, C.POINTER(C.c_double), vp]

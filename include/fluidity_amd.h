@@ -133,6 +133,15 @@ int fl_split_sweep(fl_context* ctx, const fl_field* in, const fl_field* out, int
 int fl_split_solve(fl_context* ctx, fl_field* in, fl_field* out, const fl_scheme* scheme,
                    double dtdh, void* stream);
 
+/* fl_split_solve with the CFL reduction fused into the last sweep: the
+ * proposed fused split step of the update call (solver/split_solver.hpp:
+ * 105-115 followed by set_wavespeeds + max_element on its result,
+ * simulator/simulator_impl.hpp:241-246).  max_wavespeed (host, nullable)
+ * receives the FP64 maximum wavespeed of the new states, which synchronises
+ * `stream`; NULL skips the reduction and the synchronisation. */
+int fl_split_step(fl_context* ctx, fl_field* in, fl_field* out, const fl_scheme* scheme,
+                  double dtdh, double* max_wavespeed, void* stream);
+
 /* max over cells of max_d |u_d| + a, computed in FP64 and returned on the
  * host (synchronises `stream`).  Replaces set_wavespeeds + max_element
  * (simulator/cuda/wavespeed_initialization.cuh:41-97,

@@ -340,3 +340,25 @@ def test_simulate_lf_richtmyer_bitwise(E, oracle, flux):
     got = _host(a, 1)
     assert r.iters == it and r.run_time == rt
     assert _bitwise(got, ref), _report(got, ref)
+
+
+@pytest.mark.parametrize("shape", [(1000,), (45, 70)])
+def test_split_step_fused_max(E, oracle, shape):
+    """fl_split_step equals fl_split_solve and its fused maximum equals the
+    oracle's maximum over the new states."""
+    U = cons_field("rough", shape, seed=17)
+    nd = len(shape)
+    os_ = _oscheme(oracle, 0, 0, 0, (0,) * nd)
+    dtdh = 0.4 / oracle.max_wavespeed_grid(U, os_)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0,) * nd))
+    a, b = _dev(U), torch.empty(U.shape, dtype=torch.float64, device="cuda")
+    _, out, ws = solver.step(a, b, dtdh)
+    got = _host(out)
+    c, d = _dev(U), torch.empty_like(b)
+    _, out2 = solver.solve(c, d, dtdh)
+    assert _bitwise(got, _host(out2))
+    ref = U
+    for dim in range(nd):
+        ref = oracle.sweep(ref, os_, dtdh, dim)
+    assert _bitwise(got, ref), _report(got, ref)
+    assert ws == oracle.max_wavespeed_grid(ref, os_)
