@@ -49,6 +49,18 @@ constexpr int kSegments1D = 8;
 
 bool valid_width(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0; }
 
+// Height of the x or y tiles of a 2D problem with ny rows: the tuning's base
+// height, doubled (up to 8x) while the problem still gives at least
+// kMinRowTiles tiles per column band.  Tall problems pay the warm-up rows of
+// a tile (6 in x, 8 in y) less often; short ones keep short tiles, which end
+// the phase evenly.
+constexpr int kMinRowTiles = 8;
+int64_t tile_rows(int64_t ny, int base) {
+  int64_t r = base;
+  for (int k = 0; k < 3 && ny >= 2 * r * kMinRowTiles; ++k) r *= 2;
+  return r;
+}
+
 // Tile lists for a tuning: phase 0 holds the x tiles of the 2D problems
 // (largest problems first) and then the 1D problems (longest first); phase 1
 // the y tiles of the 2D problems.
@@ -75,10 +87,10 @@ void build_tiles(const fl_batch* b, const fl_tuning& t, std::vector<bt::Tile>& t
         t0.push_back({p, (int)bx, 0, 0});
     } else {
       const int64_t ny = f.n[1];
-      const int64_t gx = ceil_div(nx, t.x_block - 2), gy = ceil_div(ny, t.x_rows);
+      const int64_t gx = ceil_div(nx, t.x_block - 2), gy = ceil_div(ny, tile_rows(ny, t.x_rows));
       for (int64_t by = 0; by < gy; ++by)
         for (int64_t bx = 0; bx < gx; ++bx) t0.push_back({p, (int)bx, (int)by, 0});
-      const int64_t hx = ceil_div(nx, t.y_block), hy = ceil_div(ny, t.y_rows);
+      const int64_t hx = ceil_div(nx, t.y_block), hy = ceil_div(ny, tile_rows(ny, t.y_rows));
       for (int64_t by = 0; by < hy; ++by)
         for (int64_t bx = 0; bx < hx; ++bx) t1.push_back({p, (int)bx, (int)by, 0});
     }
@@ -104,12 +116,12 @@ std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
     a.form = q.scheme.form;
     a.bcx = q.scheme.bc[0];
     a.bcy = q.state.ndim > 1 ? q.scheme.bc[1] : 0;
-    a.rows_per_block = q.state.ndim > 1 ? t.x_rows : kSegments1D;
+    a.rows_per_block = q.state.ndim > 1 ? (int)tile_rows(a.ny, t.x_rows) : kSegments1D;
     a.ctl = b->ctls + p;
     a.ws_out = nullptr;
     P.ctl = b->ctls + p;
     P.nd = q.state.ndim;
-    P.rows_y = t.y_rows;
+    P.rows_y = (int)tile_rows(q.state.ndim > 1 ? q.state.n[1] : 1, t.y_rows);
   }
   return v;
 }
