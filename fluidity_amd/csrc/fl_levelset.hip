@@ -126,6 +126,40 @@ __global__ void velocity_kernel(const double* __restrict__ phi0, const double* _
   V[i] = v;
 }
 
+// Both level sets' speeds in one pass (the two-material step's
+// velocity_kernel with target 0 and target 1): V_m = u.n_m of the material
+// present, with the same expressions per target.
+template <int ND>
+__global__ void velocity2_kernel(const double* __restrict__ phi0, const double* __restrict__ phi1,
+                                 StateView u0, StateView u1, Grid g, double dh,
+                                 double* __restrict__ V0, double* __restrict__ V1) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  double v0 = 0.0, v1 = 0.0;
+  for (int m = 0; m < 2; ++m) {
+    const double p = (m == 0 ? phi0 : phi1)[i];
+    if ((p <= 0.0) || (p <= -0.0)) {
+      double n0[2], n1[2];
+      norm_at(phi0, g, x, y, dh, n0);
+      norm_at(phi1, g, x, y, dh, n1);
+      double u[2 + ND];
+      load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        s0 += (u[2 + d] / u[0]) * n0[d];
+        s1 += (u[2 + d] / u[0]) * n1[d];
+      }
+      v0 = s0;
+      v1 = s1;
+      break;
+    }
+  }
+  V0[i] = v0;
+  V1[i] = v1;
+}
+
 __global__ void project2_kernel(double* __restrict__ phi0, double* __restrict__ phi1,
                                 int64_t count) {
   const int64_t i = gid();
@@ -699,6 +733,8 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
 #define FL_LS_INST(ND)                                                                       \
   template __global__ void velocity_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, int, Grid, double, int, double*); \
+  template __global__ void velocity2_kernel<ND>(const double*, const double*, StateView,     \
+                                                StateView, Grid, double, double*, double*); \
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
                                                uint8_t*, int*);                             \

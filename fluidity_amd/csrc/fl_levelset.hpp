@@ -40,6 +40,9 @@ __global__ void masks_kernel(const double* phi, Grid g, double dh, uint8_t* mask
 template <int ND>
 __global__ void velocity_kernel(const double* phi0, const double* phi1, StateView u0,
                                 StateView u1, int nmat, Grid g, double dh, int target, double* V);
+template <int ND>
+__global__ void velocity2_kernel(const double* phi0, const double* phi1, StateView u0,
+                                 StateView u1, Grid g, double dh, double* V0, double* V1);
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
                                  const double* V, Grid g, double dt, double h1, double* out);

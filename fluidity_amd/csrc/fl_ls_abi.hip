@@ -90,6 +90,22 @@ int launch_velocity(int nmat, const double* const* phis, const fl_field* states,
   return FL_OK;
 }
 
+// Both level sets' speeds from the level sets phis and the two states.
+int launch_velocity2(double* const* phis, const fl_field& s0, const fl_field& s1,
+                     const ls::Grid& g, double dh, double* const* V, hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::velocity2_kernel<1>, grid, dim3(kBlock), 0, st,
+                       (const double*)phis[0], (const double*)phis[1], view_of(&s0),
+                       view_of(&s1), g, dh, V[0], V[1]);
+  else
+    hipLaunchKernelGGL(ls::velocity2_kernel<2>, grid, dim3(kBlock), 0, st,
+                       (const double*)phis[0], (const double*)phis[1], view_of(&s0),
+                       view_of(&s1), g, dh, V[0], V[1]);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
 // RK3: phi -> s1 -> s2 -> out, s1/s2 in work.
 int launch_advect(const double* phi, const double* V, const ls::Grid& g, double dt, double dh,
                   double* out, double* work, hipStream_t st) {
@@ -487,17 +503,28 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
   const double dh = p->dh;
   double t = 0.0, dt = 0.0, maxws = 0.0;
   int64_t it = 0;
+  // Each level set lives in the caller's phi or in the advection buffer tphi:
+  // cur[m] holds it, nxt[m] receives the next advection.  A step without
+  // reinitialisation swaps the two; one with it writes back into cur[m].
+  double* cur[2] = {mats[0].phi, mats[1].phi};
+  double* nxt[2] = {w.tphi[0], w.tphi[1]};
+  auto settle = [&]() -> int {  // the level sets back in the caller's buffers
+    for (int m = 0; m < 2; ++m)
+      if (cur[m] != mats[m].phi)
+        FL_HIP(hipMemcpyAsync(mats[m].phi, cur[m], (size_t)(cells * 8),
+                              hipMemcpyDeviceToDevice, st));
+    return FL_OK;
+  };
   while (t < p->sim_time && it < p->max_iters) {
     // ghost cells: star states of both materials from the same data, then
     // extrapolation into each material's ghost band
     ls::ExtrapJob jobs[2];
     for (int m = 0; m < 2; ++m)
-      jobs[m] = extrap_job(mats[m].phi, view_of(&mats[m].state), g, w.extrap[m]);
+      jobs[m] = extrap_job(cur[m], view_of(&mats[m].state), g, w.extrap[m]);
     for (int m = 0; m < 2; ++m) {
       const int o = 1 - m;
-      if ((rc = launch_star(mats[m].phi, mats[o].phi, &mats[m].state, &mats[o].state,
-                            &mats[m].scheme, &mats[o].scheme, g, dh, w.star[m], w.has[m], st,
-                            jobs[m].count)))
+      if ((rc = launch_star(cur[m], cur[o], &mats[m].state, &mats[o].state, &mats[m].scheme,
+                            &mats[o].scheme, g, dh, w.star[m], w.has[m], st, jobs[m].count)))
         return rc;
     }
     for (int m = 0; m < 2; ++m) band_seed(jobs[m], w.star[m], w.has[m], g, dh, st);
@@ -518,6 +545,8 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
         result->last_dt = dt;
         result->max_wavespeed = maxws;
       }
+      if ((rc = settle())) return rc;
+      FL_HIP(hipStreamSynchronize(st));
       return fail(FL_ERR_NUMERIC, "max wavespeed %g at step %lld (NaN or vacuum)", maxws,
                   (long long)it);
     }
@@ -536,29 +565,27 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
                               st));
       }
     }
-    // level sets: speeds from the old level sets and new states, advection,
-    // projection, reinitialisation back into each material's phi
-    const double* phis[2] = {mats[0].phi, mats[1].phi};
-    const fl_field states[2] = {mats[0].state, mats[1].state};
+    // level sets: speeds from the old level sets and new states (both in one
+    // pass), advection, projection, reinitialisation every `every` steps
+    if ((rc = launch_velocity2(cur, mats[0].state, mats[1].state, g, dh, w.V, st))) return rc;
     for (int m = 0; m < 2; ++m)
-      if ((rc = launch_velocity(2, phis, states, g, dh, m, w.V[m], st))) return rc;
-    for (int m = 0; m < 2; ++m)
-      if ((rc = launch_advect(mats[m].phi, w.V[m], g, dt, dh, w.tphi[m], w.adv, st))) return rc;
-    if ((rc = launch_project(w.tphi[0], w.tphi[1], cells, st))) return rc;
+      if ((rc = launch_advect(cur[m], w.V[m], g, dt, dh, nxt[m], w.adv, st))) return rc;
+    if ((rc = launch_project(nxt[0], nxt[1], cells, st))) return rc;
     const bool reinit = (it % every) == every - 1;
     for (int m = 0; m < 2; ++m) {
       if (reinit) {
-        if ((rc = run_reinit(c, w.tphi[m], g, dh, band, mats[m].phi, nullptr, w.reinit, nullptr,
-                             st)))
+        if ((rc = run_reinit(c, nxt[m], g, dh, band, cur[m], nullptr, w.reinit, nullptr, st)))
           return rc;
       } else {
-        FL_HIP(hipMemcpyAsync(mats[m].phi, w.tphi[m], (size_t)(cells * 8),
-                              hipMemcpyDeviceToDevice, st));
+        double* tmp = cur[m];
+        cur[m] = nxt[m];
+        nxt[m] = tmp;
       }
     }
     t += dt;
     it += 1;
   }
+  if ((rc = settle())) return rc;
   FL_HIP(hipStreamSynchronize(st));
   if (result) {
     result->run_time = t;
