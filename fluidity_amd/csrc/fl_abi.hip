@@ -10,7 +10,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "fl_abi_internal.hpp"
 
@@ -125,11 +127,12 @@ fl::SweepArgs<T> make_args(const fl_context* ctx, const fl_field* in, const fl_f
 }
 
 // Rows per workgroup, halved (down to 4) while the launch would have fewer
-// than kMinGroups workgroups: small grids keep the machine filled whatever
-// the tuning says.
-constexpr int64_t kMinGroups = 1024;
-int fit_rows(int rows, int64_t bands, int64_t ny) {
-  while (rows > 4 && bands * ((ny + rows - 1) / rows) < kMinGroups) rows /= 2;
+// than kMinWaves waves (1.5 per SIMD of the 1024 on the device): small grids
+// keep the machine filled whatever the tuning says.
+constexpr int64_t kMinWaves = 1536;
+int fit_rows(int rows, int64_t bands, int64_t ny, int block) {
+  const int64_t waves = block / 64;
+  while (rows > 4 && bands * ((ny + rows - 1) / rows) * waves < kMinWaves) rows /= 2;
   return rows;
 }
 
@@ -145,6 +148,44 @@ void launch_flux(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
     hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX>), grid, dim3(B), 0, stream, a);
 }
 
+// Workgroups of B lanes of one sweep kernel that a CU holds at once.
+template <typename T, int FLUX>
+int sweep_blocks_per_cu(int dim, int B) {
+  const void* k = dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX>)
+                           : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX>);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, 0) != hipSuccess) return 0;
+  return n;
+}
+
+template <typename T>
+int sweep_blocks_per_cu_t(int dim, int flux, int B) {
+  switch (flux) {
+    case FL_FLUX_HLLC: return sweep_blocks_per_cu<T, fl::kFluxHllc>(dim, B);
+    case FL_FLUX_FORCE: return sweep_blocks_per_cu<T, fl::kFluxForce>(dim, B);
+    case FL_FLUX_LAX_FRIEDRICHS: return sweep_blocks_per_cu<T, fl::kFluxLaxFriedrichs>(dim, B);
+    default: return sweep_blocks_per_cu<T, fl::kFluxRichtmyer>(dim, B);
+  }
+}
+
+// Tile heights to time for one 2D sweep: the powers of two in `base`, and,
+// for R = 1..16, the smallest height whose launch fits in R rounds of the
+// workgroups the device holds at once (`slots`).  On a grid whose launch is
+// a few rounds, one a little over a whole number of rounds leaves most of
+// the machine idle in its last round.
+std::vector<int> row_candidates(const int* base, int nbase, int64_t bands, int64_t ny,
+                                int64_t slots) {
+  std::vector<int> rows(base, base + nbase);
+  for (int64_t R = 1; slots > 0 && R <= 16; ++R) {
+    const int64_t chunks = R * slots / bands;
+    if (chunks < 1) continue;
+    const int64_t h = (ny + chunks - 1) / chunks;
+    if (h >= 4 && h <= 128 && std::find(rows.begin(), rows.end(), (int)h) == rows.end())
+      rows.push_back((int)h);
+  }
+  return rows;
+}
+
 template <typename T>
 int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
                    const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
@@ -152,8 +193,8 @@ int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* ou
   fl_tuning tu = ctx->tuning;
   if (in->ndim == 2) {
     const int64_t nx = in->n[0], ny = in->n[1];
-    tu.x_rows = fit_rows(tu.x_rows, (nx + tu.x_block - 3) / (tu.x_block - 2), ny);
-    tu.y_rows = fit_rows(tu.y_rows, (nx + tu.y_block - 1) / tu.y_block, ny);
+    tu.x_rows = fit_rows(tu.x_rows, (nx + tu.x_block - 3) / (tu.x_block - 2), ny, tu.x_block);
+    tu.y_rows = fit_rows(tu.y_rows, (nx + tu.y_block - 1) / tu.y_block, ny, tu.y_block);
   }
   dim3 grid;
   int B;
@@ -332,6 +373,8 @@ int fl_ctx_autotune(fl_context* c, const fl_field* in, const fl_field* work,
   static const int widths[] = {64, 128, 256};
   static const int xrows[] = {8, 16, 32, 64};
   static const int yrows[] = {16, 32, 64, 128};
+  int cus = 0;
+  FL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   hipEvent_t e0, e1;
   FL_HIP(hipEventCreate(&e0));
   FL_HIP(hipEventCreate(&e1));
@@ -340,13 +383,19 @@ int fl_ctx_autotune(fl_context* c, const fl_field* in, const fl_field* work,
   for (int dim = 0; dim < in->ndim && !rc; ++dim) {
     float best_ms = 1e30f;
     for (int w : widths) {
-      const int* rows = dim ? yrows : xrows;
-      const int nr = in->ndim == 1 ? 1 : 4;
-      for (int k = 0; k < nr && !rc; ++k) {
+      std::vector<int> rows(1, 0);  // 1D: the height is unused
+      if (in->ndim == 2) {
+        const int64_t nx = in->n[0], ny = in->n[1];
+        const int64_t bands = dim ? (nx + w - 1) / w : (nx + w - 3) / (w - 2);
+        const int per_cu = in->dtype == FL_F64 ? sweep_blocks_per_cu_t<double>(dim, s->flux, w)
+                                               : sweep_blocks_per_cu_t<float>(dim, s->flux, w);
+        rows = row_candidates(dim ? yrows : xrows, 4, bands, ny, (int64_t)per_cu * cus);
+      }
+      for (size_t k = 0; k < rows.size() && !rc; ++k) {
         fl_tuning t = best;
         if (dim == 0) {
           t.x_block = w;
-          t.x_rows = rows[k];
+          if (rows[k]) t.x_rows = rows[k];
         } else {
           t.y_block = w;
           t.y_rows = rows[k];
