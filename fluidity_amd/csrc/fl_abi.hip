@@ -313,8 +313,8 @@ int fl_ctx_create(int device, fl_context** out) {
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_partial, sizeof(double) * kSumBlocks * 4, 0);
   if (e == hipSuccess) e = hipStreamCreate(&c->cap_stream);
   c->use_graphs = 1;
-  if (e == hipSuccess) e = hipMalloc((void**)&c->eik, sizeof(fl::ls::EikCtl));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, sizeof(fl::ls::EikCtl), 0);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->eik, 2 * sizeof(fl::ls::EikCtl));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, 2 * sizeof(fl::ls::EikCtl), 0);
   if (e != hipSuccess) {
     fl_ctx_destroy(c);
     return fail(FL_ERR_HIP, "context allocation failed: %s", hipGetErrorString(e));

@@ -30,7 +30,7 @@ struct fl_context {
   double* partial;             // device partial sums, kSumBlocks * 4
   fl::StepCtl* host_ctl;       // pinned host mirror of ctl
   double* host_partial;        // pinned host mirror of partial
-  fl::ls::EikCtl* eik;         // device control words of the eikonal loop
+  fl::ls::EikCtl* eik;         // device control words of the eikonal loop, 2 jobs
   fl::ls::EikCtl* host_eik;    // pinned host mirror of eik
   fl_tuning tuning;
   int use_graphs;              // fl_simulate replays a captured pair of steps

@@ -274,14 +274,23 @@ __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
 
 __device__ __forceinline__ int signum(double x) { return (0.0 < x) - (x < 0.0); }
 
+// The job of this workgroup (selected by value so that the kernel argument
+// block is never indexed dynamically).
+__device__ __forceinline__ EikJob eik_job(const EikJobs& js) {
+  return blockIdx.z ? js.j[1] : js.j[0];
+}
+
 // Sources (fast_iterative_init, scheme/cuda/fast_iterative_method.cuh:48-85):
 // a cell beside a sign change with the smaller |phi|; distance |phi| there.
 // info = source | (signum(phi) + 1) << 1.  Both Jacobi buffers start equal.
-__global__ void eik_init_kernel(const double* __restrict__ phi, Grid g,
-                                double* __restrict__ d, double* __restrict__ d1,
-                                uint8_t* __restrict__ info) {
+__global__ void eik_init_kernel(EikJobs jobs, Grid g) {
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
+  const EikJob J = eik_job(jobs);
+  const double* __restrict__ phi = J.phi;
+  double* __restrict__ d = J.d[0];
+  double* __restrict__ d1 = J.d[1];
+  uint8_t* __restrict__ info = J.info;
   const int64_t x = i % g.nx, y = i / g.nx;
   const double c = phi[i];
   int source = 0;
@@ -310,7 +319,10 @@ __device__ __forceinline__ void eik_cell_of(int64_t tr, int64_t& x, int64_t& y) 
 }
 
 __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_tile_src_kernel(
-    const uint8_t* __restrict__ info, Grid g, int64_t trows, uint8_t* __restrict__ src) {
+    EikJobs jobs, Grid g, int64_t trows) {
+  const EikJob J = eik_job(jobs);
+  const uint8_t* __restrict__ info = J.info;
+  uint8_t* __restrict__ src = J.src;
   const int64_t segs = gridDim.x;
   for (int64_t tr = blockIdx.y; tr < trows; tr += gridDim.y) {
     int64_t x, y;
@@ -321,15 +333,19 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_tile_src_kernel(
   }
 }
 
-// act[t] = some tile within `tr_r` tile rows and `segs_r` tiles of t holds a
-// source.  A cell's distance grows by at least dh/sqrt(2) per axis step from
-// the nearest source (the Godunov update q >= min(a, b) + dh/sqrt(2)), so a
-// cell farther than band * sqrt(2) steps from every source never enters the
-// band and tiles without such cells can be skipped by every sweep.
-__global__ void eik_tile_act_kernel(const uint8_t* __restrict__ src, int64_t trows, int64_t segs,
-                                    int tr_r, int segs_r, uint8_t* __restrict__ act) {
+// Tile t is active when some tile within `tr_r` tile rows and `segs_r`
+// tiles of it holds a source; the active tiles are appended to the job's
+// list (one atomicAdd per wave; ctl->ntiles starts at zero).  A cell's
+// distance grows by at least dh/sqrt(2) per axis step from the nearest
+// source (the Godunov update q >= min(a, b) + dh/sqrt(2)), so a cell farther
+// than band * sqrt(2) steps from every source never enters the band and
+// tiles without such cells can be skipped by every sweep.
+__global__ void eik_tile_act_kernel(EikJobs jobs, int64_t trows, int64_t segs, int tr_r,
+                                    int segs_r) {
   const int64_t t = gid();
   if (t >= trows * segs) return;
+  const EikJob J = eik_job(jobs);
+  const uint8_t* __restrict__ src = J.src;
   const int64_t y = t / segs, sx = t % segs;
   const int64_t y0 = y - tr_r < 0 ? 0 : y - tr_r, y1 = y + tr_r >= trows ? trows - 1 : y + tr_r;
   const int64_t s0 = sx - segs_r < 0 ? 0 : sx - segs_r;
@@ -337,7 +353,14 @@ __global__ void eik_tile_act_kernel(const uint8_t* __restrict__ src, int64_t tro
   uint8_t a = 0;
   for (int64_t yy = y0; yy <= y1 && !a; ++yy)
     for (int64_t ss = s0; ss <= s1; ++ss) a |= src[yy * segs + ss];
-  act[t] = a;
+  const unsigned long long m = __ballot(a != 0);
+  if (!m) return;
+  const int lane = (int)(threadIdx.x % 64);
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&J.ctl->ntiles, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (a) J.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)t;
 }
 
 // Neighbour distance seen from a cell with sign bits sc: negated across the
@@ -349,10 +372,15 @@ __device__ __forceinline__ double eik_nbr(const double* __restrict__ d,
   return ((info[j] & 6) == sc || v == DBL_MAX) ? v : -v;
 }
 
-__global__ void eik_ctl_init_kernel(EikCtl* c) {
+// One thread per job.
+__global__ void eik_ctl_init_kernel(EikJobs jobs, int njobs) {
+  if ((int)threadIdx.x >= njobs) return;
+  EikCtl* c = threadIdx.x ? jobs.j[1].ctl : jobs.j[0].ctl;
   c->flags[0] = c->flags[1] = c->flags[2] = 0;
   c->done = 0;
   c->iters = 0;
+  c->ntiles = 0;
+  c->pad = 0;
 }
 
 // Sweep k of the Godunov eikonal update
@@ -362,12 +390,20 @@ __global__ void eik_ctl_init_kernel(EikCtl* c) {
 // batch of sweeps can be launched ahead of the convergence test.  Every
 // active sweep writes every cell of the active tiles (the others hold their
 // initial value in both buffers), so once converged both buffers agree.
-// Launched like eik_tile_src_kernel.
+// Sweep k reads the job's buffer d[k & 1] and writes d[(k + 1) & 1].  Each
+// job has its own control words, so one job's sweeps return at once when it
+// has converged while the other's go on.  Launched on (kEikSweepGroups, 1,
+// jobs) workgroups of kEikTileW * kEikTileH threads that stride over the
+// job's active tiles, one tile per workgroup at a time.
 __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
-    const double* __restrict__ d, const uint8_t* __restrict__ info, Grid g, int64_t trows,
-    double dh, double limit, double* __restrict__ e, EikCtl* __restrict__ ctl, int k,
-    const uint8_t* __restrict__ act) {
-  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    EikJobs jobs, Grid g, int64_t segs, double dh, double limit, int k) {
+  const EikJob J = eik_job(jobs);
+  const double* __restrict__ d = (k & 1) ? J.d[1] : J.d[0];
+  double* __restrict__ e = (k & 1) ? J.d[0] : J.d[1];
+  const uint8_t* __restrict__ info = J.info;
+  const int* __restrict__ list = J.list;
+  EikCtl* __restrict__ ctl = J.ctl;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (ctl->done) return;
   if (k > 0 && ctl->flags[(k - 1) % 3] == 0) {
     if (first) ctl->done = 1;
@@ -377,12 +413,13 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
     ctl->flags[(k + 1) % 3] = 0;
     ctl->iters = k + 1;
   }
-  const int64_t segs = gridDim.x;
+  const int ntiles = ctl->ntiles;
   bool changed = false;
-  for (int64_t tr = blockIdx.y; tr < trows; tr += gridDim.y) {
-    int64_t x, y;
-    eik_cell_of(tr, x, y);
-    if (!act[tr * segs + blockIdx.x] || x >= g.nx || y >= g.ny) continue;
+  for (int li = blockIdx.x; li < ntiles; li += gridDim.x) {
+    const int64_t t = list[li];
+    const int64_t x = (t % segs) * kEikTileW + (threadIdx.x % kEikTileW);
+    const int64_t y = (t / segs) * kEikTileH + (threadIdx.x / kEikTileW);
+    if (x >= g.nx || y >= g.ny) continue;
     const int64_t i = y * g.nx + x;
     const double cur = d[i];
     double nv = cur;
@@ -416,11 +453,16 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
   if (m && (int)(threadIdx.x % 64) == __ffsll((long long)m) - 1) ctl->flags[k % 3] = 1;
 }
 
-__global__ void eik_finish_kernel(const double* __restrict__ phi, const double* __restrict__ d,
-                                  Grid g, double limit, double clamp, double* __restrict__ out,
-                                  uint8_t* __restrict__ band_mask) {
+// d = the converged buffer d[iters & 1] of the job; both hold the same
+// values once the loop has converged.
+__global__ void eik_finish_kernel(EikJobs jobs, Grid g, double limit, double clamp) {
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
+  const EikJob J = eik_job(jobs);
+  const double* __restrict__ phi = J.phi;
+  const double* __restrict__ d = (J.ctl->iters & 1) ? J.d[1] : J.d[0];
+  double* __restrict__ out = J.out;
+  uint8_t* __restrict__ band_mask = J.band_mask;
   const int in_band = d[i] <= limit;
   const double dist = in_band ? d[i] : clamp;
   out[i] = (double)signum(phi[i]) * dist;

@@ -28,6 +28,25 @@ struct EikCtl {
   int flags[3];
   int done;
   long long iters;
+  int ntiles;  // entries of the job's active-tile list
+  int pad;
+};
+
+// One eikonal reinitialisation: the level set in, its two Jacobi distance
+// buffers, per-cell info bytes, the tile flags, its loop control and the
+// result.  Launches carry up to two jobs; blockIdx.z selects one.
+struct EikJob {
+  const double* phi;
+  double* d[2];
+  uint8_t* info;
+  uint8_t* src;   // per tile: holds a source
+  int* list;      // the tiles a sweep visits, in no particular order
+  EikCtl* ctl;
+  double* out;
+  uint8_t* band_mask;  // nullable
+};
+struct EikJobs {
+  EikJob j[2];
 };
 
 struct Eos {
@@ -46,19 +65,19 @@ __global__ void velocity2_kernel(const double* phi0, const double* phi1, StateVi
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
                                  const double* V, Grid g, double dt, double h1, double* out);
-__global__ void eik_init_kernel(const double* phi, Grid g, double* d, double* d1, uint8_t* info);
+__global__ void eik_init_kernel(EikJobs jobs, Grid g);
 // Eikonal tiles: kEikTileW cells x kEikTileH rows (one workgroup each).
 constexpr int kEikTileW = 64;
 constexpr int kEikTileH = 4;
-__global__ void eik_tile_src_kernel(const uint8_t* info, Grid g, int64_t trows, uint8_t* src);
-__global__ void eik_tile_act_kernel(const uint8_t* src, int64_t trows, int64_t segs, int tr_r,
-                                    int segs_r, uint8_t* act);
-__global__ void eik_ctl_init_kernel(EikCtl* c);
-__global__ void eik_sweep_kernel(const double* d, const uint8_t* info, Grid g, int64_t trows,
-                                 double dh, double limit, double* e, EikCtl* ctl, int k,
-                                 const uint8_t* act);
-__global__ void eik_finish_kernel(const double* phi, const double* d, Grid g, double limit,
-                                  double clamp, double* out, uint8_t* band_mask);
+__global__ void eik_tile_src_kernel(EikJobs jobs, Grid g, int64_t trows);
+__global__ void eik_tile_act_kernel(EikJobs jobs, int64_t trows, int64_t segs, int tr_r,
+                                    int segs_r);
+__global__ void eik_ctl_init_kernel(EikJobs jobs, int njobs);
+// Workgroups per job of an eikonal sweep; they stride over the active tiles.
+constexpr int kEikSweepGroups = 512;
+__global__ void eik_sweep_kernel(EikJobs jobs, Grid g, int64_t segs, double dh, double limit,
+                                 int k);
+__global__ void eik_finish_kernel(EikJobs jobs, Grid g, double limit, double clamp);
 template <int ND>
 __global__ void gfm_star_kernel(const double* phi_m, const double* phi_o, StateView um,
                                 StateView uo, Eos em, Eos eo, Grid g, double dh, double* star,

@@ -135,65 +135,88 @@ int launch_project(double* phi0, double* phi1, int64_t cells, hipStream_t st) {
 int64_t eik_segs(const ls::Grid& g) { return (g.nx + ls::kEikTileW - 1) / ls::kEikTileW; }
 int64_t eik_trows(const ls::Grid& g) { return (g.ny + ls::kEikTileH - 1) / ls::kEikTileH; }
 
-// Two distance buffers, the per-cell info bytes and two tile-flag arrays.
+// Two distance buffers, the per-cell info bytes, the per-tile source flags
+// and the active-tile list.
 int64_t reinit_work_bytes(const ls::Grid& g) {
   const int64_t cells = g.nx * g.ny;
   const int64_t tiles = eik_segs(g) * eik_trows(g);
-  return 2 * align256(cells * 8) + align256(cells) + 2 * align256(tiles);
+  return 2 * align256(cells * 8) + align256(cells) + align256(tiles) + align256(tiles * 4);
 }
 
-int run_reinit(fl_context* c, const double* phi, const ls::Grid& g, double dh, int band,
-               double* out, uint8_t* band_mask, void* work, int64_t* iterations,
-               hipStream_t st) {
+// One reinitialisation job: phi in, out (and the nullable band mask) out,
+// `work` of reinit_work_bytes, control words ctl.
+ls::EikJob eik_job(const double* phi, const ls::Grid& g, double* out, uint8_t* band_mask,
+                   void* work, ls::EikCtl* ctl) {
   const int64_t cells = g.nx * g.ny;
+  const int64_t tiles = eik_segs(g) * eik_trows(g);
   char* w = (char*)work;
-  double* buf[2] = {(double*)w, (double*)(w + align256(cells * 8))};
-  uint8_t* info = (uint8_t*)(w + 2 * align256(cells * 8));
+  ls::EikJob j;
+  j.phi = phi;
+  j.d[0] = (double*)w;
+  j.d[1] = (double*)(w + align256(cells * 8));
+  j.info = (uint8_t*)(w + 2 * align256(cells * 8));
+  j.src = j.info + align256(cells);
+  j.list = (int*)(j.src + align256(tiles));
+  j.ctl = ctl;
+  j.out = out;
+  j.band_mask = band_mask;
+  return j;
+}
+
+// Eikonal reinitialisation of njobs (1 or 2) level sets on one grid, every
+// launch covering all jobs; iterations (nullable) receives each job's sweep
+// count.  The host tests convergence after every kEikBatch sweeps.
+int run_reinit(fl_context* c, const ls::EikJobs& jobs, int njobs, const ls::Grid& g, double dh,
+               int band, int64_t* iterations, hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
   const int64_t segs = eik_segs(g), trows = eik_trows(g), tiles = segs * trows;
-  uint8_t* src = info + align256(cells);
-  uint8_t* act = src + align256(tiles);
   const double limit = band * dh;
   const double clamp = (band + 1) * dh;
-  const dim3 grid = grid_for(cells);
-  const dim3 tgrid((unsigned)segs, (unsigned)(trows < 65535 ? trows : 65535));
+  dim3 grid = grid_for(cells);
+  grid.z = njobs;
+  const dim3 tgrid((unsigned)segs, (unsigned)(trows < 65535 ? trows : 65535), njobs);
   const dim3 tblock(ls::kEikTileW * ls::kEikTileH);
+  const dim3 sgrid((unsigned)(tiles < ls::kEikSweepGroups ? tiles : ls::kEikSweepGroups), 1,
+                   njobs);
+  dim3 agrid = grid_for(tiles);
+  agrid.z = njobs;
   // cells within which a source can pull a cell into the band, in tiles
   const int reach = (int)ceil(band * sqrt(2.0)) + 2;
   const int segs_r = (reach + ls::kEikTileW - 1) / ls::kEikTileW;
   const int tr_r = g.nd > 1 ? (reach + ls::kEikTileH - 1) / ls::kEikTileH : 0;
-  hipLaunchKernelGGL(ls::eik_init_kernel, grid, dim3(kBlock), 0, st, phi, g, buf[0], buf[1],
-                     info);
-  hipLaunchKernelGGL(ls::eik_tile_src_kernel, tgrid, tblock, 0, st, (const uint8_t*)info, g,
-                     trows, src);
-  hipLaunchKernelGGL(ls::eik_tile_act_kernel, grid_for(tiles), dim3(kBlock), 0, st,
-                     (const uint8_t*)src, trows, segs, tr_r, segs_r, act);
-  hipLaunchKernelGGL(ls::eik_ctl_init_kernel, dim3(1), dim3(1), 0, st, c->eik);
+  hipLaunchKernelGGL(ls::eik_init_kernel, grid, dim3(kBlock), 0, st, jobs, g);
+  hipLaunchKernelGGL(ls::eik_tile_src_kernel, tgrid, tblock, 0, st, jobs, g, trows);
+  hipLaunchKernelGGL(ls::eik_ctl_init_kernel, dim3(1), dim3(64), 0, st, jobs, njobs);
+  hipLaunchKernelGGL(ls::eik_tile_act_kernel, agrid, dim3(kBlock), 0, st, jobs, trows, segs,
+                     tr_r, segs_r);
   FL_HIP(hipGetLastError());
-  // Every sweep strictly lowers some cell, so the loop ends; the bound only
-  // guards the host against a device that never reports convergence.
+  // Every sweep strictly lowers some cell of an unconverged job, so the loop
+  // ends; the bound only guards the host against a device that never
+  // reports convergence.
   const int64_t max_sweeps = cells + 2;
   int64_t launched = 0;
   for (;;) {
-    for (int b = 0; b < kEikBatch; ++b, ++launched) {
-      const int k = (int)launched;
-      hipLaunchKernelGGL(ls::eik_sweep_kernel, tgrid, tblock, 0, st,
-                         (const double*)buf[k & 1], (const uint8_t*)info, g, trows, dh, limit,
-                         buf[(k + 1) & 1], c->eik, k, (const uint8_t*)act);
-    }
+    for (int b = 0; b < kEikBatch; ++b, ++launched)
+      hipLaunchKernelGGL(ls::eik_sweep_kernel, sgrid, tblock, 0, st, jobs, g, segs, dh, limit,
+                         (int)launched);
     FL_HIP(hipGetLastError());
-    FL_HIP(hipMemcpyAsync(c->host_eik, c->eik, sizeof(ls::EikCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipMemcpyAsync(c->host_eik, c->eik, njobs * sizeof(ls::EikCtl),
+                          hipMemcpyDeviceToHost, st));
     FL_HIP(hipStreamSynchronize(st));
-    const ls::EikCtl h = *c->host_eik;
-    if (h.done || h.flags[(h.iters - 1) % 3] == 0) break;
+    bool converged = true;
+    for (int j = 0; j < njobs; ++j) {
+      const ls::EikCtl& h = c->host_eik[j];
+      converged = converged && (h.done || h.flags[(h.iters - 1) % 3] == 0);
+    }
+    if (converged) break;
     if (launched >= max_sweeps)
       return fail(FL_ERR_NUMERIC, "eikonal solve did not converge in %lld sweeps",
                   (long long)launched);
   }
-  const int64_t iters = c->host_eik->iters;
-  hipLaunchKernelGGL(ls::eik_finish_kernel, grid, dim3(kBlock), 0, st, phi,
-                     (const double*)buf[iters & 1], g, limit, clamp, out, band_mask);
+  hipLaunchKernelGGL(ls::eik_finish_kernel, grid, dim3(kBlock), 0, st, jobs, g, limit, clamp);
   FL_HIP(hipGetLastError());
-  if (iterations) *iterations = iters;
+  if (iterations)
+    for (int j = 0; j < njobs; ++j) iterations[j] = c->host_eik[j].iters;
   return FL_OK;
 }
 
@@ -301,7 +324,7 @@ struct MmWork {
   double* V[2];
   double* adv;      // 2 * cells
   double* tphi[2];
-  void* reinit;
+  void* reinit[2];  // reinit_work_bytes each
   uint8_t* has[2];
   int64_t bytes;
 };
@@ -326,7 +349,8 @@ MmWork mm_layout(void* base, const ls::Grid& g) {
   w.adv = (double*)take(2 * cells * 8);
   w.tphi[0] = (double*)take(cells * 8);
   w.tphi[1] = (double*)take(cells * 8);
-  w.reinit = take(reinit_work_bytes(g));
+  w.reinit[0] = take(reinit_work_bytes(g));
+  w.reinit[1] = take(reinit_work_bytes(g));
   w.has[0] = (uint8_t*)take(cells);
   w.has[1] = (uint8_t*)take(cells);
   w.bytes = off;
@@ -404,7 +428,9 @@ int fl_ls_reinit(fl_context* c, const double* phi, int ndim, const int64_t* n, d
   if (out == phi) return fail(FL_ERR_ARGUMENT, "out aliases phi");
   if (!aligned(work)) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
   DeviceGuard dg(c->device);
-  return run_reinit(c, phi, g, dh, band, out, band_mask, work, iterations, (hipStream_t)stream);
+  ls::EikJobs jobs;
+  jobs.j[0] = jobs.j[1] = eik_job(phi, g, out, band_mask, work, c->eik);
+  return run_reinit(c, jobs, 1, g, dh, band, iterations, (hipStream_t)stream);
 }
 
 int fl_gfm_star(const double* phi_m, const double* phi_o, const fl_field* state_m,
@@ -572,11 +598,13 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
       if ((rc = launch_advect(cur[m], w.V[m], g, dt, dh, nxt[m], w.adv, st))) return rc;
     if ((rc = launch_project(nxt[0], nxt[1], cells, st))) return rc;
     const bool reinit = (it % every) == every - 1;
-    for (int m = 0; m < 2; ++m) {
-      if (reinit) {
-        if ((rc = run_reinit(c, nxt[m], g, dh, band, cur[m], nullptr, w.reinit, nullptr, st)))
-          return rc;
-      } else {
+    if (reinit) {  // both level sets in one solve, written back into cur[m]
+      ls::EikJobs ej;
+      for (int m = 0; m < 2; ++m)
+        ej.j[m] = eik_job(nxt[m], g, cur[m], nullptr, w.reinit[m], c->eik + m);
+      if ((rc = run_reinit(c, ej, 2, g, dh, band, nullptr, st))) return rc;
+    } else {
+      for (int m = 0; m < 2; ++m) {
         double* tmp = cur[m];
         cur[m] = nxt[m];
         nxt[m] = tmp;
