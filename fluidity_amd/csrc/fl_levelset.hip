@@ -583,26 +583,23 @@ __device__ void interp_state(const StateView& v, const Grid& g, int64_t x, int64
   }
 }
 
-// RiemannGFM<3>::invoke (ghost_fluid/riemann_ghost_fluid.hpp:50-195).
+// RiemannGFM<3>::invoke (ghost_fluid/riemann_ghost_fluid.hpp:50-195) at an
+// inside-interfacial cell i = (x, y) of material m: p = phi_m[i], po =
+// phi_o[i].  Writes star[i] and sets has_star[i] when the cell takes a star
+// state.
 template <int ND>
-__global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* __restrict__ phi_o,
-                                StateView um, StateView uo, Eos em, Eos eo, Grid g, double dh,
-                                double* __restrict__ star, uint8_t* __restrict__ has_star,
-                                int* __restrict__ count_reset) {
+__device__ __forceinline__ void star_cell(const double* __restrict__ phi_m, double po,
+                                          const StateView& um, const StateView& uo,
+                                          const Eos& em, const Eos& eo, const Grid& g, double dh,
+                                          int64_t x, int64_t y, int64_t i, double p,
+                                          double* __restrict__ star,
+                                          uint8_t* __restrict__ has_star) {
   constexpr int S = 2 + ND;
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
-  // zeroes the band-list count the extrapolation that follows appends to
-  if (count_reset && i == 0) *count_reset = 0;
-  const int64_t x = i % g.nx, y = i / g.nx;
-  has_star[i] = 0;
-  const double p = phi_m[i];
-  if (!(mask_value(p, dh) & 8)) return;
   for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
     if (phi_at(phi_m, g, x - dx, y - dy) == 0.0 || phi_at(phi_m, g, x + dx, y + dy) == 0.0) return;
   }
-  const uint8_t mo = mask_value(phi_o[i], dh);
+  const uint8_t mo = mask_value(po, dh);
   if (!((mo & 16) || (mo & 4))) return;
   double nrm[2] = {0.0, 0.0};
   norm_at(phi_m, g, x, y, dh, nrm);
@@ -647,6 +644,48 @@ __global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* 
   has_star[i] = 1;
 }
 
+// Star states of one material (one thread per cell); count_reset (nullable)
+// is zeroed for the extrapolation that follows.
+template <int ND>
+__global__ void gfm_star_kernel(const double* __restrict__ phi_m, const double* __restrict__ phi_o,
+                                StateView um, StateView uo, Eos em, Eos eo, Grid g, double dh,
+                                double* __restrict__ star, uint8_t* __restrict__ has_star,
+                                int* __restrict__ count_reset) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  if (count_reset && i == 0) *count_reset = 0;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  has_star[i] = 0;
+  const double p = phi_m[i];
+  if (!(mask_value(p, dh) & 8)) return;
+  star_cell<ND>(phi_m, phi_o[i], um, uo, em, eo, g, dh, x, y, i, p, star, has_star);
+}
+
+// Star states of both materials of a two-material step in one pass: thread i
+// tests cell i for both, from one load of each level set; both band-list
+// counts are zeroed.
+template <int ND>
+__global__ void gfm_star2_kernel(const double* __restrict__ phi0, const double* __restrict__ phi1,
+                                 StateView u0, StateView u1, Eos e0, Eos e1, Grid g, double dh,
+                                 double* __restrict__ star0, double* __restrict__ star1,
+                                 uint8_t* __restrict__ has0, uint8_t* __restrict__ has1,
+                                 int* __restrict__ count0, int* __restrict__ count1) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  if (i == 0) {
+    *count0 = 0;
+    *count1 = 0;
+  }
+  const int64_t x = i % g.nx, y = i / g.nx;
+  const double p0 = phi0[i], p1 = phi1[i];
+  has0[i] = 0;
+  has1[i] = 0;
+  if (mask_value(p0, dh) & 8)
+    star_cell<ND>(phi0, p1, u0, u1, e0, e1, g, dh, x, y, i, p0, star0, has0);
+  if (mask_value(p1, dh) & 8)
+    star_cell<ND>(phi1, p0, u1, u0, e1, e0, g, dh, x, y, i, p1, star1, has1);
+}
+
 __device__ __forceinline__ bool in_band(double p, double upper) { return p > 0.0 && p <= upper; }
 
 // Extrapolation source field, in both Jacobi buffers, and the band list, in
@@ -657,25 +696,40 @@ __device__ __forceinline__ bool in_band(double p, double upper) { return p > 0.0
 // in no particular order: each band cell's update is independent of the
 // others within a sweep.
 template <int ND>
-__global__ void gfm_band_seed_kernel(const double* __restrict__ phi_m, double upper,
-                                     StateView um, const double* __restrict__ star,
-                                     const uint8_t* __restrict__ has_star, Grid g,
-                                     double* __restrict__ A, double* __restrict__ B,
-                                     int* __restrict__ list, int* __restrict__ count) {
-  constexpr int S = 2 + ND;
-  const int64_t i = gid();
-  if (i >= g.nx * g.ny) return;
-  const int64_t x = i % g.nx, y = i / g.nx;
-  const bool band = in_band(phi_m[i], upper);
-  bool needed = band;
+__device__ __forceinline__ void band_flags(const double* __restrict__ phi_m, double upper,
+                                           const Grid& g, int64_t x, int64_t y, int64_t i,
+                                           bool& band, bool& needed) {
+  band = in_band(phi_m[i], upper);
+  needed = band;
   if (x > 0) needed |= in_band(phi_m[i - 1], upper);
   if (x < g.nx - 1) needed |= in_band(phi_m[i + 1], upper);
   if (ND > 1) {
     if (y > 0) needed |= in_band(phi_m[i - g.nx], upper);
     if (y < g.ny - 1) needed |= in_band(phi_m[i + g.nx], upper);
   }
-  if (!needed) return;
-  if (band) list[atomicAdd(count, 1)] = (int)i;
+}
+
+// Appends cell i to `list` where `take` holds: one atomicAdd per wave.  Every
+// lane of the wave that is still running must call it.
+__device__ __forceinline__ void list_append(bool take, int64_t i, int* __restrict__ list,
+                                            int* __restrict__ count) {
+  const unsigned long long m = __ballot(take);
+  if (!m) return;
+  const int lane = (int)(threadIdx.x % 64);
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)i;
+}
+
+template <int ND>
+__device__ __forceinline__ void band_seed_cell(const StateView& um,
+                                               const double* __restrict__ star,
+                                               const uint8_t* __restrict__ has_star,
+                                               const Grid& g, int64_t x, int64_t y, int64_t i,
+                                               double* __restrict__ A, double* __restrict__ B) {
+  constexpr int S = 2 + ND;
   double u[S];
   if (has_star[i]) {
 #pragma unroll
@@ -688,6 +742,40 @@ __global__ void gfm_band_seed_kernel(const double* __restrict__ phi_m, double up
     A[i * S + k] = u[k];
     B[i * S + k] = u[k];
   }
+}
+
+template <int ND>
+__global__ void gfm_band_seed_kernel(const double* __restrict__ phi_m, double upper,
+                                     StateView um, const double* __restrict__ star,
+                                     const uint8_t* __restrict__ has_star, Grid g,
+                                     double* __restrict__ A, double* __restrict__ B,
+                                     int* __restrict__ list, int* __restrict__ count) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  bool band, needed;
+  band_flags<ND>(phi_m, upper, g, x, y, i, band, needed);
+  list_append(band, i, list, count);
+  if (needed) band_seed_cell<ND>(um, star, has_star, g, x, y, i, A, B);
+}
+
+// Both materials' band seeds in one pass (thread i takes cell i of both).
+template <int ND>
+__global__ void gfm_band_seed2_kernel(ExtrapJob j0, ExtrapJob j1, double upper,
+                                      const double* __restrict__ star0,
+                                      const double* __restrict__ star1,
+                                      const uint8_t* __restrict__ has0,
+                                      const uint8_t* __restrict__ has1, Grid g) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  bool band0, needed0, band1, needed1;
+  band_flags<ND>(j0.phi, upper, g, x, y, i, band0, needed0);
+  band_flags<ND>(j1.phi, upper, g, x, y, i, band1, needed1);
+  list_append(band0, i, const_cast<int*>(j0.list), j0.count);
+  list_append(band1, i, const_cast<int*>(j1.list), j1.count);
+  if (needed0) band_seed_cell<ND>(j0.um, star0, has0, g, x, y, i, j0.A, j0.B);
+  if (needed1) band_seed_cell<ND>(j1.um, star1, has1, g, x, y, i, j1.A, j1.B);
 }
 
 __global__ void count_clear_kernel(int* count) { *count = 0; }
@@ -783,6 +871,12 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
   template __global__ void gfm_band_seed_kernel<ND>(const double*, double, StateView,       \
                                                     const double*, const uint8_t*, Grid,      \
                                                     double*, double*, int*, int*);            \
+  template __global__ void gfm_star2_kernel<ND>(const double*, const double*, StateView,     \
+                                                StateView, Eos, Eos, Grid, double, double*,  \
+                                                double*, uint8_t*, uint8_t*, int*, int*);    \
+  template __global__ void gfm_band_seed2_kernel<ND>(ExtrapJob, ExtrapJob, double,          \
+                                                     const double*, const double*,           \
+                                                     const uint8_t*, const uint8_t*, Grid);  \
   template __global__ void gfm_extrap_kernel<ND>(ExtrapJob, ExtrapJob, Grid, double, int,   \
                                                  int);                                        \
   template __global__ void combine_kernel<ND>(const double*, StateView, StateView, Grid,     \

@@ -82,6 +82,12 @@ template <int ND>
 __global__ void gfm_star_kernel(const double* phi_m, const double* phi_o, StateView um,
                                 StateView uo, Eos em, Eos eo, Grid g, double dh, double* star,
                                 uint8_t* has_star, int* count_reset);
+// Both materials of a two-material step in one pass each.
+template <int ND>
+__global__ void gfm_star2_kernel(const double* phi0, const double* phi1, StateView u0,
+                                 StateView u1, Eos e0, Eos e1, Grid g, double dh, double* star0,
+                                 double* star1, uint8_t* has0, uint8_t* has1, int* count0,
+                                 int* count1);
 // One extrapolation job: a material's level set and state, its two Jacobi
 // buffers (AoS), and its band list with the list's count.
 struct ExtrapJob {
@@ -98,6 +104,10 @@ template <int ND>
 __global__ void gfm_band_seed_kernel(const double* phi_m, double upper, StateView um,
                                      const double* star, const uint8_t* has_star, Grid g,
                                      double* A, double* B, int* list, int* count);
+template <int ND>
+__global__ void gfm_band_seed2_kernel(ExtrapJob j0, ExtrapJob j1, double upper,
+                                      const double* star0, const double* star1,
+                                      const uint8_t* has0, const uint8_t* has1, Grid g);
 __global__ void count_clear_kernel(int* count);
 template <int ND>
 __global__ void gfm_extrap_kernel(ExtrapJob j0, ExtrapJob j1, Grid g, double dh, int sweep,

@@ -262,6 +262,12 @@ ls::ExtrapJob extrap_job(const double* phi_m, ls::StateView um, const ls::Grid& 
 
 double extrap_upper(double dh) { return 6.0 * dh * (1.0 / sqrt(2.0)); }
 
+// The two materials' star-state planes and star flags of a step.
+struct MmGhost {
+  double* star[2];
+  uint8_t* has[2];
+};
+
 // Seeds the Jacobi buffers and lists the band of one material (the job's
 // count must be zero).
 void band_seed(const ls::ExtrapJob& j, const double* star, const uint8_t* has,
@@ -273,6 +279,39 @@ void band_seed(const ls::ExtrapJob& j, const double* star, const uint8_t* has,
   else
     hipLaunchKernelGGL(ls::gfm_band_seed_kernel<2>, grid, dim3(kBlock), 0, st, j.phi,
                        extrap_upper(dh), j.um, star, has, g, j.A, j.B, (int*)j.list, j.count);
+}
+
+// Star states of both materials (each from the same level sets and states)
+// and zeroed band-list counts of both jobs.
+void launch_star2(double* const cur[2], const fl_material* mats, const ls::Grid& g, double dh,
+                  const MmGhost& w, const ls::ExtrapJob jobs[2], hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  const ls::StateView u0 = view_of(&mats[0].state), u1 = view_of(&mats[1].state);
+  const ls::Eos e0 = eos_of(&mats[0].scheme), e1 = eos_of(&mats[1].scheme);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::gfm_star2_kernel<1>, grid, dim3(kBlock), 0, st,
+                       (const double*)cur[0], (const double*)cur[1], u0, u1, e0, e1, g, dh,
+                       w.star[0], w.star[1], w.has[0], w.has[1], jobs[0].count, jobs[1].count);
+  else
+    hipLaunchKernelGGL(ls::gfm_star2_kernel<2>, grid, dim3(kBlock), 0, st,
+                       (const double*)cur[0], (const double*)cur[1], u0, u1, e0, e1, g, dh,
+                       w.star[0], w.star[1], w.has[0], w.has[1], jobs[0].count, jobs[1].count);
+}
+
+// Band seeds of both materials (the jobs' counts must be zero).
+void band_seed2(const ls::ExtrapJob jobs[2], const MmGhost& w, const ls::Grid& g, double dh,
+                hipStream_t st) {
+  const dim3 grid = grid_for(g.nx * g.ny);
+  if (g.nd == 1)
+    hipLaunchKernelGGL(ls::gfm_band_seed2_kernel<1>, grid, dim3(kBlock), 0, st, jobs[0],
+                       jobs[1], extrap_upper(dh), (const double*)w.star[0],
+                       (const double*)w.star[1], (const uint8_t*)w.has[0],
+                       (const uint8_t*)w.has[1], g);
+  else
+    hipLaunchKernelGGL(ls::gfm_band_seed2_kernel<2>, grid, dim3(kBlock), 0, st, jobs[0],
+                       jobs[1], extrap_upper(dh), (const double*)w.star[0],
+                       (const double*)w.star[1], (const uint8_t*)w.has[0],
+                       (const uint8_t*)w.has[1], g);
 }
 
 // The sweeps and write-back of `njobs` (1 or 2) materials: one launch per
@@ -547,13 +586,9 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     ls::ExtrapJob jobs[2];
     for (int m = 0; m < 2; ++m)
       jobs[m] = extrap_job(cur[m], view_of(&mats[m].state), g, w.extrap[m]);
-    for (int m = 0; m < 2; ++m) {
-      const int o = 1 - m;
-      if ((rc = launch_star(cur[m], cur[o], &mats[m].state, &mats[o].state, &mats[m].scheme,
-                            &mats[o].scheme, g, dh, w.star[m], w.has[m], st, jobs[m].count)))
-        return rc;
-    }
-    for (int m = 0; m < 2; ++m) band_seed(jobs[m], w.star[m], w.has[m], g, dh, st);
+    const MmGhost gw = {{w.star[0], w.star[1]}, {w.has[0], w.has[1]}};
+    launch_star2(cur, mats, g, dh, gw, jobs, st);
+    band_seed2(jobs, gw, g, dh, st);
     extrap_sweeps(jobs[0], jobs[1], 2, g, dh, sweeps, st);
     FL_HIP(hipGetLastError());
     // dt from the maximum wavespeed over both grids
