@@ -27,6 +27,12 @@
 namespace fl {
 namespace ls {
 
+// Empty statement that uses a loaded value: the compiler cannot move the
+// load below it, so loads issued before it are waited for together instead
+// of one at a time inside the branches that use them.
+__device__ __forceinline__ void keep(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void keep(int v) { asm volatile("" ::"v"(v)); }
+
 // Mirrored ghost index; clamped so that no input can address outside the grid.
 __device__ __forceinline__ int64_t mirror(int64_t i, int64_t n) {
   if (i < 0) i = -i - 1;
@@ -41,6 +47,14 @@ __device__ __forceinline__ double phi_at(const double* __restrict__ phi, const G
   return phi[yi * g.nx + xi];
 }
 
+// Cell index of the mirrored cell (x, y); ND = g.nd.
+template <int ND>
+__device__ __forceinline__ int64_t mirror_index(const Grid& g, int64_t x, int64_t y) {
+  const int64_t xi = mirror(x, g.nx);
+  const int64_t yi = ND > 1 ? mirror(y, g.ny) : 0;
+  return yi * g.nx + xi;
+}
+
 // levelset.hpp:166-217 as mask bits.
 __device__ __forceinline__ uint8_t mask_value(double p, double dh) {
   const int inside = (p <= 0.0) || (p <= -0.0);
@@ -52,18 +66,22 @@ __device__ __forceinline__ uint8_t mask_value(double p, double dh) {
                    (out_ifc << 4));
 }
 
-// MultidimIterator::norm (iterator/multidim_iterator.hpp:291-306).
+// MultidimIterator::norm (iterator/multidim_iterator.hpp:291-306); ND = g.nd.
+// Unrolled so that the 2 ND neighbour loads are issued together.
+template <int ND>
 __device__ __forceinline__ void norm_at(const double* __restrict__ phi, const Grid& g,
                                         int64_t x, int64_t y, double dh, double* nrm) {
   const double fact = 0.5 / dh;
   double mag = 0.0;
-  for (int d = 0; d < g.nd; ++d) {
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
     nrm[d] = fact * (phi_at(phi, g, x + dx, y + dy) - phi_at(phi, g, x - dx, y - dy));
     mag += nrm[d] * nrm[d];
   }
   const double s = sqrt(mag);
-  for (int d = 0; d < g.nd; ++d) nrm[d] = (mag == 0.0) ? 0.0 : nrm[d] / s;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) nrm[d] = (mag == 0.0) ? 0.0 : nrm[d] / s;
 }
 
 template <int S>
@@ -113,7 +131,7 @@ __global__ void velocity_kernel(const double* __restrict__ phi0, const double* _
     const double p = phi[i];
     if ((p <= 0.0) || (p <= -0.0)) {
       double nrm[2];
-      norm_at(target < 0 ? phi : (target == 0 ? phi0 : phi1), g, x, y, dh, nrm);
+      norm_at<ND>(target < 0 ? phi : (target == 0 ? phi0 : phi1), g, x, y, dh, nrm);
       double u[2 + ND];
       load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
       double sum = 0.0;
@@ -136,25 +154,30 @@ __global__ void velocity2_kernel(const double* __restrict__ phi0, const double* 
   const int64_t i = gid();
   if (i >= g.nx * g.ny) return;
   const int64_t x = i % g.nx, y = i / g.nx;
+  // the first material whose phi <= 0 (both values loaded up front)
+  // (bitwise ors: no short circuit that would move the second load after
+  // the first test)
+  const double p0 = phi0[i], p1 = phi1[i];
+  keep(p0);
+  keep(p1);
+  const bool in0 = ((int)(p0 <= 0.0) | (int)(p0 <= -0.0)) != 0;
+  const bool in1 = ((int)(p1 <= 0.0) | (int)(p1 <= -0.0)) != 0;
+  const int m = in0 ? 0 : (in1 ? 1 : -1);
   double v0 = 0.0, v1 = 0.0;
-  for (int m = 0; m < 2; ++m) {
-    const double p = (m == 0 ? phi0 : phi1)[i];
-    if ((p <= 0.0) || (p <= -0.0)) {
-      double n0[2], n1[2];
-      norm_at(phi0, g, x, y, dh, n0);
-      norm_at(phi1, g, x, y, dh, n1);
-      double u[2 + ND];
-      load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
-      double s0 = 0.0, s1 = 0.0;
+  if (m >= 0) {
+    double n0[2], n1[2];
+    norm_at<ND>(phi0, g, x, y, dh, n0);
+    norm_at<ND>(phi1, g, x, y, dh, n1);
+    double u[2 + ND];
+    load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        s0 += (u[2 + d] / u[0]) * n0[d];
-        s1 += (u[2 + d] / u[0]) * n1[d];
-      }
-      v0 = s0;
-      v1 = s1;
-      break;
+    for (int d = 0; d < ND; ++d) {
+      s0 += (u[2 + d] / u[0]) * n0[d];
+      s1 += (u[2 + d] / u[0]) * n1[d];
     }
+    v0 = s0;
+    v1 = s1;
   }
   V0[i] = v0;
   V1[i] = v1;
@@ -293,16 +316,17 @@ __global__ void eik_init_kernel(EikJobs jobs, Grid g) {
   uint8_t* __restrict__ info = J.info;
   const int64_t x = i % g.nx, y = i / g.nx;
   const double c = phi[i];
+  // the four axis neighbours (those past the edge, or in y on a 1D grid,
+  // read the cell itself and do not count), loaded together
+  const bool ok[4] = {x + 1 < g.nx, x > 0, g.nd > 1 && y + 1 < g.ny, g.nd > 1 && y > 0};
+  const int64_t nb[4] = {i + 1, i - 1, i + g.nx, i - g.nx};
+  double o[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = phi[ok[k] ? nb[k] : i];
   int source = 0;
-  for (int dd = 0; dd < g.nd && !source; ++dd) {
-    const int64_t dx = (dd == 0), dy = (dd == 1);
-    for (int s = 0; s < 2 && !source; ++s) {
-      const int64_t ox = x + (s == 0 ? dx : -dx), oy = y + (s == 0 ? dy : -dy);
-      if (ox < 0 || ox >= g.nx || oy < 0 || oy >= g.ny) continue;
-      const double o = phi[oy * g.nx + ox];
-      if (signum(c) != signum(o) && fabs(c) < fabs(o)) source = 1;
-    }
-  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (ok[k] && signum(c) != signum(o[k]) && fabs(c) < fabs(o[k])) source = 1;
   info[i] = (uint8_t)(source | ((signum(c) + 1) << 1));
   const double v = source ? fabs(c) : DBL_MAX;
   d[i] = v;
@@ -363,13 +387,10 @@ __global__ void eik_tile_act_kernel(EikJobs jobs, int64_t trows, int64_t segs, i
   if (a) J.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)t;
 }
 
-// Neighbour distance seen from a cell with sign bits sc: negated across the
-// interface unless not yet reached.
-__device__ __forceinline__ double eik_nbr(const double* __restrict__ d,
-                                          const uint8_t* __restrict__ info, int64_t j,
-                                          uint8_t sc) {
-  const double v = d[j];
-  return ((info[j] & 6) == sc || v == DBL_MAX) ? v : -v;
+// Neighbour distance v (info byte inf) seen from a cell with sign bits sc:
+// negated across the interface unless not yet reached.
+__device__ __forceinline__ double eik_nbr_v(double v, uint8_t inf, uint8_t sc) {
+  return ((inf & 6) == sc || v == DBL_MAX) ? v : -v;
 }
 
 // One thread per job.
@@ -404,8 +425,14 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
   const int* __restrict__ list = J.list;
   EikCtl* __restrict__ ctl = J.ctl;
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-  if (ctl->done) return;
-  if (k > 0 && ctl->flags[(k - 1) % 3] == 0) {
+  // the control words, loaded together ((k + 2) % 3 is sweep k-1's flag)
+  const int done = ctl->done, prev_changed = ctl->flags[(k + 2) % 3];
+  const int ntiles = ctl->ntiles;
+  keep(done);
+  keep(prev_changed);
+  keep(ntiles);
+  if (done) return;
+  if (k > 0 && prev_changed == 0) {
     if (first) ctl->done = 1;
     return;
   }
@@ -413,7 +440,6 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
     ctl->flags[(k + 1) % 3] = 0;
     ctl->iters = k + 1;
   }
-  const int ntiles = ctl->ntiles;
   bool changed = false;
   for (int li = blockIdx.x; li < ntiles; li += gridDim.x) {
     const int64_t t = list[li];
@@ -424,18 +450,31 @@ __global__ void __launch_bounds__(kEikTileW * kEikTileH) eik_sweep_kernel(
     const double cur = d[i];
     double nv = cur;
     const uint8_t me = info[i];
+    // neighbours loaded together, before the branch (those past the edge
+    // read the cell itself and are replaced by DBL_MAX)
+    const bool okxm = x > 0, okxp = x < g.nx - 1;
+    const bool okym = g.nd > 1 && y > 0, okyp = g.nd > 1 && y < g.ny - 1;
+    const int64_t jxm = okxm ? i - 1 : i, jxp = okxp ? i + 1 : i;
+    const int64_t jym = okym ? i - g.nx : i, jyp = okyp ? i + g.nx : i;
+    const double dxm = d[jxm], dxp = d[jxp], dym = d[jym], dyp = d[jyp];
+    const uint8_t ixm = info[jxm], ixp = info[jxp], iym = info[jym], iyp = info[jyp];
+    keep(dxm);
+    keep(dxp);
+    keep(dym);
+    keep(dyp);
+    keep((int)(ixm | (ixp << 8) | (iym << 16) | (iyp << 24)));
     if (!(me & 1)) {
       const uint8_t sc = me & 6;
       const double fh = 1.0 * dh;
-      const double xm = x > 0 ? eik_nbr(d, info, i - 1, sc) : DBL_MAX;
-      const double xp = x < g.nx - 1 ? eik_nbr(d, info, i + 1, sc) : DBL_MAX;
+      const double xm = okxm ? eik_nbr_v(dxm, ixm, sc) : DBL_MAX;
+      const double xp = okxp ? eik_nbr_v(dxp, ixp, sc) : DBL_MAX;
       const double a = smin(xm, xp);
       double q;
       if (g.nd == 1) {
         q = a + fh;
       } else {
-        const double ym = y > 0 ? eik_nbr(d, info, i - g.nx, sc) : DBL_MAX;
-        const double yp = y < g.ny - 1 ? eik_nbr(d, info, i + g.nx, sc) : DBL_MAX;
+        const double ym = okym ? eik_nbr_v(dym, iym, sc) : DBL_MAX;
+        const double yp = okyp ? eik_nbr_v(dyp, iyp, sc) : DBL_MAX;
         const double b = smin(ym, yp);
         const double amb = a - b;
         q = fabs(amb) >= fh ? smin(a, b) + fh
@@ -602,7 +641,7 @@ __device__ __forceinline__ void star_cell(const double* __restrict__ phi_m, doub
   const uint8_t mo = mask_value(po, dh);
   if (!((mo & 16) || (mo & 4))) return;
   double nrm[2] = {0.0, 0.0};
-  norm_at(phi_m, g, x, y, dh, nrm);
+  norm_at<ND>(phi_m, g, x, y, dh, nrm);
   const double abs_phi = fabs(p);
   const double dh_three_two = 1.5 * dh;
   double w[2] = {0.0, 0.0}, wl[S], wr[S];
@@ -686,7 +725,11 @@ __global__ void gfm_star2_kernel(const double* __restrict__ phi0, const double* 
     star_cell<ND>(phi1, p0, u1, u0, e1, e0, g, dh, x, y, i, p1, star1, has1);
 }
 
-__device__ __forceinline__ bool in_band(double p, double upper) { return p > 0.0 && p <= upper; }
+// 0 < p <= upper, as an int so that several can be or-ed without a short
+// circuit.
+__device__ __forceinline__ int in_band(double p, double upper) {
+  return (int)(p > 0.0) & (int)(p <= upper);
+}
 
 // Extrapolation source field, in both Jacobi buffers, and the band list, in
 // one pass: the material's own state (AoS scratch), with the star state at
@@ -699,14 +742,20 @@ template <int ND>
 __device__ __forceinline__ void band_flags(const double* __restrict__ phi_m, double upper,
                                            const Grid& g, int64_t x, int64_t y, int64_t i,
                                            bool& band, bool& needed) {
-  band = in_band(phi_m[i], upper);
-  needed = band;
-  if (x > 0) needed |= in_band(phi_m[i - 1], upper);
-  if (x < g.nx - 1) needed |= in_band(phi_m[i + 1], upper);
+  // neighbours past the edge read the cell itself, which adds nothing to
+  // `needed`; the loads are unconditional so that they are issued together
+  const double pc = phi_m[i];
+  const double pxm = phi_m[x > 0 ? i - 1 : i];
+  const double pxp = phi_m[x < g.nx - 1 ? i + 1 : i];
+  const int b = in_band(pc, upper);
+  int n = b | in_band(pxm, upper) | in_band(pxp, upper);
   if (ND > 1) {
-    if (y > 0) needed |= in_band(phi_m[i - g.nx], upper);
-    if (y < g.ny - 1) needed |= in_band(phi_m[i + g.nx], upper);
+    const double pym = phi_m[y > 0 ? i - g.nx : i];
+    const double pyp = phi_m[y < g.ny - 1 ? i + g.nx : i];
+    n |= in_band(pym, upper) | in_band(pyp, upper);
   }
+  band = b != 0;
+  needed = n != 0;
 }
 
 // Appends cell i to `list` where `take` holds: one atomicAdd per wave.  Every
@@ -790,7 +839,7 @@ __device__ __forceinline__ void extrap_cell(const double* __restrict__ phi_m,
   const double p = phi_m[i];
   const int64_t x = i % g.nx, y = i / g.nx;
   double nrm[2] = {0.0, 0.0};
-  norm_at(phi_m, g, x, y, dh, nrm);
+  norm_at<ND>(phi_m, g, x, y, dh, nrm);
   const int sg = signum(p);
   double l1 = 0.0;
 #pragma unroll
@@ -804,13 +853,20 @@ __device__ __forceinline__ void extrap_cell(const double* __restrict__ phi_m,
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
-    const int dir = phi_at(phi_m, g, x - dx, y - dy) < phi_at(phi_m, g, x + dx, y + dy) ? -1 : 1;
-    const int64_t ux = mirror(x + dir * dx, g.nx);
-    const int64_t uy = ND > 1 ? mirror(y + dir * dy, g.ny) : 0;
-    const double* src = A + (uy * g.nx + ux) * S;
+    // both neighbours' states are loaded with the level-set values; the
+    // upwind one is taken
+    const double* lo = A + (mirror_index<ND>(g, x - dx, y - dy)) * S;
+    const double* hi = A + (mirror_index<ND>(g, x + dx, y + dy)) * S;
+    double ul[S], uh[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      ul[k] = lo[k];
+      uh[k] = hi[k];
+    }
+    const bool down = phi_at(phi_m, g, x - dx, y - dy) < phi_at(phi_m, g, x + dx, y + dy);
     const double wgt = l1 > 1e-2 ? fabs(nrm[d]) : 0.5;
 #pragma unroll
-    for (int k = 0; k < S; ++k) acc[k] += src[k] * wgt;
+    for (int k = 0; k < S; ++k) acc[k] += (down ? ul[k] : uh[k]) * wgt;
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) B[i * S + k] = (l1 > 1e-2) ? acc[k] / l1 : acc[k];
