@@ -233,6 +233,90 @@ int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out,
   return launch_sweep_t<float>(ctx, in, out, dim, s, dtdh, ctl, ws_out, stream);
 }
 
+// Both materials' sweeps of dimension `dim` in one launch: in[m] -> out[m]
+// with scheme s[m].  The two grids share shape, layout and dtype (FP64); the
+// flux methods must be equal (the launch instantiates one), else each grid
+// is swept on its own.
+template <int FLUX>
+void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<double>& a0,
+                  const fl::SweepArgs<double>& a1, hipStream_t stream) {
+  if (ndim == 1)
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<double, 1, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+  else if (dim == 0)
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<double, 2, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+  else
+    hipLaunchKernelGGL((fl::sweep_y2_kernel<double, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+}
+
+int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
+                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream) {
+  if (s[0]->flux != s[1]->flux || in[0]->dtype != FL_F64) {
+    for (int m = 0; m < 2; ++m) {
+      const int rc = launch_sweep(ctx, in[m], out[m], dim, s[m], dtdh, nullptr, nullptr, stream);
+      if (rc) return rc;
+    }
+    return FL_OK;
+  }
+  fl_tuning tu = ctx->tuning;
+  const fl_field* f = in[0];
+  int rows = 0, B;
+  dim3 grid;
+  if (f->ndim == 1) {
+    B = tu.x_block;
+    grid = dim3((unsigned)((f->n[0] + (B - 3)) / (B - 2)), 1, 2);
+  } else {
+    const int64_t nx = f->n[0], ny = f->n[1];
+    // the waves of both grids count towards filling the machine
+    if (dim == 0) {
+      B = tu.x_block;
+      rows = fit_rows(tu.x_rows, 2 * ((nx + B - 3) / (B - 2)), ny, B);
+      grid = dim3((unsigned)((nx + B - 3) / (B - 2)), (unsigned)((ny + rows - 1) / rows), 2);
+    } else {
+      B = tu.y_block;
+      rows = fit_rows(tu.y_rows, 2 * ((nx + B - 1) / B), ny, B);
+      grid = dim3((unsigned)((nx + B - 1) / B), (unsigned)((ny + rows - 1) / rows), 2);
+    }
+  }
+  const fl::SweepArgs<double> a0 = make_args<double>(ctx, in[0], out[0], s[0], dtdh, nullptr,
+                                                     nullptr, f->ndim == 1 ? 1 : rows);
+  const fl::SweepArgs<double> a1 = make_args<double>(ctx, in[1], out[1], s[1], dtdh, nullptr,
+                                                     nullptr, f->ndim == 1 ? 1 : rows);
+  switch (s[0]->flux) {
+    case FL_FLUX_HLLC: launch_flux2<fl::kFluxHllc>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    case FL_FLUX_FORCE: launch_flux2<fl::kFluxForce>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    case FL_FLUX_LAX_FRIEDRICHS:
+      launch_flux2<fl::kFluxLaxFriedrichs>(f->ndim, dim, grid, B, a0, a1, stream);
+      break;
+    default: launch_flux2<fl::kFluxRichtmyer>(f->ndim, dim, grid, B, a0, a1, stream); break;
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// The max wavespeed over two FP64 grids of one shape and layout, max-ed into
+// dst (not cleared here).
+int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
+                   const fl_scheme* s1, unsigned long long* dst, hipStream_t stream) {
+  int64_t es, cs;
+  strides(f0, es, cs);
+  const int64_t n = cell_count(f0);
+  const int B = fl::kMaxBlock;
+  int64_t blocks = (n + B - 1) / B;
+  if (blocks > fl::kMaxWsBlocks) blocks = fl::kMaxWsBlocks;
+  if (blocks < 1) blocks = 1;
+  const fl::Material<double> m0 = make_material<double>(s0), m1 = make_material<double>(s1);
+  if (f0->ndim == 1)
+    hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 1>), dim3((unsigned)blocks), dim3(B), 0,
+                       stream, (const double*)f0->data, (const double*)f1->data, es, cs, n, m0,
+                       m1, dst);
+  else
+    hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 2>), dim3((unsigned)blocks), dim3(B), 0,
+                       stream, (const double*)f0->data, (const double*)f1->data, es, cs, n, m0,
+                       m1, dst);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
 int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst,
                   hipStream_t stream) {
   int64_t es, cs;

@@ -90,6 +90,10 @@ int64_t cell_count(const fl_field* f);
 int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
                  const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
                  unsigned long long* ws_out, hipStream_t stream);
+int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
+                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream);
+int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
+                   const fl_scheme* s1, unsigned long long* dst, hipStream_t stream);
 // atomicMax of the FP64 wavespeed bits of every cell into *dst.
 int launch_max_ws(const fl_field* f, const fl_scheme* s, unsigned long long* dst,
                   hipStream_t stream);

@@ -57,6 +57,13 @@ constexpr int kMaxWsUnroll = 4;
 namespace fl {
 template <typename T, int ND, int FLUX> __global__ void sweep_x_kernel(SweepArgs<T> a);
 template <typename T, int FLUX> __global__ void sweep_y_kernel(SweepArgs<T> a);
+template <typename T, int ND, int FLUX>
+__global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
+template <typename T, int FLUX> __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
+template <typename T, int ND>
+__global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es, int64_t cs,
+                                      int64_t ncells, Material<double> m0, Material<double> m1,
+                                      unsigned long long* dst);
 template <typename T, int ND>
 __global__ void max_wavespeed_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,
                                      Material<double> m64, unsigned long long* dst);

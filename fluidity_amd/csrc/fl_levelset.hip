@@ -232,21 +232,38 @@ __device__ __forceinline__ int mirror32(int i, int n) {
 }
 
 // L = V * H(phi) (levelset_hamiltonian.hpp:63-98, godunov_upwind.hpp:94-166).
-// h1 = 1 / dh, computed once by the caller (the same IEEE quotient).
+// h1 = 1 / dh, computed once by the caller (the same IEEE quotient).  ND =
+// g.nd; the 7-cell stencils of all dimensions are loaded before any is used.
+// xin / yin: every cell of the stencil in x / y lies inside the grid (the
+// caller's workgroup-uniform test), so the mirror index math is skipped.
+template <int ND>
 __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, int y, double v,
-                         double h1) {
+                         double h1, bool xin, bool yin) {
   double result = 0.0;
   const int nx = (int)g.nx, ny = (int)g.ny;
-  for (int d = 0; d < g.nd; ++d) {
-    double f[7];
-    if (d == 0) {
-      const double* row = phi + (int64_t)y * nx;
+  double fs[ND][7];
+  {
+    const double* row = phi + (int64_t)y * nx;
+    if (xin) {
 #pragma unroll
-      for (int k = -3; k <= 3; ++k) f[k + 3] = row[mirror32(x + k, nx)];
+      for (int k = -3; k <= 3; ++k) fs[0][k + 3] = row[x + k];
     } else {
 #pragma unroll
-      for (int k = -3; k <= 3; ++k) f[k + 3] = phi[(int64_t)mirror32(y + k, ny) * nx + x];
+      for (int k = -3; k <= 3; ++k) fs[0][k + 3] = row[mirror32(x + k, nx)];
     }
+  }
+  if (ND > 1) {
+    if (yin) {
+#pragma unroll
+      for (int k = -3; k <= 3; ++k) fs[ND - 1][k + 3] = phi[(int64_t)(y + k) * nx + x];
+    } else {
+#pragma unroll
+      for (int k = -3; k <= 3; ++k) fs[ND - 1][k + 3] = phi[(int64_t)mirror32(y + k, ny) * nx + x];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double* f = fs[d];
     const double bd = weno_diff(h1 * (f[1] - f[0]), h1 * (f[2] - f[1]), h1 * (f[3] - f[2]),
                                 h1 * (f[4] - f[3]), h1 * (f[5] - f[4]));
     const double fd = weno_diff(h1 * (f[6] - f[5]), h1 * (f[5] - f[4]), h1 * (f[4] - f[3]),
@@ -264,20 +281,24 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
   return v * sqrt(result);
 }
 
-// One RK3 stage (scheme/updaters/runge_kutta_3.hpp:117-148):
+// One RK3 stage (scheme/updaters/runge_kutta_3.hpp:117-148), ND = g.nd:
 //  stage 0: out = phi_n - dt L(phi_n)
 //  stage 1: out = 0.75 phi_n + 0.25 (s1 - dt L(s1))
 //  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
 // Launched on a (ceil(nx / blockDim.x), min(ny, 65535)) grid; rows stride by
 // gridDim.y.
+template <int ND>
 __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
                                  const double* __restrict__ src, const double* __restrict__ V,
                                  Grid g, double dt, double h1, double* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t x0 = (int64_t)blockIdx.x * blockDim.x;
+  const bool xin = x0 >= 3 && x0 + blockDim.x + 3 <= g.nx;  // workgroup-uniform
   if (x >= g.nx) return;
   for (int y = blockIdx.y; y < g.ny; y += gridDim.y) {
     const int64_t i = (int64_t)y * g.nx + x;
-    const double l = ls_rhs(src, g, x, y, V[i], h1);
+    const bool yin = y >= 3 && y + 3 < g.ny;
+    const double l = ls_rhs<ND>(src, g, x, y, V[i], h1, xin, yin);
     if (stage == 0) {
       out[i] = phi_n[i] - dt * l;
     } else if (stage == 1) {
@@ -921,6 +942,8 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
                                                StateView, int, Grid, double, int, double*); \
   template __global__ void velocity2_kernel<ND>(const double*, const double*, StateView,     \
                                                 StateView, Grid, double, double*, double*); \
+  template __global__ void rk3_stage_kernel<ND>(int, const double*, const double*,            \
+                                                const double*, Grid, double, double, double*); \
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
                                                uint8_t*, int*);                             \

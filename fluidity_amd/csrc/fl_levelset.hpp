@@ -63,6 +63,7 @@ template <int ND>
 __global__ void velocity2_kernel(const double* phi0, const double* phi1, StateView u0,
                                  StateView u1, Grid g, double dh, double* V0, double* V1);
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
+template <int ND>
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
                                  const double* V, Grid g, double dt, double h1, double* out);
 __global__ void eik_init_kernel(EikJobs jobs, Grid g);

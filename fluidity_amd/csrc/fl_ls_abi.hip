@@ -115,12 +115,10 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
   const dim3 grid((unsigned)((g.nx + kBlock - 1) / kBlock),
                   (unsigned)(g.ny < 65535 ? g.ny : 65535));
   const double h1 = 1.0 / dh;
-  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, h1,
-                     s1);
-  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1,
-                     V, g, dt, h1, s2);
-  hipLaunchKernelGGL(ls::rk3_stage_kernel, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2,
-                     V, g, dt, h1, out);
+  auto k = g.nd == 1 ? ls::rk3_stage_kernel<1> : ls::rk3_stage_kernel<2>;
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, h1, s1);
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1, V, g, dt, h1, s2);
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2, V, g, dt, h1, out);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -593,8 +591,9 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     FL_HIP(hipGetLastError());
     // dt from the maximum wavespeed over both grids
     hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
-    for (int m = 0; m < 2; ++m)
-      if ((rc = launch_max_ws(&mats[m].state, &mats[m].scheme, c->ws_tmp, st))) return rc;
+    if ((rc = launch_max_ws2(&mats[0].state, &mats[1].state, &mats[0].scheme, &mats[1].scheme,
+                             c->ws_tmp, st)))
+      return rc;
     FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, st));
     FL_HIP(hipStreamSynchronize(st));
@@ -613,17 +612,18 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     }
     dt = p->cfl * dh / maxws;
     const double dtdh = dt / dh;
-    // each material's split solve
-    for (int m = 0; m < 2; ++m) {
-      fl_field* u = &mats[m].state;
-      fl_field* v = &mats[m].work;
-      if ((rc = launch_sweep(c, u, v, 0, &mats[m].scheme, dtdh, nullptr, nullptr, st))) return rc;
+    // both materials' split solves, each sweep of both grids in one launch
+    {
+      const fl_field* const u[2] = {&mats[0].state, &mats[1].state};
+      const fl_field* const v[2] = {&mats[0].work, &mats[1].work};
+      const fl_scheme* const sc[2] = {&mats[0].scheme, &mats[1].scheme};
+      if ((rc = launch_sweep2(c, u, v, 0, sc, dtdh, st))) return rc;
       if (g.nd > 1) {
-        if ((rc = launch_sweep(c, v, u, 1, &mats[m].scheme, dtdh, nullptr, nullptr, st)))
-          return rc;
+        if ((rc = launch_sweep2(c, v, u, 1, sc, dtdh, st))) return rc;
       } else {
-        FL_HIP(hipMemcpyAsync(u->data, v->data, (size_t)(cells * S * 8), hipMemcpyDeviceToDevice,
-                              st));
+        for (int m = 0; m < 2; ++m)
+          FL_HIP(hipMemcpyAsync(mats[m].state.data, mats[m].work.data, (size_t)(cells * S * 8),
+                                hipMemcpyDeviceToDevice, st));
       }
     }
     // level sets: speeds from the old level sets and new states (both in one

@@ -44,6 +44,24 @@ sweep_y_kernel(SweepArgs<T> a) {
   sweep_y_tile<T, FLUX>(a, bx, by);
 }
 
+// The same sweeps over two grids of one shape in one launch (the two
+// materials of a two-material step): blockIdx.z selects the grid.
+template <typename T, int ND, int FLUX>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
+sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1) {
+  int64_t bx, by;
+  tile_of_block(bx, by);
+  sweep_x_tile<T, ND, FLUX>(blockIdx.z ? a1 : a0, bx, by);
+}
+
+template <typename T, int FLUX>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
+sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1) {
+  int64_t bx, by;
+  tile_of_block(bx, by);
+  sweep_y_tile<T, FLUX>(blockIdx.z ? a1 : a0, bx, by);
+}
+
 // ---------------------------------------------------------------------------
 // standalone wavespeed max, step control
 // ---------------------------------------------------------------------------
@@ -77,6 +95,42 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed_kernel(const T* __res
     load_cell<T, 2 + ND>(in, i, es, cs, u);
     const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u, m64));
     v = b > v ? b : v;
+  }
+  block_atomic_max(v, dst);
+}
+
+// The max over two grids of one shape and layout (the two materials of a
+// two-material step): each thread takes its cells of both.
+template <typename T, int ND>
+__global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
+    const T* __restrict__ in0, const T* __restrict__ in1, int64_t es, int64_t cs,
+    int64_t ncells, Material<double> m0, Material<double> m1, unsigned long long* dst) {
+  constexpr int U = kMaxWsUnroll;
+  unsigned long long v = 0ull;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U / 2 - 1) * stride < ncells; i += (U / 2) * stride) {
+    T u[U][2 + ND];
+#pragma unroll
+    for (int k = 0; k < U / 2; ++k) {
+      load_cell<T, 2 + ND>(in0, i + k * stride, es, cs, u[2 * k]);
+      load_cell<T, 2 + ND>(in1, i + k * stride, es, cs, u[2 * k + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], (k & 1) ? m1 : m0));
+      v = b > v ? b : v;
+    }
+  }
+  for (; i < ncells; i += stride) {
+    T u[2][2 + ND];
+    load_cell<T, 2 + ND>(in0, i, es, cs, u[0]);
+    load_cell<T, 2 + ND>(in1, i, es, cs, u[1]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], k ? m1 : m0));
+      v = b > v ? b : v;
+    }
   }
   block_atomic_max(v, dst);
 }
@@ -158,6 +212,22 @@ __global__ void __launch_bounds__(kMaxBlock) component_sums_kernel(const T* __re
 // ---------------------------------------------------------------------------
 // explicit instantiations (launched from fl_abi.cpp)
 // ---------------------------------------------------------------------------
+
+#define FL_INST_SWEEPS2(T, FLUX)                                                 \
+  template __global__ void sweep_x2_kernel<T, 1, FLUX>(SweepArgs<T>, SweepArgs<T>); \
+  template __global__ void sweep_x2_kernel<T, 2, FLUX>(SweepArgs<T>, SweepArgs<T>); \
+  template __global__ void sweep_y2_kernel<T, FLUX>(SweepArgs<T>, SweepArgs<T>);
+
+FL_INST_SWEEPS2(double, kFluxHllc)
+FL_INST_SWEEPS2(double, kFluxForce)
+FL_INST_SWEEPS2(double, kFluxLaxFriedrichs)
+FL_INST_SWEEPS2(double, kFluxRichtmyer)
+template __global__ void max_wavespeed2_kernel<double, 1>(const double*, const double*, int64_t,
+                                                          int64_t, int64_t, Material<double>,
+                                                          Material<double>, unsigned long long*);
+template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const double*, int64_t,
+                                                          int64_t, int64_t, Material<double>,
+                                                          Material<double>, unsigned long long*);
 
 #define FL_INST_SWEEPS(T)                                                      \
   template __global__ void sweep_x_kernel<T, 1, kFluxHllc>(SweepArgs<T>);      \
