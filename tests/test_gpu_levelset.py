@@ -224,6 +224,33 @@ def test_mm_simulate_bitwise(L, E, oracle, stiff, layout, every):
     _same(p1, R[3])
 
 
+@pytest.mark.parametrize("flux1,lim1,form1", [(1, 0, 0), (0, 1, 1), (0, 1, 0)],
+                         ids=["force-fallback", "superbee-prim", "superbee"])
+def test_mm_simulate_mixed_schemes(L, E, oracle, flux1, lim1, form1):
+    """Materials with different schemes: with equal flux methods both grids
+    share each sweep launch (each with its own limiter, form and EOS); with
+    different ones each grid is swept on its own.  Bit for bit either way."""
+    sb = _bubble(192, 96)
+    os0 = oracle.scheme(gamma=sb.gamma0)
+    os1 = oracle.scheme(gamma=sb.gamma1, flux=flux1, limiter=lim1, form=form1)
+    es0 = E.Scheme(gamma=sb.gamma0)
+    es1 = E.Scheme(gamma=sb.gamma1, flux=flux1, limiter=lim1, form=form1)
+    steps = 6
+    sim = L.MultimaterialSimulator(layout=1)
+    U0, U1 = _dev(sb.U0, 1), _dev(sb.U1, 1)
+    p0, p1 = _dev(sb.phi0), _dev(sb.phi1)
+    sim.add_material(U0, p0, es0)
+    sim.add_material(U1, p1, es1)
+    r = sim.simulate_mm(0.5, sb.dh, max_iters=steps, reinit_every=2)
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
+                           max_iters=steps, reinit_every=2)
+    assert r.iters == R[5] == steps
+    _same(_host(U0, 1), R[0])
+    _same(_host(U1, 1), R[1])
+    _same(_host(p0), R[2])
+    _same(_host(p1), R[3])
+
+
 def test_mm_simulate_full_size(L, E, oracle):
     """The 2048 x 1024 shock-bubble workload: two steps bit for bit against
     the oracle, then more steps checked through size-independent properties
