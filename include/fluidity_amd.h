@@ -403,10 +403,13 @@ typedef struct fl_mm_params {
  * (simulator/multimaterial_simulator.hpp:227-303, 367-483).  Per step: star
  * states of both materials, then extrapolation of each; dt = cfl dh / max
  * wavespeed over both materials' grids; each material's split solve; level
- * set m advected with u . n_m; projection; reinitialisation after every
- * reinit_every-th step (level sets copied back otherwise).  Stops when
- * run_time >= sim_time or iters >= max_iters.  work: fl_mm_work_bytes().
- * Synchronises `stream` every step.  mp may be NULL (defaults). */
+ * set m advected with u . n_m; projection; reinitialisation of both level
+ * sets after every reinit_every-th step.  Each pass covers both materials in
+ * one launch (the sweeps too, when both schemes use the same flux method).
+ * Between reinitialisations the level sets alternate between phi and a work
+ * buffer; they are back in phi on return.  Stops when run_time >= sim_time
+ * or iters >= max_iters.  work: fl_mm_work_bytes().  Synchronises `stream`
+ * every step (and every few eikonal sweeps).  mp may be NULL (defaults). */
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n);
 int fl_mm_simulate(fl_context* ctx, fl_material* mats, int nmat, const fl_sim_params* params,
                    const fl_mm_params* mp, void* work, int64_t work_bytes,
