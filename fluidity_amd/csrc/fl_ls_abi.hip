@@ -30,8 +30,10 @@ int check_shape(int ndim, const int64_t* n, ls::Grid* g) {
   g->ny = ndim > 1 ? n[1] : 1;
   g->nd = ndim;
   g->pad = 0;
-  if (g->nx * g->ny > ((int64_t)1 << 32))
-    return fail(FL_ERR_ARGUMENT, "grid of %lld cells too large", (long long)(g->nx * g->ny));
+  // band lists and active-tile lists hold cell and tile indices as int
+  if (g->nx * g->ny >= ((int64_t)1 << 31))
+    return fail(FL_ERR_ARGUMENT, "grid of %lld cells too large (at most 2^31 - 1)",
+                (long long)(g->nx * g->ny));
   return FL_OK;
 }
 

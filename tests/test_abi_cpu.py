@@ -138,3 +138,18 @@ def test_io_entry_points_reject_bad_arguments(N):
     assert lib.fl_write_results_separate_raw(None, C.byref(f), C.byref(s), 0.0, b"x",
                                              None) == N.FL_ERR_ARGUMENT
     assert b"ctx" in lib.fl_last_error()
+
+
+def test_level_set_grid_size_limits(N):
+    """Level-set grids hold cell indices as int in their band lists: a grid of
+    2^31 cells or more is rejected by the workspace queries (host-only calls),
+    the largest square grid below that is accepted."""
+    import ctypes as C
+    big = (C.c_int64 * 3)(46341, 46341, 1)       # 2 147 488 281 cells >= 2^31
+    ok = (C.c_int64 * 3)(46340, 46340, 1)        # 2 147 395 600 cells
+    for fn in (N.lib.fl_ls_reinit_work_bytes, N.lib.fl_mm_work_bytes,
+               N.lib.fl_gfm_extrapolate_work_bytes):
+        assert fn(2, big) == -1
+        assert fn(2, ok) > 0
+    long1d = (C.c_int64 * 3)((1 << 30) + 1, 1, 1)  # a side above 2^30
+    assert N.lib.fl_ls_reinit_work_bytes(1, long1d) == -1
