@@ -11,12 +11,27 @@
 // q = (gamma+1)/(2 gamma) depend on it; p_inf is a constant).
 // The flattening factor chi is held constant (its derivative is zero
 // wherever it is 1 or clamped, which covers smooth flow).
+//
+// Primal quantities recomputed here (which decide branches) use the same
+// correctly rounded divisions as the forward step.  Derivative terms that
+// divide by one denominator share one reciprocal, recip(): they may differ
+// from a division in the last bits, which the adjoint's tolerance admits.
 #pragma once
 
 #include "fl_numerics.hpp"
 
 namespace fl {
 namespace adj {
+
+// 1/x for the derivative terms: v_rcp_f64 refined by two Newton steps
+// (r' = r + r (1 - x r), each step squaring the relative error), within a
+// few ulps of 1/x for finite nonzero x, which every caller guarantees.
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  return r;
+}
 
 // Adjoint accumulators of one material's parameters.
 struct GMat {
@@ -36,11 +51,12 @@ __device__ __forceinline__ void prim_adj(const double* U, const Prim<double, ND>
   const double t = -0.5 * g_p * m.gm1;
   gU[0] += t * sum;
   const double g_sum = t * U[0];
+  const double inv_rho = recip(U[0]);
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     const double gv = g_vel[i] + 2.0 * g_sum * w.vel[i];
-    gU[2 + i] += gv / U[0];
-    gU[0] -= gv * w.vel[i] / U[0];
+    gU[2 + i] += gv * inv_rho;
+    gU[0] -= gv * w.vel[i] * inv_rho;
   }
 }
 
@@ -49,10 +65,11 @@ __device__ __forceinline__ void sound_adj(double p, double rho, double a,
                                           const Material<double>& m, double g_a, double& g_p,
                                           double& g_rho, GMat& gm) {
   const double pp = m.stiff ? p + m.p_inf : p;
-  const double g_s = g_a / (2.0 * a);
-  gm.gamma += g_s * pp / rho;
-  g_p += g_s * m.gamma / rho;
-  g_rho -= g_s * (m.gamma * pp / rho) / rho;
+  const double g_s = g_a * recip(2.0 * a);
+  const double inv_rho = recip(rho);
+  gm.gamma += g_s * pp * inv_rho;
+  g_p += g_s * m.gamma * inv_rho;
+  g_rho -= g_s * (m.gamma * pp * inv_rho) * inv_rho;
 }
 
 // Euler flux in dimension D from U and its primitives; adds to gU and to the
@@ -105,20 +122,21 @@ __device__ __forceinline__ void limit_adj(int limiter, double L, double R, doubl
       if (inner < Er) {
         dphi_dr = (2.0 < r) ? 0.0 : 1.0;
       } else {
-        dphi_dr = -2.0 / ((1.0 + r) * (1.0 + r));
+        dphi_dr = -2.0 * recip((1.0 + r) * (1.0 + r));
       }
     }
   } else {
     if (r <= 0.0 || R == 0.0) return;
     if (1.0 < r) {
-      dphi_dr = -2.0 / ((1.0 + r) * (1.0 + r));   // 2 / (1 + r)
+      dphi_dr = -2.0 * recip((1.0 + r) * (1.0 + r));   // 2 / (1 + r)
     } else {
-      dphi_dr = 2.0 / ((1.0 + r) * (1.0 + r));    // 2 r / (1 + r)
+      dphi_dr = 2.0 * recip((1.0 + r) * (1.0 + r));    // 2 r / (1 + r)
     }
   }
   const double gr = g * dphi_dr;
-  gL += gr / R;
-  gR -= gr * r / R;
+  const double inv_R = recip(R);
+  gL += gr * inv_R;
+  gR -= gr * r * inv_R;
 }
 
 // Adjoint of evolve_faces<double, ND, D> with conservative-form limiting.
@@ -191,11 +209,12 @@ __device__ __forceinline__ void hllc_wavespeed_adj(double u, double p, double ps
   const double s = 1.0 + q * (t - 1.0);
   const double sr = sqrt(s);
   g_sound += g * sr;
-  const double g_s = g * sound / (2.0 * sr);
+  const double g_s = g * sound * recip(2.0 * sr);
   g_q += g_s * (t - 1.0);
   const double g_t = g_s * q;
-  g_pstar += g_t / p;
-  g_p -= g_t * t / p;
+  const double inv_p = recip(p);
+  g_pstar += g_t * inv_p;
+  g_p -= g_t * t * inv_p;
 }
 
 // Adjoint of hllc_flux<double, ND, D>(ul, ur, m, F) given gF; adds to gl, gr
@@ -273,8 +292,9 @@ __device__ __forceinline__ void hllc_adj(const double* ul, const double* ur,
       const double g_Bt = g_inner * st[0] * A;
       g_wsstar += g_A + g_Bt;
       g_vel[D] -= g_A;
-      g_p += g_Bt / den2;
-      const double g_den2 = -g_Bt * Bq / den2;
+      const double inv_den2 = recip(den2);
+      g_p += g_Bt * inv_den2;
+      const double g_den2 = -g_Bt * Bq * inv_den2;
       g_st[0] += g_den2 * sf;
       g_sf += g_den2 * st[0];
     } else if (k == 2 + D) {
@@ -287,9 +307,11 @@ __device__ __forceinline__ void hllc_adj(const double* ul, const double* ur,
     }
   }
   // scale = sf / (ws - ws_star)
-  g_sf += g_scale / dd;
-  g_ws -= g_scale * scale / dd;
-  g_wsstar += g_scale * scale / dd;
+  const double inv_dd = recip(dd);
+  const double g_dd = g_scale * scale * inv_dd;
+  g_sf += g_scale * inv_dd;
+  g_ws -= g_dd;
+  g_wsstar += g_dd;
   // sf = ws - v
   g_ws += g_sf;
   g_vel[D] -= g_sf;
@@ -314,8 +336,9 @@ __device__ __forceinline__ void hllc_adj(const double* ul, const double* ur,
   }
   double g_wsl = left ? g_ws : 0.0, g_wsr = left ? 0.0 : g_ws;
   // ws_star = num / den
-  const double g_num = g_wsstar / den;
-  const double g_den = -g_wsstar * ws_star / den;
+  const double inv_den = recip(den);
+  const double g_num = g_wsstar * inv_den;
+  const double g_den = -g_wsstar * ws_star * inv_den;
   g_pr += g_num;
   g_pl -= g_num;
   g_vl[D] += g_num * factorl;
