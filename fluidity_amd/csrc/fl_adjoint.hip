@@ -85,27 +85,18 @@ __global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __r
   auto at = [&](const double* A, int64_t xx, int64_t yy) {
     return A[(ND > 1 ? (yy + 4) * px : 0) + xx + 4];
   };
-  // the value at offset -1, 0 or +1 of three (selects, not an indexed array)
-  auto pick = [](int o, double lo, double mid, double hi) { return o < 0 ? lo : (o > 0 ? hi : mid); };
-  double c = DBL_MAX;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
+  auto chi_d = [&](int64_t xx, int64_t yy, int d) {
     const int64_t dx = (d == 0), dy = (d == 1);
     const double* V = d == 0 ? V0 : V1;
-    // the pressures at -3..+3 and velocities at -2..+2 that this cell's factor
-    // and its higher-pressure neighbour's need, loaded before the neighbour is
-    // known (all within the 4-cell padding)
-    double p[7], v[5];
-#pragma unroll
-    for (int k = -3; k <= 3; ++k) p[k + 3] = at(P, x + k * dx, y + k * dy);
-#pragma unroll
-    for (int k = -2; k <= 2; ++k) v[k + 2] = at(V, x + k * dx, y + k * dy);
-    const int o = signum(p[4] - p[2]);
-    const double own = chi_factor<double>(p[1], p[2], p[4], p[5], v[1], v[3]);
-    const double nbr = chi_factor<double>(pick(o, p[0], p[1], p[2]), pick(o, p[1], p[2], p[3]),
-                                          pick(o, p[3], p[4], p[5]), pick(o, p[4], p[5], p[6]),
-                                          pick(o, v[0], v[1], v[2]), pick(o, v[2], v[3], v[4]));
-    c = smin(c, smin(own, nbr));
+    return chi_factor<double>(at(P, xx - 2 * dx, yy - 2 * dy), at(P, xx - dx, yy - dy),
+                              at(P, xx + dx, yy + dy), at(P, xx + 2 * dx, yy + 2 * dy),
+                              at(V, xx - dx, yy - dy), at(V, xx + dx, yy + dy));
+  };
+  double c = DBL_MAX;
+  for (int d = 0; d < ND; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const int sign = signum(at(P, x + dx, y + dy) - at(P, x - dx, y - dy));
+    c = smin(c, smin(chi_d(x, y, d), chi_d(x + sign * dx, y + sign * dy, d)));
   }
   chi[q] = c;
 }
