@@ -842,8 +842,8 @@ __global__ void gfm_band_seed2_kernel(ExtrapJob j0, ExtrapJob j1, double upper,
   bool band0, needed0, band1, needed1;
   band_flags<ND>(j0.phi, upper, g, x, y, i, band0, needed0);
   band_flags<ND>(j1.phi, upper, g, x, y, i, band1, needed1);
-  list_append(band0, i, const_cast<int*>(j0.list), j0.count);
-  list_append(band1, i, const_cast<int*>(j1.list), j1.count);
+  list_append(band0, i, j0.list, j0.count);
+  list_append(band1, i, j1.list, j1.count);
   if (needed0) band_seed_cell<ND>(j0.um, star0, has0, g, x, y, i, j0.A, j0.B);
   if (needed1) band_seed_cell<ND>(j1.um, star1, has1, g, x, y, i, j1.A, j1.B);
 }

@@ -96,7 +96,7 @@ struct ExtrapJob {
   StateView um;
   double* A;
   double* B;
-  const int* list;
+  int* list;  // written by the band seed, read by the sweeps
   int* count;
 };
 constexpr int kExtrapBlocks = 128;  // workgroups per job of one sweep launch

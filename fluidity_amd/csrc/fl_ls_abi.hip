@@ -255,7 +255,7 @@ ls::ExtrapJob extrap_job(const double* phi_m, ls::StateView um, const ls::Grid& 
   j.um = um;
   j.A = (double*)w;
   j.B = (double*)(w + bb);
-  j.list = (const int*)(w + 2 * bb);
+  j.list = (int*)(w + 2 * bb);
   j.count = (int*)(w + 2 * bb + align256(cells * 4));
   return j;
 }
@@ -275,10 +275,10 @@ void band_seed(const ls::ExtrapJob& j, const double* star, const uint8_t* has,
   const dim3 grid = grid_for(g.nx * g.ny);
   if (g.nd == 1)
     hipLaunchKernelGGL(ls::gfm_band_seed_kernel<1>, grid, dim3(kBlock), 0, st, j.phi,
-                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, (int*)j.list, j.count);
+                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, j.list, j.count);
   else
     hipLaunchKernelGGL(ls::gfm_band_seed_kernel<2>, grid, dim3(kBlock), 0, st, j.phi,
-                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, (int*)j.list, j.count);
+                       extrap_upper(dh), j.um, star, has, g, j.A, j.B, j.list, j.count);
 }
 
 // Star states of both materials (each from the same level sets and states)
