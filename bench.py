@@ -204,6 +204,7 @@ def main():
             del s2, a2, w2
             torch.cuda.empty_cache()
 
+        variants["riemann_1d_f64"] = riemann_1d_case(torch, N, W, stream)
         variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
         variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
         variants["io_f64"] = io_case(torch, N, W, stream)
@@ -215,6 +216,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, cfl, args.cpu_steps)
+        if not args.no_variants:
+            cpu["riemann_1d_f64"] = cpu_riemann_1d(W)
 
     line = {
         "metric": "cell-updates/s, 2D 4096^2 split MUSCL-Hancock+HLLC FP64",
@@ -237,6 +240,56 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def riemann_1d_case(torch, N, W, stream, runs=3):
+    """The 1D Riemann workload (4096 cells, seed 1000, HLLC + van Leer, CFL
+    0.9) run until the fastest wave has travelled from x = 0.5 to 0.9, as a
+    whole: the device step loop with graph-replayed steps, timed over `runs`
+    runs after one untimed run.  On 4096 cells a step is a few microseconds
+    of launches, so this measures the step loop's launch overhead more than
+    the sweep."""
+    from fluidity_amd import euler as E
+    r1 = W.riemann_1d()
+    dev = torch.device("cuda:0")
+    solver = E.SplitSolver(E.Scheme())
+    with torch.cuda.stream(stream):
+        a0 = torch.from_numpy(r1.U).to(dev)
+        a = a0.clone()
+        w = torch.empty_like(a)
+    stream.synchronize()
+    solver.simulate(a, w, 0.9, r1.dh, sim_time=r1.t_end, stream=stream)   # untimed
+    times, iters = [], 0
+    for _ in range(runs):
+        with torch.cuda.stream(stream):
+            a.copy_(a0)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        res = solver.simulate(a, w, 0.9, r1.dh, sim_time=r1.t_end, stream=stream)
+        stream.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters = res.iters
+    el = min(times)
+    n = r1.U.shape[0]
+    return {"value": round(iters * n / el, 1), "unit": "cell-updates/s",
+            "run_ms": round(el * 1e3, 3), "steps": iters, "us_per_step": round(el * 1e6 / iters, 2),
+            "workload": f"1D Euler {n} cells, seed 1000, HLLC + van Leer, CFL 0.9, until the "
+                        "fastest wave has travelled 0.4 (x = 0.5 -> 0.9), FP64"}
+
+
+def cpu_riemann_1d(W):
+    """The same 1D run on the CPU oracle, single-threaded, as the reference's
+    CPU path runs it."""
+    from oracle import oracle as O
+    r1 = W.riemann_1d()
+    t0 = time.perf_counter()
+    _, _, it, _ = O.simulate(r1.U, O.scheme(flux=O.FLUX_HLLC), 0.9, r1.dh, sim_time=r1.t_end,
+                             max_iters=1000000, nthreads=1)
+    el = time.perf_counter() - t0
+    n = r1.U.shape[0]
+    return {"value": round(it * n / el, 1), "unit": "cell-updates/s", "cores": 1,
+            "kind": "port", "run_ms": round(el * 1e3, 1), "steps": it,
+            "sample": f"the whole 1D run ({it} steps of {n} cells)"}
 
 
 def adjoint_case(torch, N, W, stream, steps=200, every=20, n=1024):
