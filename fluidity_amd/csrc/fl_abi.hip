@@ -293,12 +293,13 @@ int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_f
   return FL_OK;
 }
 
-// The max wavespeed over two FP64 grids of one shape and layout, max-ed into
-// dst (not cleared here).
+// The max wavespeed over two FP64 grids of one shape (each in its own
+// layout), max-ed into dst (not cleared here).
 int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
                    const fl_scheme* s1, unsigned long long* dst, hipStream_t stream) {
-  int64_t es, cs;
-  strides(f0, es, cs);
+  int64_t es0, cs0, es1, cs1;
+  strides(f0, es0, cs0);
+  strides(f1, es1, cs1);
   const int64_t n = cell_count(f0);
   const int B = fl::kMaxBlock;
   int64_t blocks = (n + B - 1) / B;
@@ -307,12 +308,12 @@ int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
   const fl::Material<double> m0 = make_material<double>(s0), m1 = make_material<double>(s1);
   if (f0->ndim == 1)
     hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 1>), dim3((unsigned)blocks), dim3(B), 0,
-                       stream, (const double*)f0->data, (const double*)f1->data, es, cs, n, m0,
-                       m1, dst);
+                       stream, (const double*)f0->data, (const double*)f1->data, es0, cs0, es1,
+                       cs1, n, m0, m1, dst);
   else
     hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 2>), dim3((unsigned)blocks), dim3(B), 0,
-                       stream, (const double*)f0->data, (const double*)f1->data, es, cs, n, m0,
-                       m1, dst);
+                       stream, (const double*)f0->data, (const double*)f1->data, es0, cs0, es1,
+                       cs1, n, m0, m1, dst);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

@@ -61,8 +61,9 @@ template <typename T, int ND, int FLUX>
 __global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int FLUX> __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int ND>
-__global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es, int64_t cs,
-                                      int64_t ncells, Material<double> m0, Material<double> m1,
+__global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es0, int64_t cs0,
+                                      int64_t es1, int64_t cs1, int64_t ncells,
+                                      Material<double> m0, Material<double> m1,
                                       unsigned long long* dst);
 template <typename T, int ND>
 __global__ void max_wavespeed_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,

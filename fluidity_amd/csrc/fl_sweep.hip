@@ -99,12 +99,14 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed_kernel(const T* __res
   block_atomic_max(v, dst);
 }
 
-// The max over two grids of one shape and layout (the two materials of a
-// two-material step): each thread takes its cells of both.
+// The max over two grids of one shape (the two materials of a two-material
+// step; each grid with its own element and component strides): each thread
+// takes its cells of both.
 template <typename T, int ND>
 __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
-    const T* __restrict__ in0, const T* __restrict__ in1, int64_t es, int64_t cs,
-    int64_t ncells, Material<double> m0, Material<double> m1, unsigned long long* dst) {
+    const T* __restrict__ in0, const T* __restrict__ in1, int64_t es0, int64_t cs0, int64_t es1,
+    int64_t cs1, int64_t ncells, Material<double> m0, Material<double> m1,
+    unsigned long long* dst) {
   constexpr int U = kMaxWsUnroll;
   unsigned long long v = 0ull;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -113,8 +115,8 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
     T u[U][2 + ND];
 #pragma unroll
     for (int k = 0; k < U / 2; ++k) {
-      load_cell<T, 2 + ND>(in0, i + k * stride, es, cs, u[2 * k]);
-      load_cell<T, 2 + ND>(in1, i + k * stride, es, cs, u[2 * k + 1]);
+      load_cell<T, 2 + ND>(in0, i + k * stride, es0, cs0, u[2 * k]);
+      load_cell<T, 2 + ND>(in1, i + k * stride, es1, cs1, u[2 * k + 1]);
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -124,8 +126,8 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
   }
   for (; i < ncells; i += stride) {
     T u[2][2 + ND];
-    load_cell<T, 2 + ND>(in0, i, es, cs, u[0]);
-    load_cell<T, 2 + ND>(in1, i, es, cs, u[1]);
+    load_cell<T, 2 + ND>(in0, i, es0, cs0, u[0]);
+    load_cell<T, 2 + ND>(in1, i, es1, cs1, u[1]);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], k ? m1 : m0));
@@ -223,11 +225,13 @@ FL_INST_SWEEPS2(double, kFluxForce)
 FL_INST_SWEEPS2(double, kFluxLaxFriedrichs)
 FL_INST_SWEEPS2(double, kFluxRichtmyer)
 template __global__ void max_wavespeed2_kernel<double, 1>(const double*, const double*, int64_t,
-                                                          int64_t, int64_t, Material<double>,
-                                                          Material<double>, unsigned long long*);
+                                                          int64_t, int64_t, int64_t, int64_t,
+                                                          Material<double>, Material<double>,
+                                                          unsigned long long*);
 template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const double*, int64_t,
-                                                          int64_t, int64_t, Material<double>,
-                                                          Material<double>, unsigned long long*);
+                                                          int64_t, int64_t, int64_t, int64_t,
+                                                          Material<double>, Material<double>,
+                                                          unsigned long long*);
 
 #define FL_INST_SWEEPS(T)                                                      \
   template __global__ void sweep_x_kernel<T, 1, kFluxHllc>(SweepArgs<T>);      \

@@ -251,6 +251,44 @@ def test_mm_simulate_mixed_schemes(L, E, oracle, flux1, lim1, form1):
     _same(_host(p1), R[3])
 
 
+def test_mm_simulate_mixed_layouts(L, E, oracle):
+    """fl_mm_simulate through the C ABI with material 0 in AoS and material 1
+    in SoA (each grid keeps its own strides in every joint launch)."""
+    import ctypes as C
+    from fluidity_amd import native as N
+    from fluidity_amd.euler import grid_field
+    sb = _bubble(160, 96)
+    os0, os1, es0, es1 = _schemes(oracle, E, sb)
+    steps = 5
+    U0, U1 = _dev(sb.U0, 0), _dev(sb.U1, 1)
+    W0, W1 = torch.empty_like(U0), torch.empty_like(U1)
+    p0, p1 = _dev(sb.phi0), _dev(sb.phi1)
+    mats = (N.Material * 2)()
+    for m, (U, Wk, p, sc, lay) in enumerate([(U0, W0, p0, es0, N.FL_LAYOUT_AOS),
+                                            (U1, W1, p1, es1, N.FL_LAYOUT_SOA)]):
+        mats[m].state = grid_field(U, lay)
+        mats[m].work = grid_field(Wk, lay)
+        mats[m].phi = p.data_ptr()
+        mats[m].scheme = sc.c()
+    n = (C.c_int64 * 2)(sb.phi0.shape[1], sb.phi0.shape[0])
+    wbytes = N.lib.fl_mm_work_bytes(2, n)
+    work = torch.empty(wbytes, dtype=torch.uint8, device="cuda")
+    ctx = N.Context(0)
+    p = N.SimParams(0.5, sb.dh, float("inf"), steps)
+    mp = N.MmParams(6, 8, 2, 0)
+    r = N.SimResult()
+    N.check(N.lib.fl_mm_simulate(ctx.handle, mats, 2, C.byref(p), C.byref(mp), work.data_ptr(),
+                                 wbytes, C.byref(r), None))
+    torch.cuda.synchronize()
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
+                           max_iters=steps, reinit_every=2)
+    assert r.iters == R[5] == steps
+    _same(_host(U0, 0), R[0])
+    _same(_host(U1, 1), R[1])
+    _same(_host(p0), R[2])
+    _same(_host(p1), R[3])
+
+
 def test_mm_simulate_full_size(L, E, oracle):
     """The 2048 x 1024 shock-bubble workload: two steps bit for bit against
     the oracle, then more steps checked through size-independent properties
