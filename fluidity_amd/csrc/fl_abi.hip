@@ -418,6 +418,8 @@ int fl_ctx_destroy(fl_context* c) {
   if (c->host_partial) (void)hipHostFree(c->host_partial);
   if (c->sim_exec) (void)hipGraphExecDestroy(c->sim_exec);
   if (c->sim_graph) (void)hipGraphDestroy(c->sim_graph);
+  if (c->sim_exec_run) (void)hipGraphExecDestroy(c->sim_exec_run);
+  if (c->sim_graph_run) (void)hipGraphDestroy(c->sim_graph_run);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->eik) (void)hipFree(c->eik);
   if (c->host_eik) (void)hipHostFree(c->host_eik);
@@ -622,10 +624,14 @@ int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme*
   if (rc) return rc;
 
   // Steps are launched in batches; between batches the host reads the control
-  // block once to stop when the loop condition fails.  With graphs on, a pair
-  // of steps (both parities) is captured once and replayed; a step past the
-  // stop condition does nothing, so an odd batch may run one step more.
+  // block once to stop when the loop condition fails.  With graphs on, runs
+  // of kSimGraphSteps steps and pairs of steps (both parities) are captured
+  // once and replayed, the runs while a batch has that many steps left (one
+  // host launch per run instead of per pair: small grids' steps are shorter
+  // than a graph launch); a step past the stop condition does nothing, so an
+  // odd batch may run one step more.
   const int64_t kBatch = 256;
+  constexpr int kSimGraphSteps = 16;
   int64_t launched = 0;
   fl_field a = *state, b = *work;
   auto one_step = [&](int parity, hipStream_t sx) -> int {
@@ -656,31 +662,42 @@ int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme*
     key.sim_time = p->sim_time;
     key.max_iters = p->max_iters;
     key.tuning = c->tuning;
-    if (!c->sim_exec || memcmp(&key, &c->sim_key, sizeof(key)) != 0) {
+    if (!c->sim_exec || !c->sim_exec_run || memcmp(&key, &c->sim_key, sizeof(key)) != 0) {
       if (c->sim_exec) (void)hipGraphExecDestroy(c->sim_exec);
       if (c->sim_graph) (void)hipGraphDestroy(c->sim_graph);
-      c->sim_exec = nullptr;
-      c->sim_graph = nullptr;
-      FL_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-      rc = one_step(0, c->cap_stream);
-      if (!rc) rc = one_step(1, c->cap_stream);
-      hipGraph_t gr = nullptr;
-      const hipError_t e = hipStreamEndCapture(c->cap_stream, &gr);
-      if (rc) {
-        if (gr) (void)hipGraphDestroy(gr);
-        return rc;
-      }
-      if (e != hipSuccess)
-        return fail(FL_ERR_HIP, "step graph capture failed: %s", hipGetErrorString(e));
-      c->sim_graph = gr;
-      FL_HIP(hipGraphInstantiate(&c->sim_exec, c->sim_graph, nullptr, nullptr, 0));
+      if (c->sim_exec_run) (void)hipGraphExecDestroy(c->sim_exec_run);
+      if (c->sim_graph_run) (void)hipGraphDestroy(c->sim_graph_run);
+      c->sim_exec = c->sim_exec_run = nullptr;
+      c->sim_graph = c->sim_graph_run = nullptr;
+      // captures `steps` steps (even) into *graph and instantiates *exec
+      auto capture = [&](int steps, hipGraph_t* graph, hipGraphExec_t* exec) -> int {
+        FL_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+        int r = FL_OK;
+        for (int k = 0; k < steps && !r; ++k) r = one_step(k & 1, c->cap_stream);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->cap_stream, &gr);
+        if (r) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return r;
+        }
+        if (e != hipSuccess)
+          return fail(FL_ERR_HIP, "step graph capture failed: %s", hipGetErrorString(e));
+        *graph = gr;
+        FL_HIP(hipGraphInstantiate(exec, gr, nullptr, nullptr, 0));
+        return FL_OK;
+      };
+      if ((rc = capture(2, &c->sim_graph, &c->sim_exec))) return rc;
+      if ((rc = capture(kSimGraphSteps, &c->sim_graph_run, &c->sim_exec_run))) return rc;
       c->sim_key = key;
     }
   }
   while (launched < p->max_iters) {
     const int64_t nb = (p->max_iters - launched) < kBatch ? (p->max_iters - launched) : kBatch;
     if (c->use_graphs) {
-      for (int64_t k = 0; k < nb; k += 2) FL_HIP(hipGraphLaunch(c->sim_exec, st));
+      int64_t k = 0;
+      for (; k + kSimGraphSteps <= nb; k += kSimGraphSteps)
+        FL_HIP(hipGraphLaunch(c->sim_exec_run, st));
+      for (; k < nb; k += 2) FL_HIP(hipGraphLaunch(c->sim_exec, st));
     } else {
       for (int64_t k = 0; k < nb; ++k) {
         rc = one_step((int)((launched + k) & 1), st);

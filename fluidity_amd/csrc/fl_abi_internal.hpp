@@ -35,8 +35,10 @@ struct fl_context {
   fl_tuning tuning;
   int use_graphs;              // fl_simulate replays a captured pair of steps
   hipStream_t cap_stream;      // capture stream of that graph
-  hipGraph_t sim_graph;
+  hipGraph_t sim_graph;        // a pair of steps (both parities)
   hipGraphExec_t sim_exec;
+  hipGraph_t sim_graph_run;    // kSimGraphSteps steps
+  hipGraphExec_t sim_exec_run;
   SimGraphKey sim_key;
 };
 
