@@ -113,10 +113,11 @@ int fl_ctx_destroy(fl_context* ctx);
 int fl_ctx_set_tuning(fl_context* ctx, const fl_tuning* tuning);
 int fl_ctx_get_tuning(const fl_context* ctx, fl_tuning* tuning);
 /* Launch-shape autotuner: times every candidate (block width 64/128/256 x
- * rows per workgroup) of each sweep on this grid (in -> work, a fixed small
- * dt/dh; `in` is not modified), keeps the fastest in the context and returns
- * it.  Synchronises.  Replaces the fixed thread_sizes()/block_sizes()
- * (solver/split_solver.hpp:117-125). */
+ * rows per workgroup: 8..128 and the heights that make a launch fill whole
+ * rounds of the workgroups the device holds at once) of each sweep on this
+ * grid (in -> work, a fixed small dt/dh; `in` is not modified), keeps the
+ * fastest in the context and returns it.  Synchronises.  Replaces the fixed
+ * thread_sizes()/block_sizes() (solver/split_solver.hpp:117-125). */
 int fl_ctx_autotune(fl_context* ctx, const fl_field* in, const fl_field* work,
                     const fl_scheme* scheme, int reps, fl_tuning* chosen, void* stream);
 
