@@ -253,23 +253,24 @@ def test_simulate_graph_replay_equals_launches(E, nd, steps):
 
 def test_headline_workload_full_size_bitwise(E, oracle):
     """The 4096^2 FP64 HLLC four-quadrant workload itself (seed 1001), after
-    the launch-shape autotuner: 3 steps of the device loop equal 3 steps of
+    the launch-shape autotuner: 12 steps of the device loop equal 12 steps of
     the oracle (16 host threads) bit for bit, with the same run time."""
     import os
     from fluidity_amd import workloads as W
     wl = W.riemann_2d(n=4096, seed=1001)
     os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
-    ref, rt, it, _ = oracle.simulate(wl.U, os_, 0.9, wl.dh, max_iters=3,
+    steps = 12
+    ref, rt, it, _ = oracle.simulate(wl.U, os_, 0.9, wl.dh, max_iters=steps,
                                      nthreads=min(os.cpu_count() or 1, 16))
     solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
     a = _dev(wl.U, layout=1)
     w = torch.empty_like(a)
     solver.autotune(a, w, reps=1)
-    r = solver.simulate(a, w, 0.9, wl.dh, max_iters=3)
+    r = solver.simulate(a, w, 0.9, wl.dh, max_iters=steps)
     got = _host(a, 1)
     del a, w
     torch.cuda.empty_cache()
-    assert r.iters == it == 3 and r.run_time == rt
+    assert r.iters == it == steps and r.run_time == rt
     assert _bitwise(got, ref), _report(got, ref)
 
 
