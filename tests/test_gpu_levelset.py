@@ -290,15 +290,16 @@ def test_mm_simulate_mixed_layouts(L, E, oracle):
 
 
 def test_mm_simulate_full_size(L, E, oracle):
-    """The 2048 x 1024 shock-bubble workload: two steps bit for bit against
+    """The 2048 x 1024 shock-bubble workload: six steps bit for bit against
     the oracle, then more steps checked through size-independent properties
     (finite states, phi1 == -phi0 exactly after projection + reinitialisation,
     the bubble's area conserved to first order)."""
     sb = _bubble(2048, 1024)
     os0, os1, _, _ = _schemes(oracle, E, sb)
-    r, U0, U1, p0, p1 = _run_mm(L, E, sb, 2)
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, 6)
     R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
-                           max_iters=2, nthreads=16)
+                           max_iters=6, nthreads=16)
+    assert r.iters == R[5] == 6
     _same(U0, R[0])
     _same(U1, R[1])
     _same(p0, R[2])
