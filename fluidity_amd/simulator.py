@@ -296,7 +296,7 @@ class MultimaterialSimulator(_Base):
     """Two materials separated by level sets: MultimaterialSimulator
     (simulator/multimaterial_simulator.hpp) on the device ghost-fluid loop
     (levelset.MultimaterialSimulator).  The level-set band, extrapolation
-    sweeps and reinitialisation interval are this build's extensions."""
+    sweeps and reinitialisation interval are this library's extensions."""
 
     def __init__(self, dimensions: int = 1, scheme: Scheme | None = None,
                  layout: int = N.FL_LAYOUT_SOA, device: int = 0, band: int = 6,

@@ -24,7 +24,7 @@ extern "C" {
 typedef enum fl_status {
   FL_OK = 0,
   FL_ERR_ARGUMENT = 1,     /* null pointer, bad size, bad enum, dtdh <= 0 ...  */
-  FL_ERR_UNSUPPORTED = 2,  /* a valid combination this build does not run      */
+  FL_ERR_UNSUPPORTED = 2,  /* a valid combination this library does not run    */
   FL_ERR_HIP = 3,          /* the HIP runtime reported an error                 */
   FL_ERR_NUMERIC = 4       /* NaN or non-positive maximum wavespeed (vacuum)    */
 } fl_status;

@@ -16,8 +16,8 @@ reference.  Sources (paths relative to the reference root):
                    IdealGas<double>>::flux_delta evaluated by the compiled
                    reference headers on the host at a Sod interface: cells
                    0..7 = (rho, p, u) (1, 1, 0), cells 8..15 = (0.125, 0.1, 0),
-                   dtdh = 0.1, cell 7.  Values as recorded in the survey of the
-                   reference (printed with 17 significant digits).
+                   dtdh = 0.1, cell 7.  Values as printed by that evaluation
+                   (17 significant digits).
  toro_cases.json   validation/1d_toro_case_{1..5}_validation.cu -- the five
                    Toro shock-tube set-ups (membrane, left/right states, end
                    time, resolution, CFL).
