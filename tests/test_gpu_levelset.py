@@ -289,6 +289,37 @@ def test_mm_simulate_mixed_layouts(L, E, oracle):
     _same(_host(p1), R[3])
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_mm_simulate_1d_bitwise(L, E, oracle, layout):
+    """The two-material loop on a 1D shock tube (every pass in its 1D form:
+    joint sweeps and maximum, star states, band seeds, RK3, eikonal), 12
+    steps with reinitialisation every 3, bit for bit against the oracle."""
+    n = 300
+    dh = 1.0 / n
+    x = (np.arange(n) + 0.5) * dh
+    phi0 = np.ascontiguousarray(x - 0.4137)
+    phi1 = -phi0
+    U0 = W.prim_to_cons(np.broadcast_to(np.array([1.0, 2.0, 0.3]), (n, 3)).copy(), 1.4)
+    U1 = W.prim_to_cons(np.broadcast_to(np.array([0.2, 0.5, -0.2]), (n, 3)).copy(), 1.67)
+    os0, os1 = oracle.scheme(gamma=1.4), oracle.scheme(gamma=1.67)
+    es0, es1 = E.Scheme(gamma=1.4), E.Scheme(gamma=1.67)
+    sim = L.MultimaterialSimulator(layout=layout)
+    d0, d1 = _dev(U0, layout), _dev(U1, layout)
+    p0, p1 = _dev(phi0), _dev(phi1)
+    sim.add_material(d0, p0, es0)
+    sim.add_material(d1, p1, es1)
+    steps = 12
+    r = sim.simulate_mm(0.5, dh, max_iters=steps, reinit_every=3)
+    R = oracle.mm_simulate(U0, U1, phi0, phi1, os0, os1, 0.5, dh, max_iters=steps,
+                           reinit_every=3)
+    assert r.iters == R[5] == steps
+    assert r.run_time == R[4]
+    _same(_host(d0, layout), R[0])
+    _same(_host(d1, layout), R[1])
+    _same(_host(p0), R[2])
+    _same(_host(p1), R[3])
+
+
 def test_mm_simulate_full_size(L, E, oracle):
     """The 2048 x 1024 shock-bubble workload: six steps bit for bit against
     the oracle, then more steps checked through size-independent properties
