@@ -408,8 +408,12 @@ typedef struct fl_mm_params {
  * sets after every reinit_every-th step.  Each pass covers both materials in
  * one launch (the sweeps too, when both schemes use the same flux method).
  * Between reinitialisations the level sets alternate between phi and a work
- * buffer; they are back in phi on return.  Stops when run_time >= sim_time
- * or iters >= max_iters.  work: fl_mm_work_bytes().  Synchronises `stream`
+ * buffer; they are back in phi on FL_OK and on the FL_ERR_NUMERIC of a NaN or
+ * vacuum wavespeed (the fields then hold the last completed step, with the
+ * ghost cells the failed step had written, as the reference's loop would).  After
+ * other errors (a HIP failure, an eikonal solve that does not converge) the
+ * states and level sets are unspecified.  Stops when run_time >= sim_time or
+ * iters >= max_iters.  work: fl_mm_work_bytes().  Synchronises `stream`
  * every step (and every few eikonal sweeps).  mp may be NULL (defaults). */
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n);
 int fl_mm_simulate(fl_context* ctx, fl_material* mats, int nmat, const fl_sim_params* params,
