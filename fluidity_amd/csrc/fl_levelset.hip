@@ -47,12 +47,17 @@ __device__ __forceinline__ double phi_at(const double* __restrict__ phi, const G
   return phi[yi * g.nx + xi];
 }
 
-// Cell index of the mirrored cell (x, y); ND = g.nd.
+// phi at (x, y) for x, y at most one cell outside the grid: the mirror of
+// -1 is 0 and of n is n - 1, so the mirror of such an index is a clamp.
+__device__ __forceinline__ int64_t clamp1(int64_t i, int64_t n) {
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
 template <int ND>
-__device__ __forceinline__ int64_t mirror_index(const Grid& g, int64_t x, int64_t y) {
-  const int64_t xi = mirror(x, g.nx);
-  const int64_t yi = ND > 1 ? mirror(y, g.ny) : 0;
-  return yi * g.nx + xi;
+__device__ __forceinline__ double phi_at1(const double* __restrict__ phi, const Grid& g,
+                                          int64_t x, int64_t y) {
+  const int64_t yi = ND > 1 ? clamp1(y, g.ny) : 0;
+  return phi[yi * g.nx + clamp1(x, g.nx)];
 }
 
 // levelset.hpp:166-217 as mask bits.
@@ -76,7 +81,7 @@ __device__ __forceinline__ void norm_at(const double* __restrict__ phi, const Gr
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
-    nrm[d] = fact * (phi_at(phi, g, x + dx, y + dy) - phi_at(phi, g, x - dx, y - dy));
+    nrm[d] = fact * (phi_at1<ND>(phi, g, x + dx, y + dy) - phi_at1<ND>(phi, g, x - dx, y - dy));
     mag += nrm[d] * nrm[d];
   }
   const double s = sqrt(mag);
@@ -90,6 +95,14 @@ __device__ __forceinline__ void load_state(const StateView& v, const Grid& g, in
   const int64_t xi = mirror(x, g.nx);
   const int64_t yi = g.nd > 1 ? mirror(y, g.ny) : 0;
   const double* p = v.data + (yi * g.nx + xi) * v.es;
+#pragma unroll
+  for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
+}
+
+// The state of cell i of the grid (no ghost mapping).
+template <int S>
+__device__ __forceinline__ void load_state_i(const StateView& v, int64_t i, double* u) {
+  const double* p = v.data + i * v.es;
 #pragma unroll
   for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
 }
@@ -133,7 +146,7 @@ __global__ void velocity_kernel(const double* __restrict__ phi0, const double* _
       double nrm[2];
       norm_at<ND>(target < 0 ? phi : (target == 0 ? phi0 : phi1), g, x, y, dh, nrm);
       double u[2 + ND];
-      load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
+      load_state_i<2 + ND>(m == 0 ? u0 : u1, i, u);
       double sum = 0.0;
 #pragma unroll
       for (int d = 0; d < ND; ++d) sum += (u[2 + d] / u[0]) * nrm[d];
@@ -169,7 +182,7 @@ __global__ void velocity2_kernel(const double* __restrict__ phi0, const double* 
     norm_at<ND>(phi0, g, x, y, dh, n0);
     norm_at<ND>(phi1, g, x, y, dh, n1);
     double u[2 + ND];
-    load_state<2 + ND>(m == 0 ? u0 : u1, g, x, y, u);
+    load_state_i<2 + ND>(m == 0 ? u0 : u1, i, u);
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -657,7 +670,8 @@ __device__ __forceinline__ void star_cell(const double* __restrict__ phi_m, doub
   constexpr int S = 2 + ND;
   for (int d = 0; d < ND; ++d) {
     const int64_t dx = (d == 0), dy = (d == 1);
-    if (phi_at(phi_m, g, x - dx, y - dy) == 0.0 || phi_at(phi_m, g, x + dx, y + dy) == 0.0) return;
+    if (phi_at1<ND>(phi_m, g, x - dx, y - dy) == 0.0 || phi_at1<ND>(phi_m, g, x + dx, y + dy) == 0.0)
+      return;
   }
   const uint8_t mo = mask_value(po, dh);
   if (!((mo & 16) || (mo & 4))) return;
@@ -876,15 +890,16 @@ __device__ __forceinline__ void extrap_cell(const double* __restrict__ phi_m,
     const int64_t dx = (d == 0), dy = (d == 1);
     // both neighbours' states are loaded with the level-set values; the
     // upwind one is taken
-    const double* lo = A + (mirror_index<ND>(g, x - dx, y - dy)) * S;
-    const double* hi = A + (mirror_index<ND>(g, x + dx, y + dy)) * S;
+    const int64_t ylo = ND > 1 ? clamp1(y - dy, g.ny) : 0, yhi = ND > 1 ? clamp1(y + dy, g.ny) : 0;
+    const double* lo = A + (ylo * g.nx + clamp1(x - dx, g.nx)) * S;
+    const double* hi = A + (yhi * g.nx + clamp1(x + dx, g.nx)) * S;
     double ul[S], uh[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       ul[k] = lo[k];
       uh[k] = hi[k];
     }
-    const bool down = phi_at(phi_m, g, x - dx, y - dy) < phi_at(phi_m, g, x + dx, y + dy);
+    const bool down = phi_at1<ND>(phi_m, g, x - dx, y - dy) < phi_at1<ND>(phi_m, g, x + dx, y + dy);
     const double wgt = l1 > 1e-2 ? fabs(nrm[d]) : 0.5;
 #pragma unroll
     for (int k = 0; k < S; ++k) acc[k] += (down ? ul[k] : uh[k]) * wgt;
