@@ -234,10 +234,12 @@ def test_vacuum_reports_numeric_error(E, N):
     assert ei.value.status == N.FL_ERR_NUMERIC
 
 
-@pytest.mark.parametrize("nd,steps", [(1, 37), (2, 13)])
+@pytest.mark.parametrize("nd,steps", [(1, 37), (2, 13), (2, 45), (1, 300)])
 def test_simulate_graph_replay_equals_launches(E, nd, steps):
-    """The hipGraph replay of step pairs gives the same state, step count and
-    run time as launching every kernel, also for an odd number of steps."""
+    """The hipGraph replay (16-step runs while a batch has 16 steps left, then
+    step pairs) gives the same state, step count and run time as launching
+    every kernel: odd counts, counts that mix both graphs, and a count past
+    one 256-step batch."""
     U = cons_field("smooth", (97,) if nd == 1 else (40, 52), 0)
     res = []
     for graphs in (True, False):
