@@ -3,8 +3,9 @@
 // Device functions for fluidity's MUSCL-Hancock / HLLC / FORCE finite-volume
 // step.  For T = double every expression keeps the operation order of the
 // reference header cited beside it; the library is compiled with
-// -ffp-contract=off, and f64 division and square root on gfx950 are correctly
-// rounded, so the FP64 path reproduces the reference bit for bit.  For
+// -ffp-contract=off, and f64 division (for operands away from the ends of the
+// exponent range, see dv) and square root are correctly rounded, so the FP64
+// path reproduces the reference bit for bit.  For
 // T = float the same formulas run in single precision (an extension: the
 // reference templates do not compile for float).
 //
@@ -38,11 +39,28 @@ template <typename T> __device__ __forceinline__ int signum(T x) {
   return (T(0) < x) - (x < T(0));
 }
 
-// Division and square root.  FP64: IEEE correctly rounded (the hardware
-// div_scale/div_fmas/div_fixup sequence and the Newton-refined rsq), which is
-// what bit parity with the reference needs.  FP32 (bulk-precision extension):
-// v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far inside the FP32 error budget.
-__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+// Division and square root.  FP64 division: the reciprocal of b refined by
+// two Newton steps, the quotient a*r corrected by one residual step, then
+// v_div_fixup for zero, infinite and NaN operands.  That is the compiler's
+// IEEE a / b sequence without its operand rescaling (v_div_scale and
+// v_div_fmas), which only acts when an operand or the quotient is near the
+// ends of the exponent range (b denormal or |b| >= 2^1021, a nonzero with
+// |a| < 2^-969, a quotient that would be denormal, or exponents of a and b
+// 768 or more apart).  Outside those cases the result is the correctly
+// rounded quotient, bit for bit what a / b gives, and since the refined
+// reciprocal depends on b alone, quotients with one denominator (m_x/rho,
+// m_y/rho, gamma p/rho) share it.  FP64 square root: IEEE correctly rounded.
+// FP32 (bulk-precision extension): v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far
+// inside the FP32 error budget.
+__device__ __forceinline__ double dv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  return __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-b, q, a), r, q), b, a);
+}
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
