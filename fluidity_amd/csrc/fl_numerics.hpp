@@ -49,8 +49,12 @@ template <typename T> __device__ __forceinline__ int signum(T x) {
 // 768 or more apart).  Outside those cases the result is the correctly
 // rounded quotient, bit for bit what a / b gives, and since the refined
 // reciprocal depends on b alone, quotients with one denominator (m_x/rho,
-// m_y/rho, gamma p/rho) share it.  FP64 square root: IEEE correctly rounded.
-// FP32 (bulk-precision extension): v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far
+// m_y/rho, gamma p/rho) share it.  FP64 square root: the compiler's IEEE
+// sqrt sequence (v_rsq_f64, one Goldschmidt step for root and half-inverse
+// root, two residual corrections, x itself for +-0 and +inf) without its
+// rescaling of arguments below 2^-767; for every other argument, negative,
+// infinite and NaN ones included, it is the same computation and gives the
+// correctly rounded root.  FP32 (bulk-precision extension): v_rcp_f32 / v_sqrt_f32 (about 1 ulp), far
 // inside the FP32 error budget.
 __device__ __forceinline__ double dv(double a, double b) {
   double r = __builtin_amdgcn_rcp(b);
@@ -62,7 +66,19 @@ __device__ __forceinline__ double dv(double a, double b) {
   return __builtin_amdgcn_div_fixup(__builtin_fma(__builtin_fma(-b, q, a), r, q), b, a);
 }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+__device__ __forceinline__ double sq(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return __builtin_amdgcn_class(x, 0x260) ? x : g;  // class bits: -0, +0, +inf
+}
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Material parameters; derived constants are computed once on the host with
