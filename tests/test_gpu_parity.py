@@ -165,6 +165,25 @@ def test_simulate_2d_bitwise(E, oracle):
         assert _bitwise(got, ref), _report(got, ref)
 
 
+@pytest.mark.parametrize("k", [-900, -400, 400, 900])
+def test_simulate_2d_scaled_states_bitwise(E, oracle, k):
+    """Density, momenta and energy scaled by 2^k (exact: velocities and
+    wavespeeds unchanged, pressures scaled by 2^k): the step loop stays
+    bit-identical to the oracle, so the FP64 division and square root agree
+    with IEEE ones for operands far from 1 as well."""
+    U = cons_field("riemann", (64, 64), seed=1001) * (2.0 ** k)
+    for flux in (0, 1):
+        os_ = _oscheme(oracle, 0, 0, flux, (0, 0))
+        ref, rt, it, _ = oracle.simulate(U, os_, 0.9, 1.0 / 64, max_iters=20)
+        solver = E.SplitSolver(_escheme(E, 0, 0, flux, (0, 0)), layout=1)
+        a = _dev(U, layout=1)
+        r = solver.simulate(a, torch.empty_like(a), 0.9, 1.0 / 64, max_iters=20)
+        got = _host(a, 1)
+        assert r.iters == it
+        assert r.run_time == rt
+        assert _bitwise(got, ref), _report(got, ref)
+
+
 def test_simulate_1d_config0_bitwise(E, oracle):
     """The 1D Riemann workload (4096 cells, run until the fastest wave has
     travelled 0.4) reproduces the oracle bit for bit, including the step
