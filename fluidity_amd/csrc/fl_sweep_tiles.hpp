@@ -38,8 +38,13 @@
 
 // Rows each lane keeps in flight ahead of the row it computes, per
 // precision (the FP32 kernels run twice the waves in half the registers).
+// The FP64 x sweep runs faster with one row (433 against 449 us at 4096^2),
+// the FP64 y sweep with two (410 against 428 us).
 #ifndef FL_PREFETCH
 #define FL_PREFETCH 2
+#endif
+#ifndef FL_PREFETCH_X
+#define FL_PREFETCH_X 1
 #endif
 #ifndef FL_PREFETCH_F32
 #define FL_PREFETCH_F32 2
@@ -50,6 +55,8 @@ template <typename T> struct SweepWaves { static constexpr int value = FL_SWEEP_
 template <> struct SweepWaves<float> { static constexpr int value = FL_SWEEP_WAVES_F32; };
 template <typename T> struct SweepPrefetch { static constexpr int value = FL_PREFETCH; };
 template <> struct SweepPrefetch<float> { static constexpr int value = FL_PREFETCH_F32; };
+template <typename T> struct SweepPrefetchX { static constexpr int value = FL_PREFETCH_X; };
+template <> struct SweepPrefetchX<float> { static constexpr int value = FL_PREFETCH_F32; };
 }  // namespace fl
 
 
@@ -242,7 +249,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   // the compiler can count the loads issued after each one and wait only for
   // the one it needs.
   auto hrow = [&](int64_t r) { return (ND == 2) ? r - 3 : int64_t(0); };
-  constexpr int PF = SweepPrefetch<T>::value;
+  constexpr int PF = SweepPrefetchX<T>::value;
   Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
