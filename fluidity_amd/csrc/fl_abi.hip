@@ -709,6 +709,12 @@ int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme*
     FL_HIP(hipStreamSynchronize(st));
     if (!c->host_ctl->active) break;
   }
+  if (launched == 0) {
+    // max_iters == 0: no step ran; the result is the freshly initialised
+    // control block (t = 0, no iterations) with the initial max wavespeed.
+    FL_HIP(hipMemcpyAsync(c->host_ctl, c->ctl, sizeof(fl::StepCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+  }
   const fl::StepCtl hc = *c->host_ctl;
   if (state->ndim == 1 && (hc.iters & 1)) {
     const int64_t bytes = cell_count(state) * 3 * (state->dtype == FL_F64 ? 8 : 4);

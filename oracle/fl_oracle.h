@@ -86,6 +86,8 @@ void orc_ls_velocity(int nmat, const double* const* phis, const double* const* s
                      const int64_t* n, int nd, double dh, int target, double* V);
 void orc_ls_advect(const double* phi, const double* V, const int64_t* n, int nd, double dt,
                    double dh, double* out);
+void orc_ls_weno(const double* phi, int64_t n, double dh, int64_t i, double* dminus,
+                 double* dplus);
 void orc_ls_project2(double* phi0, double* phi1, int64_t count);
 int  orc_ls_reinit(const double* phi, const int64_t* n, int nd, double dh, int band,
                    double* out, uint8_t* band_mask, int64_t* iterations);

@@ -202,6 +202,18 @@ static double weno_forward(const double* f, double h) {
   return weno_diff(d1, d2, d3, d4, d5);
 }
 
+/* The two one-sided WENO-HJ5 derivatives (D-phi, D+phi) at cell i of a 1D
+ * line with mirrored ghosts: the stencil on its own, for the pins of its
+ * weights, smoothness indicators and epsilon. */
+void orc_ls_weno(const double* phi, int64_t n, double dh, int64_t i, double* dminus,
+                 double* dplus) {
+  lsgrid_t g = {n, 1, 1};
+  double f[7];
+  line7(phi, &g, i, 0, 0, f);
+  *dminus = weno_backward(f, dh);
+  *dplus = weno_forward(f, dh);
+}
+
 /* L = V * H (levelset_hamiltonian.hpp:63-98, godunov_upwind.hpp:94-166). */
 static double ls_rhs(const double* phi, const lsgrid_t* g, int64_t x, int64_t y, double v,
                      double dh) {

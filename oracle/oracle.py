@@ -105,6 +105,8 @@ def lib():
         L.orc_ls_velocity.restype = None
         L.orc_ls_advect.argtypes = [dp, dp, i64p, C.c_int, C.c_double, C.c_double, dp]
         L.orc_ls_advect.restype = None
+        L.orc_ls_weno.argtypes = [dp, C.c_int64, C.c_double, C.c_int64, dp, dp]
+        L.orc_ls_weno.restype = None
         L.orc_ls_project2.argtypes = [dp, dp, C.c_int64]
         L.orc_ls_project2.restype = None
         L.orc_ls_reinit.argtypes = [dp, i64p, C.c_int, C.c_double, C.c_int, dp, u8p, i64p]
@@ -320,6 +322,15 @@ def ls_advect(phi, V, dt, dh):
     out = np.empty_like(phi)
     lib().orc_ls_advect(_dp(phi), _dp(V), _n_ptr(n), nd, dt, dh, _dp(out))
     return out
+
+
+def ls_weno(phi, dh, i):
+    """(D-phi, D+phi) of WENO-HJ5 at cell i of the 1D line phi (mirrored ghosts)."""
+    phi = _f64(phi)
+    dm = C.c_double(0.0)
+    dp_ = C.c_double(0.0)
+    lib().orc_ls_weno(_dp(phi), phi.shape[0], dh, int(i), C.byref(dm), C.byref(dp_))
+    return dm.value, dp_.value
 
 
 def ls_project2(phi0, phi1):
