@@ -598,6 +598,71 @@ int fl_max_wavespeed(fl_context* c, const fl_field* in, const fl_scheme* s, doub
   return FL_OK;
 }
 
+int fl_set_wavespeeds(fl_context* c, const fl_field* in, const fl_scheme* s, double* ws,
+                      void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  if (!ws) return fail(FL_ERR_ARGUMENT, "wavespeeds is NULL");
+  int rc = check_field(in, "in");
+  if (rc) return rc;
+  rc = check_scheme(s, in->ndim);
+  if (rc) return rc;
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  int64_t es, cs;
+  strides(in, es, cs);
+  const int64_t n = cell_count(in);
+  const fl::Material<double> m64 = make_material<double>(s);
+  const int B = fl::kMaxBlock;
+  int64_t blocks = (n + B - 1) / B;
+  if (blocks > 8192) blocks = 8192;
+  const dim3 grid((unsigned)blocks);
+  if (in->dtype == FL_F64) {
+    if (in->ndim == 1)
+      hipLaunchKernelGGL((fl::wavespeeds_kernel<double, 1>), grid, dim3(B), 0, st,
+                         (const double*)in->data, es, cs, n, m64, ws);
+    else
+      hipLaunchKernelGGL((fl::wavespeeds_kernel<double, 2>), grid, dim3(B), 0, st,
+                         (const double*)in->data, es, cs, n, m64, ws);
+  } else {
+    if (in->ndim == 1)
+      hipLaunchKernelGGL((fl::wavespeeds_kernel<float, 1>), grid, dim3(B), 0, st,
+                         (const float*)in->data, es, cs, n, m64, ws);
+    else
+      hipLaunchKernelGGL((fl::wavespeeds_kernel<float, 2>), grid, dim3(B), 0, st,
+                         (const float*)in->data, es, cs, n, m64, ws);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int fl_max_element(fl_context* c, const double* first, int64_t count, double* result,
+                   void* stream) {
+  if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
+  if (!first || !result) return fail(FL_ERR_ARGUMENT, "first or result is NULL");
+  if (count <= 0) return fail(FL_ERR_ARGUMENT, "count must be > 0 (got %lld)", (long long)count);
+  DeviceGuard g(c->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int B = fl::kMaxBlock;
+  int64_t blocks = (count + B - 1) / B;
+  if (blocks > fl::kMaxWsBlocks) blocks = fl::kMaxWsBlocks;
+  hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+  hipLaunchKernelGGL(fl::max_element_kernel, dim3((unsigned)blocks), dim3(B), 0, st, first, count,
+                     c->ws_tmp);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, st));
+  FL_HIP(hipStreamSynchronize(st));
+  const unsigned long long k = c->host_ctl->ws_bits[0];
+  if (k == ~0ull) {
+    *result = NAN;
+  } else {
+    // invert the order-preserving key of the max_element kernel
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    *result = bits_to_double(b);
+  }
+  return FL_OK;
+}
+
 int fl_ctx_set_graphs(fl_context* c, int enable) {
   if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
   c->use_graphs = enable ? 1 : 0;

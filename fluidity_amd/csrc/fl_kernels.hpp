@@ -71,6 +71,10 @@ __global__ void max_wavespeed_kernel(const T* in, int64_t es, int64_t cs, int64_
 template <typename T, int S>
 __global__ void component_sums_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,
                                       double* partial);
+template <typename T, int ND>
+__global__ void wavespeeds_kernel(const T* in, int64_t es, int64_t cs, int64_t ncells,
+                                  Material<double> m64, double* ws);
+__global__ void max_element_kernel(const double* v, int64_t n, unsigned long long* dst);
 __global__ void zero_u64_kernel(unsigned long long* p);
 __global__ void step_control_kernel(StepCtl* ctl, int parity, double cfl, double dh,
                                     double sim_time, long long max_iters);

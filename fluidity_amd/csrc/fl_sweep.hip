@@ -137,6 +137,37 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
   block_atomic_max(v, dst);
 }
 
+// Per-cell FP64 wavespeeds max_d |u_d| + a of a grid into a device array
+// (the wavespeed tensor that the reference's set_wavespeeds fills).
+template <typename T, int ND>
+__global__ void __launch_bounds__(kMaxBlock) wavespeeds_kernel(const T* __restrict__ in,
+                                                                int64_t es, int64_t cs,
+                                                                int64_t ncells,
+                                                                Material<double> m64,
+                                                                double* __restrict__ ws) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += stride) {
+    T u[2 + ND];
+    load_cell<T, 2 + ND>(in, i, es, cs, u);
+    ws[i] = cell_wavespeed64<T, ND>(u, m64);
+  }
+}
+
+// Maximum of n doubles as an order-preserving 64-bit key (see ordered_key),
+// one atomicMax per workgroup into *dst (cleared by the caller to 0, which
+// is below every key).
+__global__ void __launch_bounds__(kMaxBlock) max_element_kernel(const double* __restrict__ v,
+                                                                 int64_t n,
+                                                                 unsigned long long* dst) {
+  unsigned long long m = 0ull;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const unsigned long long k = ordered_key(v[i]);
+    m = k > m ? k : m;
+  }
+  block_atomic_max(m, dst);
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 // One step of Parameters::update_time_delta / dt_dh / update_simulation_info
@@ -250,6 +281,10 @@ template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const d
                                                       Material<double>, unsigned long long*); \
   template __global__ void max_wavespeed_kernel<T, 2>(const T*, int64_t, int64_t, int64_t, \
                                                       Material<double>, unsigned long long*); \
+  template __global__ void wavespeeds_kernel<T, 1>(const T*, int64_t, int64_t, int64_t,        \
+                                                   Material<double>, double*);                   \
+  template __global__ void wavespeeds_kernel<T, 2>(const T*, int64_t, int64_t, int64_t,        \
+                                                   Material<double>, double*);                   \
   template __global__ void component_sums_kernel<T, 3>(const T*, int64_t, int64_t, int64_t, double*); \
   template __global__ void component_sums_kernel<T, 4>(const T*, int64_t, int64_t, int64_t, double*);
 

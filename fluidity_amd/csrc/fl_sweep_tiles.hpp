@@ -132,6 +132,16 @@ __device__ __forceinline__ unsigned long long ws_bits(double w) {
   return (unsigned long long)__double_as_longlong(w);
 }
 
+// A 64-bit key of a double whose unsigned order is the order of the values
+// (-0 and +0 share one key); every NaN maps to the largest key, so a NaN
+// among the values surfaces as the maximum.  Key 0 is below every value.
+__device__ __forceinline__ unsigned long long ordered_key(double v) {
+  if (v != v) return ~0ull;
+  v = (v == 0.0) ? 0.0 : v;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {

@@ -29,7 +29,8 @@ EXPORTED = [
     "fl_ctx_set_tuning", "fl_ctx_get_tuning", "fl_ctx_autotune", "fl_ctx_set_graphs",
     "fl_split_sweep",
     "fl_split_solve", "fl_split_step",
-    "fl_max_wavespeed", "fl_simulate", "fl_conservation_sums",
+    "fl_max_wavespeed", "fl_set_wavespeeds", "fl_max_element", "fl_simulate",
+    "fl_conservation_sums",
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
     "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate_work_bytes",
     "fl_gfm_extrapolate",

@@ -150,6 +150,22 @@ int fl_split_step(fl_context* ctx, fl_field* in, fl_field* out, const fl_scheme*
 int fl_max_wavespeed(fl_context* ctx, const fl_field* in, const fl_scheme* scheme,
                      double* max_wavespeed, void* stream);
 
+/* Per-cell wavespeeds max_d |u_d| + a of `in`, in FP64, into `wavespeeds`
+ * (device, one double per cell, x fastest).  Replaces set_wavespeeds
+ * (simulator/wavespeed_initialization.hpp:34-53,
+ * simulator/cuda/wavespeed_initialization.cuh:41-97) as SimulatorImpl::simulate
+ * calls it (simulator/simulator_impl.hpp:241). */
+int fl_set_wavespeeds(fl_context* ctx, const fl_field* in, const fl_scheme* scheme,
+                      double* wavespeeds, void* stream);
+
+/* The maximum of `count` doubles at `first` (device), returned on the host
+ * (synchronises `stream`).  A NaN among them gives NaN; count must be > 0.
+ * Replaces max_element over a device tensor's range (algorithm/max_element.hpp:
+ * 28-38, algorithm/cuda/reduce.cuh) as SimulatorImpl::simulate calls it
+ * (simulator/simulator_impl.hpp:242-243). */
+int fl_max_element(fl_context* ctx, const double* first, int64_t count, double* result,
+                   void* stream);
+
 /* The simulator loop of SimulatorImpl::simulate
  * (simulator/simulator_impl.hpp:222-258): while run_time < sim_time and
  * iters < max_iters, dt = cfl*dh/max_ws, all sweeps, run_time += dt.  dt is

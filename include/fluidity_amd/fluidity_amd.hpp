@@ -29,6 +29,8 @@
 #include <type_traits>
 #include <utility>
 
+#include <hip/hip_runtime_api.h>
+
 #include "../fluidity_amd.h"
 
 namespace fluid {
@@ -74,8 +76,13 @@ template <typename T> constexpr int dtype_of();
 template <> constexpr int dtype_of<double>() { return FL_F64; }
 template <> constexpr int dtype_of<float>() { return FL_F32; }
 
-// The iterator role of MultidimIterator<State<T, conservative, D>, ..., gpu>:
-// a device pointer to interior cells and the interior sizes.
+// The iterator role of MultidimIterator<State<T, conservative, D>, DimInfo, gpu>
+// (iterator/multidim_iterator.hpp:45-400): a device pointer to interior
+// cells and the interior sizes, with the reference iterator's size(dim),
+// stride(dim), num_dimensions(), offset(amount, dim) and shift(amount, dim).
+// The entry points take an iterator at the grid's first cell, as the
+// reference's simulator passes them; an offset iterator views the same
+// memory from another cell.
 template <typename T, std::size_t D, int Layout = FL_LAYOUT_AOS>
 struct GridView {
   using value_t = T;
@@ -88,7 +95,28 @@ struct GridView {
     n[0] = nx;
     n[1] = ny;
   }
-  std::size_t size(std::size_t dim) const { return static_cast<std::size_t>(n[dim]); }
+  template <typename Dim> std::size_t size(Dim dim) const {
+    return static_cast<std::size_t>(n[static_cast<std::size_t>(dim)]);
+  }
+  constexpr std::size_t num_dimensions() const { return D; }
+  // Cells between neighbours in dimension dim (x fastest).
+  template <typename Dim> std::size_t stride(Dim dim) const {
+    std::size_t st = 1;
+    for (std::size_t d = 0; d < static_cast<std::size_t>(dim); ++d) st *= size(d);
+    return st;
+  }
+  template <typename Dim> GridView offset(int amount, Dim dim) const {
+    GridView o = *this;
+    o.shift(amount, dim);
+    return o;
+  }
+  template <typename Dim> GridView& shift(int amount, Dim dim) {
+    const int64_t cells = static_cast<int64_t>(amount) * static_cast<int64_t>(stride(dim));
+    data += Layout == FL_LAYOUT_AOS ? cells * static_cast<int64_t>(2 + D) : cells;
+    return *this;
+  }
+  T* get_ptr() const { return data; }
+  std::size_t total_size() const { return size(0) * size(1) * size(2); }
   fl_field field() const {
     fl_field f{};
     f.data = data;
@@ -100,6 +128,49 @@ struct GridView {
     f.n[2] = n[2];
     return f;
   }
+};
+
+template <typename T, std::size_t D, int Layout = FL_LAYOUT_AOS>
+using MultidimIterator = GridView<T, D, Layout>;
+
+// Device memory of one array (the role of DeviceTensor<T, 1>,
+// container/device_tensor.hpp): resize, begin/end as device pointers, and a
+// 1D multi_iterator over it.
+template <typename T>
+class DeviceArray {
+ public:
+  struct iterator {
+    using value_t = T;
+    T* ptr = nullptr;
+    T* get_ptr() const { return ptr; }
+    friend std::ptrdiff_t operator-(const iterator& a, const iterator& b) { return a.ptr - b.ptr; }
+  };
+  DeviceArray() = default;
+  explicit DeviceArray(std::size_t n) { resize(n); }
+  ~DeviceArray() { release(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  void resize(std::size_t n) {
+    if (n == size_) return;
+    release();
+    if (n && hipMalloc(reinterpret_cast<void**>(&data_), n * sizeof(T)) != hipSuccess)
+      throw Error(FL_ERR_HIP, "hipMalloc of a device array failed");
+    size_ = n;
+  }
+  std::size_t size() const { return size_; }
+  T* data() const { return data_; }
+  iterator begin() const { return iterator{data_}; }
+  iterator end() const { return iterator{data_ + size_}; }
+  GridView<T, 1> multi_iterator() const { return GridView<T, 1>(data_, (int64_t)size_); }
+
+ private:
+  void release() {
+    if (data_) (void)hipFree(data_);
+    data_ = nullptr;
+    size_ = 0;
+  }
+  T* data_ = nullptr;
+  std::size_t size_ = 0;
 };
 
 }  // namespace amd
@@ -163,6 +234,17 @@ template <typename T> struct StiffFluid {
   T p_inf_ = T(0.613);
 };
 }  // namespace material
+
+// The material fields of an fl_scheme (eos, gamma, p_inf).
+template <typename Mat>
+fl_scheme material_scheme(const Mat& mat) {
+  using M = std::decay_t<Mat>;
+  fl_scheme s{};
+  s.eos = M::eos;
+  s.gamma = static_cast<double>(mat.adiabatic());
+  s.p_inf = static_cast<double>(mat.p_inf());
+  return s;
+}
 
 namespace solver {
 
@@ -254,16 +336,124 @@ struct SplitSolver {
 template <typename It, typename Mat>
 double max_wavespeed(const It& in, const Mat& mat, void* stream = nullptr) {
   fl_field f = in.field();
-  fl_scheme s{};
-  s.gamma = static_cast<double>(mat.adiabatic());
-  s.p_inf = static_cast<double>(mat.p_inf());
-  s.eos = Mat::eos;
+  fl_scheme s = material_scheme(mat);
   double w = 0.0;
   amd::check(fl_max_wavespeed(amd::Context::instance().get(), &f, &s, &w, stream));
   return w;
 }
 
 namespace sim {
+
+// set_wavespeeds (simulator/wavespeed_initialization.hpp:34-53): the FP64
+// wavespeed of every cell of `states` into the array `wavespeeds` points at.
+template <typename I1, typename I2, typename Mat>
+void set_wavespeeds(I1&& states, I2&& wavespeeds, Mat&& mat) {
+  fl_field f = states.field();
+  fl_scheme s = material_scheme(mat);
+  amd::check(fl_set_wavespeeds(amd::Context::instance().get(), &f, &s,
+                               static_cast<double*>(wavespeeds.get_ptr()), nullptr));
+}
+
+}  // namespace sim
+
+// max_element (algorithm/max_element.hpp:28-38) over a device array's range.
+template <typename It, typename Return = typename std::decay_t<It>::value_t>
+Return max_element(It&& begin, It&& end) {
+  double m = 0.0;
+  amd::check(fl_max_element(amd::Context::instance().get(), begin.get_ptr(),
+                            static_cast<int64_t>(end - begin), &m, nullptr));
+  return static_cast<Return>(m);
+}
+
+namespace sim {
+
+// Domain (simulator/domain.hpp): the resolution the simulator uses.
+struct Domain {
+  explicit Domain(std::size_t dims = 1) : dims_(dims) {}
+  void set_resolution(double r) { res_ = r; }
+  double resolution() const { return res_; }
+  std::size_t num_dimensions() const { return dims_; }
+
+ private:
+  std::size_t dims_;
+  double res_ = 0.0;
+};
+
+// Parameters (simulator/parameters.hpp:30-99): cfl, run time, iterations and
+// the host-side dt of the reference's loop, with the same expressions.
+template <typename T>
+struct Parameters {
+  using value_t = std::decay_t<T>;
+  using iter_t = std::size_t;
+  static constexpr iter_t default_iters = static_cast<iter_t>(-1);
+
+  value_t cfl = 0.9;
+  value_t run_time = 0.0;
+  value_t sim_time = 0.0;
+  iter_t iters = 0;
+  iter_t max_iters = default_iters;
+  Domain domain;
+
+  explicit Parameters(std::size_t num_dimensions) : domain(num_dimensions) {}
+  void update_time_delta(value_t max_wavespeed) {
+    dt_ = cfl * domain.resolution() / max_wavespeed;
+  }
+  void update_simulation_info() {
+    run_time += dt_;
+    iters += iter_t{1};
+  }
+  bool continue_simulation() const { return run_time < sim_time && iters < max_iters; }
+  value_t dt() const { return dt_; }
+  value_t dt_dh() const { return dt_ / domain.resolution(); }
+
+ private:
+  value_t dt_ = 0.0;
+};
+
+// SimulationData (simulator/simulation_data.hpp:40-140) on the device: input
+// and output state grids, the wavespeed array, and swap(a, b) re-pointing the
+// input at a and the output at b.  The grids are allocated here; upload()
+// and download() move host data in the reference's AoS layout.
+template <typename T, std::size_t D>
+class SimulationData {
+ public:
+  using iter_t = amd::GridView<T, D>;
+
+  SimulationData(int64_t nx, int64_t ny = 1)
+      : nx_(nx), ny_(D > 1 ? ny : 1), a_(cells() * (2 + D)), b_(cells() * (2 + D)) {
+    in_ = a_.data();
+    out_ = b_.data();
+  }
+  std::size_t cells() const { return static_cast<std::size_t>(nx_ * ny_); }
+  void initialize() { wavespeeds_.resize(cells()); }
+  iter_t input_iterator() const { return iter_t(in_, nx_, ny_); }
+  iter_t output_iterator() const { return iter_t(out_, nx_, ny_); }
+  amd::GridView<double, D> wavespeed_iterator() const {
+    return amd::GridView<double, D>(wavespeeds_.data(), nx_, ny_);
+  }
+  amd::DeviceArray<double>& wavespeeds() { return wavespeeds_; }
+  const amd::DeviceArray<double>& wavespeeds() const { return wavespeeds_; }
+  template <typename A, typename B>
+  void swap(A&& a, B&& b) {
+    in_ = a.get_ptr();
+    out_ = b.get_ptr();
+  }
+  void upload(const T* host) {
+    if (hipMemcpy(in_, host, cells() * (2 + D) * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      throw amd::Error(FL_ERR_HIP, "upload failed");
+  }
+  void download(T* host) const {
+    if (hipMemcpy(host, in_, cells() * (2 + D) * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+      throw amd::Error(FL_ERR_HIP, "download failed");
+  }
+
+ private:
+  int64_t nx_, ny_;
+  amd::DeviceArray<T> a_, b_;
+  amd::DeviceArray<double> wavespeeds_;
+  T* in_ = nullptr;
+  T* out_ = nullptr;
+};
 
 // The parameter surface of SimulatorImpl (simulator/simulator_impl.hpp:260-291)
 // and its simulate() loop (:222-258), with dt kept on the device.
@@ -279,16 +469,25 @@ class Simulator {
   Simulator& configure_material(const eos_t& m) { mat_ = m; return *this; }
   solver::BoundarySetter& setter() { return setter_; }
 
-  // Advances `state` in place (`work` is a same-sized scratch grid).
+  // Advances `state` in place (`work` is a same-sized scratch grid).  As in
+  // the reference, run time and iterations accumulate over calls
+  // (simulator/parameters.hpp:43-76): a second call continues until the
+  // accumulated values reach sim_time or max_iters.
   template <typename It>
   fl_sim_result simulate(It& state, It& work, void* stream = nullptr) {
     fl_field fs = state.field(), fw = work.field();
     const fl_scheme s = Solver::scheme(mat_, setter_);
+    fl_sim_params p = p_;
+    p.sim_time = p_.sim_time - run_time_;
+    p.max_iters = p_.max_iters > iters_ ? p_.max_iters - iters_ : 0;
     fl_sim_result r{};
-    amd::check(fl_simulate(amd::Context::instance().get(), &fs, &fw, &s, &p_, &r, stream));
+    amd::check(fl_simulate(ctx(), &fs, &fw, &s, &p, &r, stream));
     run_time_ += r.run_time;
+    iters_ += r.iters;
     return r;
   }
+  double run_time() const { return run_time_; }
+  int64_t iterations() const { return iters_; }
 
   // print_results / write_results / write_results_separate_raw
   // (simulator/simulator_impl.hpp:343-420): the reference's ASCII formats of
@@ -320,7 +519,10 @@ class Simulator {
                                 p_.dh, path, stream));
   }
 
+  fl_context* ctx() const { return amd::Context::instance().get(); }
+
   double run_time_ = 0.0;
+  int64_t iters_ = 0;
   fl_sim_params p_{0.9, 0.0, 1e300, int64_t(1) << 40};
   eos_t mat_{};
   solver::BoundarySetter setter_{};

@@ -4,6 +4,14 @@
 //   shim_driver solve    <combo> <nd> <nx> <ny> <dtdh>  <in.bin> <out.bin>
 //   shim_driver simulate <combo> <nd> <nx> <ny> <steps> <in.bin> <out.bin>
 //   shim_driver write    <combo> <nd> <nx> <ny> <steps> <in.bin> <prefix>
+//   shim_driver twice    <combo> <nd> <nx> <ny> <steps> <in.bin> <out.bin>
+//   shim_driver impl     <combo> <nd> <nx> <ny> <steps> <in.bin> <out.bin>
+//
+// `twice` runs sim::Simulator::simulate with a limit of steps/2 iterations,
+// raises the limit to `steps` and calls it again (run time and iterations
+// accumulate across calls as in the reference).  `impl` runs RefSimulator
+// below, whose simulate() body is SimulatorImpl::simulate's loop
+// (simulator/simulator_impl.hpp:222-249) with its lines unchanged.
 //
 // `write` runs `steps` steps with sim_time 10, then writes <prefix>.dat
 // (write_results) and <prefix>_<name>_10.000000.dat (write_results_separate_raw).
@@ -53,6 +61,55 @@ static void write_file(const char* path, const std::vector<double>& v) {
   std::fclose(f);
 }
 
+// The members and loop of the reference's SimulatorImpl
+// (simulator/simulator_impl.hpp:43-211, 222-258) over the shim's types; the
+// loop body below is the reference's, line for line, minus its printing and
+// timer.  It sits in fluid::sim as SimulatorImpl does, so the loop's
+// unqualified names resolve as in the reference.
+namespace fluid {
+namespace sim {
+template <typename Solver, std::size_t D>
+class RefSimulator {
+ public:
+  using solver_t = Solver;
+  using material_t = typename Solver::eos_t;
+
+  RefSimulator(int64_t nx, int64_t ny) : _params(D), _data(nx, ny) {}
+
+  void simulate() {
+    auto solver = solver_t{_data.input_iterator()};
+    auto mat    = material_t{};
+
+    _data.initialize();
+
+    auto cfl        = _params.cfl;
+    auto wavespeeds = _data.wavespeed_iterator();
+    (void)cfl;
+
+    while (_params.continue_simulation())
+    {
+      auto in  = _data.input_iterator();
+      auto out = _data.output_iterator();
+
+      // Set the wavespeed data based on the updated state data from the previous
+      // iteration, and then update sim time delta based on max wavespeed:
+      set_wavespeeds(in, wavespeeds, mat);
+      _params.update_time_delta(max_element(_data.wavespeeds().begin(),
+                                            _data.wavespeeds().end()));
+
+      solver.solve(in, out, mat, _params.dt_dh(), _setter);
+      _params.update_simulation_info();
+      _data.swap(out, in);
+    }
+  }
+
+  Parameters<double> _params;
+  SimulationData<double, D> _data;
+  solver::BoundarySetter _setter;
+};
+}  // namespace sim
+}  // namespace fluid
+
 template <typename Solver, std::size_t D>
 static int run(const std::string& mode, int64_t nx, int64_t ny, double arg, const char* in_path,
                const char* out_path) {
@@ -71,6 +128,38 @@ static int run(const std::string& mode, int64_t nx, int64_t ny, double arg, cons
   solver::BoundarySetter setter;
   setter.configure(dim_x, solver::BoundaryIndex::first, solver::BoundaryKind::transmissive);
   try {
+    if (mode == "impl") {
+      (void)hipFree(a);
+      (void)hipFree(b);
+      sim::RefSimulator<Solver, D> sim(nx, ny);
+      sim._params.cfl = 0.9;
+      sim._params.sim_time = 1e300;
+      sim._params.max_iters = static_cast<std::size_t>(arg);
+      sim._params.domain.set_resolution(1.0 / nx);
+      sim._data.upload(host.data());
+      sim.simulate();
+      sim._data.download(host.data());
+      std::printf("run_time %.17g iters %llu\n", sim._params.run_time,
+                  (unsigned long long)sim._params.iters);
+      write_file(out_path, host);
+      return 0;
+    }
+    if (mode == "twice") {
+      sim::Simulator<Solver> simr;
+      const int64_t steps = static_cast<int64_t>(arg);
+      simr.configure_cfl(0.9).configure_resolution(1.0 / nx).configure_max_iterations(steps / 2);
+      simr.configure_material(mat);
+      simr.simulate(in, out);
+      simr.configure_max_iterations(steps);
+      const fl_sim_result r = simr.simulate(in, out);
+      std::printf("run_time %.17g iters %lld last %lld\n", simr.run_time(),
+                  (long long)simr.iterations(), (long long)r.iters);
+      (void)hipMemcpy(host.data(), in.data, n * sizeof(double), hipMemcpyDeviceToHost);
+      write_file(out_path, host);
+      (void)hipFree(a);
+      (void)hipFree(b);
+      return 0;
+    }
     if (mode == "solve") {
       Solver solver{in};
       solver.solve(in, out, mat, arg, setter);  // result in `out` afterwards
@@ -105,7 +194,7 @@ static int run(const std::string& mode, int64_t nx, int64_t ny, double arg, cons
 
 int main(int argc, char** argv) {
   if (argc != 9) {
-    std::fprintf(stderr, "usage: %s solve|simulate|write combo nd nx ny arg in.bin out\n",
+    std::fprintf(stderr, "usage: %s solve|simulate|write|twice|impl combo nd nx ny arg in.bin out\n",
                  argv[0]);
     return 1;
   }

@@ -384,3 +384,48 @@ def test_split_step_fused_max(E, oracle, shape):
         ref = oracle.sweep(ref, os_, dtdh, dim)
     assert _bitwise(got, ref), _report(got, ref)
     assert ws == oracle.max_wavespeed_grid(ref, os_)
+
+
+@pytest.mark.parametrize("shape", [(1001,), (40, 52)])
+def test_simulate_zero_steps_is_defined(E, oracle, shape):
+    """max_iters = 0 runs no step: a fresh context and one that has run
+    before both report t = 0, no iterations and the initial maximum
+    wavespeed, and leave the state untouched (1D too, whose odd step counts
+    end in the work buffer)."""
+    U = cons_field("rough", shape, seed=23)
+    nd = len(shape)
+    os_ = _oscheme(oracle, 0, 0, 0, (0,) * nd)
+    ws0 = oracle.max_wavespeed_grid(U, os_)
+    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0,) * nd))
+    a = _dev(U)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.5, 0.01, max_iters=0)
+    assert r.iters == 0 and r.run_time == 0.0 and r.max_wavespeed == ws0
+    assert _bitwise(_host(a), U)
+    r = solver.simulate(a, w, 0.5, 0.01, max_iters=3)   # odd count: 1D ends in `work`
+    assert r.iters == 3
+    after = _host(a)
+    r = solver.simulate(a, w, 0.5, 0.01, max_iters=0)
+    assert r.iters == 0 and r.run_time == 0.0
+    assert r.max_wavespeed == oracle.max_wavespeed_grid(after, os_)
+    assert _bitwise(_host(a), after)
+
+
+def test_simulator_continues_after_iteration_limit(E):
+    """Simulator.simulate() after its iteration limit was reached runs no
+    step: iterations, run time and the state stay as they were."""
+    from fluidity_amd.simulator import Simulator
+    sim = Simulator(dimensions=1)
+    sim.configure_resolution(1.0 / 256)
+    sim.configure_dimension(0, 0.0, 1.0)
+    sim.configure_max_iterations(5)
+    sim.fill_data([("rho", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.125)),
+                   ("p", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.1))])
+    sim.simulate()
+    it, t = sim.params.iters, sim.params.run_time
+    state = sim.state.clone()
+    assert it == 5 and t > 0
+    r = sim.simulate()
+    assert r.iters == 0 and r.run_time == 0.0
+    assert sim.params.iters == it and sim.params.run_time == t
+    assert torch.equal(sim.state, state)

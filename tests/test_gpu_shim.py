@@ -57,3 +57,34 @@ def test_shim_simulator_loop(driver, oracle, tmp_path):
     assert f"iters {it}" in out
     assert float(out.split()[1]) == rt
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("combo", [0, 1])
+@pytest.mark.parametrize("shape", [(300,), (48, 40)])
+def test_shim_reference_simulate_loop(driver, oracle, tmp_path, combo, shape):
+    """SimulatorImpl::simulate's loop lines (set_wavespeeds, max_element,
+    update_time_delta, solve with the reference's in/out swap, the
+    SimulationData swap) compiled unchanged against the shim: bit-identical
+    to the oracle's loop, with the same run time and iteration count."""
+    U = cons_field("riemann" if len(shape) == 2 else "rough", shape, seed=1001)
+    nx = shape[-1] if len(shape) == 2 else shape[0]
+    s = oracle.scheme(limiter=combo, form=combo, flux=combo, bc=(0, 0))
+    ref, rt, it, _ = oracle.simulate(U, s, 0.9, 1.0 / nx, max_iters=9)
+    got, out = _run(driver, tmp_path, "impl", combo, U, 9)
+    assert f"iters {it}" in out
+    assert float(out.split()[1]) == rt
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_shim_simulator_continues_across_calls(driver, oracle, tmp_path):
+    """Two simulate() calls with the iteration limit raised in between run
+    the steps of one call of the full length (run time and iterations
+    accumulate, simulator/parameters.hpp:43-76)."""
+    U = cons_field("riemann", (40, 48), seed=1001)
+    s = oracle.scheme(flux=0)
+    ref, rt, it, _ = oracle.simulate(U, s, 0.9, 1.0 / 48, max_iters=12)
+    got, out = _run(driver, tmp_path, "twice", 0, U, 12)
+    f = out.split()
+    assert int(f[3]) == it == 12 and int(f[5]) == 6
+    assert float(f[1]) == rt
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
