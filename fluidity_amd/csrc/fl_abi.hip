@@ -76,7 +76,7 @@ int check_scheme(const fl_scheme* s, int ndim) {
     return fail(FL_ERR_ARGUMENT, "scheme->form %d invalid", s->form);
   if (s->flux < FL_FLUX_HLLC || s->flux > FL_FLUX_RICHTMYER)
     return fail(FL_ERR_ARGUMENT, "scheme->flux %d invalid", s->flux);
-  if (s->eos != FL_EOS_IDEAL && s->eos != FL_EOS_STIFF)
+  if (s->eos != FL_EOS_IDEAL && s->eos != FL_EOS_STIFF && s->eos != FL_EOS_STIFF_CORRECTED)
     return fail(FL_ERR_ARGUMENT, "scheme->eos %d invalid", s->eos);
   if (!(s->gamma > 1.0) || !isfinite(s->gamma))
     return fail(FL_ERR_ARGUMENT, "scheme->gamma %g must be finite and > 1", s->gamma);
@@ -84,6 +84,14 @@ int check_scheme(const fl_scheme* s, int ndim) {
   for (int d = 0; d < ndim; ++d)
     if (s->bc[d] < FL_BC_TRANSMISSIVE || s->bc[d] > FL_BC_PERIODIC)
       return fail(FL_ERR_ARGUMENT, "scheme->bc[%d] %d invalid", d, s->bc[d]);
+  if (s->numerics != FL_NUMERICS_EXACT && s->numerics != FL_NUMERICS_FAST)
+    return fail(FL_ERR_ARGUMENT, "scheme->numerics %d invalid", s->numerics);
+  return FL_OK;
+}
+
+int require_exact(const fl_scheme* s, const char* what) {
+  if (s->numerics != FL_NUMERICS_EXACT)
+    return fail(FL_ERR_UNSUPPORTED, "%s runs the exact FP64 numerics only", what);
   return FL_OK;
 }
 
@@ -137,35 +145,46 @@ int fit_rows(int rows, int64_t bands, int64_t ny, int block) {
 }
 
 // One sweep kernel of flux FLUX: the 1D sweep, or the x or y sweep of a 2D grid.
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool FAST>
 void launch_flux(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
                  hipStream_t stream) {
   if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX, FAST>), grid, dim3(B), 0, stream, a);
   else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST>), grid, dim3(B), 0, stream, a);
   else
-    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST>), grid, dim3(B), 0, stream, a);
 }
 
+// The fast FP64 numerics were requested (FP32 sweeps use them always).
+inline bool fast_numerics(const fl_scheme* s) { return s->numerics == FL_NUMERICS_FAST; }
+
 // Workgroups of B lanes of one sweep kernel that a CU holds at once.
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool FAST>
 int sweep_blocks_per_cu(int dim, int B) {
-  const void* k = dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX>)
-                           : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX>);
+  const void* k = dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX, FAST>)
+                           : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX, FAST>);
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, 0) != hipSuccess) return 0;
   return n;
 }
 
-template <typename T>
-int sweep_blocks_per_cu_t(int dim, int flux, int B) {
+template <typename T, bool FAST>
+int sweep_blocks_per_cu_f(int dim, int flux, int B) {
   switch (flux) {
-    case FL_FLUX_HLLC: return sweep_blocks_per_cu<T, fl::kFluxHllc>(dim, B);
-    case FL_FLUX_FORCE: return sweep_blocks_per_cu<T, fl::kFluxForce>(dim, B);
-    case FL_FLUX_LAX_FRIEDRICHS: return sweep_blocks_per_cu<T, fl::kFluxLaxFriedrichs>(dim, B);
-    default: return sweep_blocks_per_cu<T, fl::kFluxRichtmyer>(dim, B);
+    case FL_FLUX_HLLC: return sweep_blocks_per_cu<T, fl::kFluxHllc, FAST>(dim, B);
+    case FL_FLUX_FORCE: return sweep_blocks_per_cu<T, fl::kFluxForce, FAST>(dim, B);
+    case FL_FLUX_LAX_FRIEDRICHS:
+      return sweep_blocks_per_cu<T, fl::kFluxLaxFriedrichs, FAST>(dim, B);
+    default: return sweep_blocks_per_cu<T, fl::kFluxRichtmyer, FAST>(dim, B);
   }
+}
+
+template <typename T>
+int sweep_blocks_per_cu_t(int dim, const fl_scheme* s, int B) {
+  if (std::is_same<T, double>::value && fast_numerics(s))
+    return sweep_blocks_per_cu_f<T, true>(dim, s->flux, B);
+  return sweep_blocks_per_cu_f<T, false>(dim, s->flux, B);
 }
 
 // Tile heights to time for one 2D sweep: the powers of two in `base`, and,
@@ -184,6 +203,19 @@ std::vector<int> row_candidates(const int* base, int nbase, int64_t bands, int64
       rows.push_back((int)h);
   }
   return rows;
+}
+
+template <typename T, bool FAST>
+void launch_flux_t(int flux, int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
+                   hipStream_t stream) {
+  switch (flux) {
+    case FL_FLUX_HLLC: launch_flux<T, fl::kFluxHllc, FAST>(ndim, dim, grid, B, a, stream); break;
+    case FL_FLUX_FORCE: launch_flux<T, fl::kFluxForce, FAST>(ndim, dim, grid, B, a, stream); break;
+    case FL_FLUX_LAX_FRIEDRICHS:
+      launch_flux<T, fl::kFluxLaxFriedrichs, FAST>(ndim, dim, grid, B, a, stream);
+      break;
+    default: launch_flux<T, fl::kFluxRichtmyer, FAST>(ndim, dim, grid, B, a, stream); break;
+  }
 }
 
 template <typename T>
@@ -213,14 +245,10 @@ int launch_sweep_t(const fl_context* ctx, const fl_field* in, const fl_field* ou
     a = make_args<T>(ctx, in, out, s, dtdh, ctl, ws_out, tu.y_rows);
     grid = dim3((unsigned)((a.nx + B - 1) / B), (unsigned)((a.ny + tu.y_rows - 1) / tu.y_rows), 1);
   }
-  switch (s->flux) {
-    case FL_FLUX_HLLC: launch_flux<T, fl::kFluxHllc>(in->ndim, dim, grid, B, a, stream); break;
-    case FL_FLUX_FORCE: launch_flux<T, fl::kFluxForce>(in->ndim, dim, grid, B, a, stream); break;
-    case FL_FLUX_LAX_FRIEDRICHS:
-      launch_flux<T, fl::kFluxLaxFriedrichs>(in->ndim, dim, grid, B, a, stream);
-      break;
-    default: launch_flux<T, fl::kFluxRichtmyer>(in->ndim, dim, grid, B, a, stream); break;
-  }
+  if (std::is_same<T, double>::value && fast_numerics(s))
+    launch_flux_t<T, true>(s->flux, in->ndim, dim, grid, B, a, stream);
+  else
+    launch_flux_t<T, false>(s->flux, in->ndim, dim, grid, B, a, stream);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -474,8 +502,8 @@ int fl_ctx_autotune(fl_context* c, const fl_field* in, const fl_field* work,
       if (in->ndim == 2) {
         const int64_t nx = in->n[0], ny = in->n[1];
         const int64_t bands = dim ? (nx + w - 1) / w : (nx + w - 3) / (w - 2);
-        const int per_cu = in->dtype == FL_F64 ? sweep_blocks_per_cu_t<double>(dim, s->flux, w)
-                                               : sweep_blocks_per_cu_t<float>(dim, s->flux, w);
+        const int per_cu = in->dtype == FL_F64 ? sweep_blocks_per_cu_t<double>(dim, s, w)
+                                               : sweep_blocks_per_cu_t<float>(dim, s, w);
         rows = row_candidates(dim ? yrows : xrows, 4, bands, ny, (int64_t)per_cu * cus);
       }
       for (size_t k = 0; k < rows.size() && !rc; ++k) {

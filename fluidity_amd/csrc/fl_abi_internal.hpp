@@ -78,13 +78,17 @@ inline fl::Material<T> make_material(const fl_scheme* s) {
   m.p_inf = (T)s->p_inf;
   m.gm1 = m.gamma - T(1);
   m.q = (m.gamma + T(1)) / (T(2) * m.gamma);
-  m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
+  m.stiff = (s->eos == FL_EOS_STIFF || s->eos == FL_EOS_STIFF_CORRECTED) ? 1 : 0;
+  m.corr = s->eos == FL_EOS_STIFF_CORRECTED ? 1 : 0;
+  m.gpinf = m.corr ? m.gamma * m.p_inf : T(0);
   return m;
 }
 
 double bits_to_double(unsigned long long b);
 int check_field(const fl_field* f, const char* name);
 int check_scheme(const fl_scheme* s, int ndim);
+// FL_ERR_UNSUPPORTED unless s asks for the exact numerics.
+int require_exact(const fl_scheme* s, const char* what);
 void strides(const fl_field* f, int64_t& es, int64_t& cs);
 int64_t cell_count(const fl_field* f);
 // One sweep; ctl/ws_out as in fl_simulate (NULL: dtdh from the argument, no

@@ -27,6 +27,9 @@ int check_adj_field(const fl_field* f, const char* name) {
 int check_adj_scheme(const fl_scheme* s, int ndim) {
   int rc = check_scheme(s, ndim);
   if (rc) return rc;
+  if ((rc = require_exact(s, "the adjoint"))) return rc;
+  if (s->eos == FL_EOS_STIFF_CORRECTED)
+    return fail(FL_ERR_UNSUPPORTED, "the adjoint covers the reference's equations of state");
   if (s->flux != FL_FLUX_HLLC || s->form != FL_FORM_CONSERVATIVE)
     return fail(FL_ERR_UNSUPPORTED,
                 "the adjoint covers HLLC with conservative-form limiting only");
@@ -45,6 +48,8 @@ fl::Material<double> mat64(const fl_scheme* s) {
   m.gm1 = m.gamma - 1.0;
   m.q = (m.gamma + 1.0) / (2.0 * m.gamma);
   m.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
+  m.corr = 0;
+  m.gpinf = 0.0;
   return m;
 }
 

@@ -85,12 +85,13 @@ __global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __r
   auto at = [&](const double* A, int64_t xx, int64_t yy) {
     return A[(ND > 1 ? (yy + 4) * px : 0) + xx + 4];
   };
+  const Material<double> no_shift{};  // the adjoint runs the reference's equations of state
   auto chi_d = [&](int64_t xx, int64_t yy, int d) {
     const int64_t dx = (d == 0), dy = (d == 1);
     const double* V = d == 0 ? V0 : V1;
     return chi_factor<double>(at(P, xx - 2 * dx, yy - 2 * dy), at(P, xx - dx, yy - dy),
                               at(P, xx + dx, yy + dy), at(P, xx + 2 * dx, yy + 2 * dy),
-                              at(V, xx - dx, yy - dy), at(V, xx + dx, yy + dy));
+                              at(V, xx - dx, yy - dy), at(V, xx + dx, yy + dy), no_shift);
   };
   double c = DBL_MAX;
   for (int d = 0; d < ND; ++d) {

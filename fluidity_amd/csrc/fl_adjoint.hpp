@@ -232,12 +232,12 @@ __device__ __forceinline__ void hllc_adj(const double* ul, const double* ur,
   const double p_pvrs =
       0.5 * ((wl.p + wr.p) - (0.25 * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
   const double p_star = smax(0.0, p_pvrs);
-  const double wsl = hllc_wavespeed(vl, wl.p, p_star, -al, m.q);
+  const double wsl = hllc_wavespeed(vl, wl.p, p_star, -al, m);
   if (0.0 <= wsl) {
     flux_full_adj<ND, D>(ul, wl, m, gF, gl, gm);
     return;
   }
-  const double wsr = hllc_wavespeed(vr, wr.p, p_star, ar, m.q);
+  const double wsr = hllc_wavespeed(vr, wr.p, p_star, ar, m);
   if (0.0 >= wsr) {
     flux_full_adj<ND, D>(ur, wr, m, gF, gr, gm);
     return;

@@ -270,6 +270,7 @@ int fl_batch_create(fl_context* c, const fl_batch_problem* problems, int nprob, 
     int rc = check_field(&q.state, "problem state");
     if (!rc) rc = check_field(&q.work, "problem work");
     if (!rc) rc = check_scheme(&q.scheme, q.state.ndim);
+    if (!rc && q.state.dtype == FL_F64) rc = require_exact(&q.scheme, "an FP64 batch");
     if (rc) return fail(rc, "problem %d: %s", p, fl_last_error());
     if (q.work.dtype != q.state.dtype || q.work.layout != q.state.layout ||
         q.work.ndim != q.state.ndim || q.work.n[0] != q.state.n[0] ||

@@ -559,6 +559,7 @@ __device__ __forceinline__ void cons_to_prim(const double* u, const Eos& e, doub
   }
   w[0] = u[0];
   w[1] = (e.gamma - 1.0) * (u[1] - 0.5 * u[0] * sum);
+  if (e.corr) w[1] = w[1] - e.gamma * e.p_inf;
 #pragma unroll
   for (int i = 0; i < ND; ++i) w[2 + i] = u[2 + i] / u[0];
 }
@@ -584,7 +585,8 @@ __device__ __forceinline__ double pressure(const double* u, const Eos& e) {
     const double v = u[2 + i] / u[0];
     sum += v * v;
   }
-  return (e.gamma - 1.0) * (u[1] - 0.5 * u[0] * sum);
+  const double p = (e.gamma - 1.0) * (u[1] - 0.5 * u[0] * sum);
+  return e.corr ? p - e.gamma * e.p_inf : p;
 }
 
 template <int ND>
@@ -602,11 +604,16 @@ __device__ void hllc_star_left(const double* ul, const double* ur, const Eos& el
   const double pl = pressure<ND>(ul, el), pr = pressure<ND>(ur, er);
   const double vl = ul[2] / ul[0], vr = ur[2] / ur[0];
   const double p_pvrs = 0.5 * ((pl + pr) - (0.25 * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
-  const double p_star = smax(0.0, p_pvrs);
+  // a corrected stiffened gas on both sides bounds p* by -min(p_inf)
+  const double p_star = (el.corr && er.corr) ? smax(-smin(el.p_inf, er.p_inf), p_pvrs)
+                                             : smax(0.0, p_pvrs);
   const double ql = (el.gamma + 1.0) / (2.0 * el.gamma);
   const double qr = (er.gamma + 1.0) / (2.0 * er.gamma);
-  const double wsl = (p_star <= pl) ? vl + -al : vl + -al * sqrt(1.0 + ql * (p_star / pl - 1.0));
-  const double wsr = (p_star <= pr) ? vr + ar : vr + ar * sqrt(1.0 + qr * (p_star / pr - 1.0));
+  // the corrected stiffened gas takes pressure ratios of p + p_inf
+  const double rl = el.corr ? (p_star + el.p_inf) / (pl + el.p_inf) : p_star / pl;
+  const double rr = er.corr ? (p_star + er.p_inf) / (pr + er.p_inf) : p_star / pr;
+  const double wsl = (p_star <= pl) ? vl + -al : vl + -al * sqrt(1.0 + ql * (rl - 1.0));
+  const double wsr = (p_star <= pr) ? vr + ar : vr + ar * sqrt(1.0 + qr * (rr - 1.0));
   const double factorl = ul[0] * (wsl - vl);
   const double factorr = ur[0] * (wsr - vr);
   const double ws_star = (pr - pl + vl * factorl - vr * factorr) / (factorl - factorr);

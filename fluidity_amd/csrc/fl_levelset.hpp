@@ -51,8 +51,8 @@ struct EikJobs {
 
 struct Eos {
   double gamma, p_inf;
-  int stiff;
-  int pad;
+  int stiff;  // either stiffened gas (energy and sound speed with p_inf)
+  int corr;   // corrected stiffened gas: pressure minus gamma p_inf
 };
 
 __global__ void masks_kernel(const double* phi, Grid g, double dh, uint8_t* masks);

@@ -64,8 +64,8 @@ ls::Eos eos_of(const fl_scheme* s) {
   ls::Eos e;
   e.gamma = s->gamma;
   e.p_inf = s->p_inf;
-  e.stiff = s->eos == FL_EOS_STIFF ? 1 : 0;
-  e.pad = 0;
+  e.stiff = (s->eos == FL_EOS_STIFF || s->eos == FL_EOS_STIFF_CORRECTED) ? 1 : 0;
+  e.corr = s->eos == FL_EOS_STIFF_CORRECTED ? 1 : 0;
   return e;
 }
 
@@ -545,6 +545,7 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
       return fail(FL_ERR_ARGUMENT, "mats[%d]: work must be a distinct buffer of the same layout", m);
     if (!mats[m].phi) return fail(FL_ERR_ARGUMENT, "mats[%d].phi is NULL", m);
     if ((rc = check_scheme(&mats[m].scheme, g.nd))) return rc;
+    if ((rc = require_exact(&mats[m].scheme, "the two-material step"))) return rc;
   }
   if (!(p->cfl > 0.0) || !isfinite(p->cfl)) return fail(FL_ERR_ARGUMENT, "cfl must be > 0");
   if ((rc = check_dh(p->dh))) return rc;

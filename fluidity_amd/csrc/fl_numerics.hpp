@@ -22,7 +22,7 @@ namespace fl {
 enum : int { kLimVanLeer = 0, kLimSuperbee = 1 };
 enum : int { kFormCons = 0, kFormPrim = 1 };
 enum : int { kFluxHllc = 0, kFluxForce = 1, kFluxLaxFriedrichs = 2, kFluxRichtmyer = 3 };
-enum : int { kEosIdeal = 0, kEosStiff = 1 };
+enum : int { kEosIdeal = 0, kEosStiff = 1, kEosStiffCorrected = 2 };
 enum : int { kBcTransmissive = 0, kBcReflective = 1, kBcPeriodic = 2 };
 
 template <typename T> struct Limits;
@@ -88,7 +88,11 @@ template <typename T> struct Material {
   T p_inf;   // stiffened-gas constant
   T gm1;     // gamma - 1           (state/state_impl.hpp:125)
   T q;       // (gamma+1)/(2 gamma) (flux_method/flux_hllc.hpp:64-67)
-  int stiff; // kEosStiff selects StiffFluid::sound_speed
+  T gpinf;   // gamma p_inf (corrected stiffened gas)
+  int stiff; // either stiffened gas: StiffFluid::sound_speed and eos
+  int corr;  // kEosStiffCorrected: the pressure of the stiffened-gas
+             // energy, p = (gamma-1)(E - rho |v|^2 / 2) - gamma p_inf, and
+             // wave-speed ratios (p* + p_inf)/(p + p_inf)
 };
 
 template <typename T, int ND> struct Cell { T u[2 + ND]; };
@@ -110,6 +114,7 @@ __device__ __forceinline__ Prim<T, ND> prim_of(const T* U, const Material<T>& m)
     sum += w.vel[i] * w.vel[i];
   }
   w.p = m.gm1 * (U[1] - T(0.5) * U[0] * sum);
+  if (m.corr) w.p = w.p - m.gpinf;
   return w;
 }
 
@@ -161,13 +166,17 @@ __device__ __forceinline__ T limit_single(int limiter, T left, T right) {
 
 // MHReconstructor::chi's flattening factor about a cell in one dimension
 // (reconstruction/muscl_reconstructor.hpp:59-87), from the pressures at
-// offsets -2,-1,+1,+2 and the velocities at -1,+1 in that dimension.
+// offsets -2,-1,+1,+2 and the velocities at -1,+1 in that dimension.  The
+// corrected stiffened gas measures the jump against min(p) + p_inf (its
+// shock relations are the ideal gas's in p + p_inf).
 template <typename T>
 __device__ __forceinline__ T chi_factor(T p_pprev, T p_prev, T p_next, T p_nnext,
-                                        T v_prev, T v_next) {
+                                        T v_prev, T v_next, const Material<T>& m) {
   const T u_factor = v_prev - v_next;
   const T dp = fabs(p_next - p_prev);
-  const bool flatten = u_factor > T(0) && dv(dp, smin(p_next, p_prev)) > T(0.33);
+  T pmin = smin(p_next, p_prev);
+  if (m.corr) pmin = pmin + m.p_inf;
+  const bool flatten = u_factor > T(0) && dv(dp, pmin) > T(0.33);
   if (!flatten) return T(1);
   const T z = dv(dp, T(fabs(p_nnext - p_pprev)));
   const T temp = T(1) - dv(z - T(0.75), T(0.85) - T(0.75));
@@ -227,11 +236,14 @@ __device__ __forceinline__ void evolve_faces(const T* um, const T* u0, const T* 
   }
 }
 
-// Hllc::Impl::wavespeed (flux_method/flux_hllc.hpp:82-99).
+// Hllc::Impl::wavespeed (flux_method/flux_hllc.hpp:82-99).  The corrected
+// stiffened gas takes the pressure ratio of p + p_inf (its shock relations
+// are the ideal gas's in p + p_inf).
 template <typename T>
-__device__ __forceinline__ T hllc_wavespeed(T u, T p, T pstar, T sound, T q) {
+__device__ __forceinline__ T hllc_wavespeed(T u, T p, T pstar, T sound, const Material<T>& m) {
   if (pstar <= p) return u + sound;
-  return u + sound * sq(T(1) + q * (dv(pstar, p) - T(1)));
+  if (m.corr) return u + sound * sq(T(1) + m.q * (dv(pstar + m.p_inf, p + m.p_inf) - T(1)));
+  return u + sound * sq(T(1) + m.q * (dv(pstar, p) - T(1)));
 }
 
 // Hllc::Impl::operator() (flux_method/flux_hllc.hpp:202-245) with
@@ -246,14 +258,15 @@ __device__ __forceinline__ void hllc_flux(const T* ul, const T* ur, const Materi
   const T ar = sound_speed(wr.p, ur[0], m);
   const T vl = wl.vel[D], vr = wr.vel[D];
   const T p_pvrs = T(0.5) * ((wl.p + wr.p) - (T(0.25) * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
-  const T p_star = smax(T(0), p_pvrs);
+  // the corrected stiffened gas bounds p* by -p_inf (p + p_inf >= 0)
+  const T p_star = m.corr ? smax(-m.p_inf, p_pvrs) : smax(T(0), p_pvrs);
 
-  const T wsl = hllc_wavespeed(vl, wl.p, p_star, -al, m.q);
+  const T wsl = hllc_wavespeed(vl, wl.p, p_star, -al, m);
   if (T(0) <= wsl) {
     euler_flux<T, ND, D>(ul, wl, F);
     return;
   }
-  const T wsr = hllc_wavespeed(vr, wr.p, p_star, ar, m.q);
+  const T wsr = hllc_wavespeed(vr, wr.p, p_star, ar, m);
   if (T(0) >= wsr) {
     euler_flux<T, ND, D>(ur, wr, F);
     return;

@@ -28,20 +28,20 @@ __device__ __forceinline__ void tile_of_block(int64_t& bx, int64_t& by) {
 #endif
 }
 
-template <typename T, int ND, int FLUX>
+template <typename T, int ND, int FLUX, bool FAST>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_x_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_x_tile<T, ND, FLUX>(a, bx, by);
+  sweep_x_tile<T, ND, FLUX, FAST>(a, bx, by);
 }
 
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool FAST>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_y_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_y_tile<T, FLUX>(a, bx, by);
+  sweep_y_tile<T, FLUX, FAST>(a, bx, by);
 }
 
 // The same sweeps over two grids of one shape in one launch (the two
@@ -264,19 +264,16 @@ template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const d
                                                           Material<double>, Material<double>,
                                                           unsigned long long*);
 
+#define FL_SWEEPS(T, FLUX, FAST)                                                 \
+  template __global__ void sweep_x_kernel<T, 1, FLUX, FAST>(SweepArgs<T>);       \
+  template __global__ void sweep_x_kernel<T, 2, FLUX, FAST>(SweepArgs<T>);       \
+  template __global__ void sweep_y_kernel<T, FLUX, FAST>(SweepArgs<T>);
+
 #define FL_INST_SWEEPS(T)                                                      \
-  template __global__ void sweep_x_kernel<T, 1, kFluxHllc>(SweepArgs<T>);      \
-  template __global__ void sweep_x_kernel<T, 1, kFluxForce>(SweepArgs<T>);     \
-  template __global__ void sweep_x_kernel<T, 2, kFluxHllc>(SweepArgs<T>);      \
-  template __global__ void sweep_x_kernel<T, 2, kFluxForce>(SweepArgs<T>);     \
-  template __global__ void sweep_y_kernel<T, kFluxHllc>(SweepArgs<T>);         \
-  template __global__ void sweep_y_kernel<T, kFluxForce>(SweepArgs<T>);        \
-  template __global__ void sweep_x_kernel<T, 1, kFluxLaxFriedrichs>(SweepArgs<T>); \
-  template __global__ void sweep_x_kernel<T, 2, kFluxLaxFriedrichs>(SweepArgs<T>); \
-  template __global__ void sweep_y_kernel<T, kFluxLaxFriedrichs>(SweepArgs<T>);    \
-  template __global__ void sweep_x_kernel<T, 1, kFluxRichtmyer>(SweepArgs<T>);     \
-  template __global__ void sweep_x_kernel<T, 2, kFluxRichtmyer>(SweepArgs<T>);     \
-  template __global__ void sweep_y_kernel<T, kFluxRichtmyer>(SweepArgs<T>);        \
+  FL_SWEEPS(T, kFluxHllc, false)                                                          \
+  FL_SWEEPS(T, kFluxForce, false)                                                         \
+  FL_SWEEPS(T, kFluxLaxFriedrichs, false)                                                 \
+  FL_SWEEPS(T, kFluxRichtmyer, false)                                                     \
   template __global__ void max_wavespeed_kernel<T, 1>(const T*, int64_t, int64_t, int64_t, \
                                                       Material<double>, unsigned long long*); \
   template __global__ void max_wavespeed_kernel<T, 2>(const T*, int64_t, int64_t, int64_t, \
@@ -290,5 +287,10 @@ template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const d
 
 FL_INST_SWEEPS(double)
 FL_INST_SWEEPS(float)
+// the fast FP64 numerics (FP32 sweeps always use them)
+FL_SWEEPS(double, kFluxHllc, true)
+FL_SWEEPS(double, kFluxForce, true)
+FL_SWEEPS(double, kFluxLaxFriedrichs, true)
+FL_SWEEPS(double, kFluxRichtmyer, true)
 
 }  // namespace fl

@@ -25,7 +25,10 @@
 // so a NaN anywhere surfaces as a NaN max).
 #pragma once
 
+#include <type_traits>
+
 #include "fl_kernels.hpp"
+#include "fl_numerics_fast.hpp"
 
 // Waves per SIMD the sweep kernels are compiled for (register budget
 // 512 / waves VGPRs), per precision.
@@ -177,6 +180,51 @@ __device__ __forceinline__ double cell_wavespeed64(const T* u, const Material<do
   return max_wavespeed<double, ND>(ud, w, m);
 }
 
+// The numerics a sweep runs: the reference's operation order (F = false,
+// FP64 bit parity) or the fast forms of fl_numerics_fast.hpp (FP64 on
+// request, FP32 always).
+template <typename T, bool FAST>
+struct UseFast { static constexpr bool value = FAST || std::is_same<T, float>::value; };
+
+template <bool F, typename T, int ND>
+__device__ __forceinline__ Prim<T, ND> prim_sel(const T* U, const Material<T>& m) {
+  if constexpr (F) return fast::as_prim(fast::prim_of<T, ND>(U, m));
+  else return prim_of<T, ND>(U, m);
+}
+
+template <bool F, typename T>
+__device__ __forceinline__ T chi_sel(T a, T b, T c, T d, T e, T f, const Material<T>& m) {
+  if constexpr (F) return fast::chi_factor(a, b, c, d, e, f, m);
+  else return chi_factor(a, b, c, d, e, f, m);
+}
+
+template <bool F, typename T, int ND, int D>
+__device__ __forceinline__ void evolve_sel(const T* um, const T* u0, const T* up, T chi,
+                                           T half_dtdh, int limiter, int form,
+                                           const Material<T>& m, T* R, T* L) {
+  if constexpr (F) fast::evolve_faces<T, ND, D>(um, u0, up, chi, half_dtdh, limiter, form, m, R, L);
+  else evolve_faces<T, ND, D>(um, u0, up, chi, half_dtdh, limiter, form, m, R, L);
+}
+
+template <bool F, typename T, int ND, int D, int FLUX>
+__device__ __forceinline__ void flux_sel(const T* ul, const T* ur, T dtdh, T dhdt,
+                                         const Material<T>& m, T* G) {
+  if constexpr (F) fast::numerical_flux<T, ND, D, FLUX>(ul, ur, dtdh, dhdt, m, G);
+  else numerical_flux<T, ND, D, FLUX>(ul, ur, dtdh, dhdt, m, G);
+}
+
+// Wavespeed of an updated cell for the fused CFL epilogue, as an FP64 value
+// for the FP64 reduction: exact FP64, fast FP64, or computed in FP32 from an
+// FP32 cell.
+template <bool F, typename T, int ND>
+__device__ __forceinline__ double cell_ws(const T* u, const SweepArgs<T>& a) {
+  if constexpr (!F) {
+    return cell_wavespeed64<T, ND>(u, a.mat64);
+  } else {
+    return (double)fast::max_wavespeed<T, ND>(fast::prim_of<T, ND>(u, a.mat), a.mat);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
   if (a.ctl) {
@@ -197,9 +245,10 @@ __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
 // is rows_per_block consecutive segments of B-2 cells, starting at segment
 // bx rows_per_block, marched like rows.  Every thread of the workgroup must
 // call it with the same arguments.
-template <typename T, int ND, int FLUX>
+template <typename T, int ND, int FLUX, bool FAST = false>
 __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int S = 2 + ND;
+  constexpr bool F = UseFast<T, FAST>::value;
   constexpr int W = kMaxBlock + 2 * kHalo;
   __shared__ T sU[S][W];
   __shared__ T sP[W];
@@ -294,7 +343,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     take<T, ND>(H, uh);
     issue_load<T, ND>(a, hcol + rn * cstep, hrow(rn), H);
     {
-      const Prim<T, ND> w = prim_of<T, ND>(Uw[3], a.mat);
+      const Prim<T, ND> w = prim_sel<F, T, ND>(Uw[3], a.mat);
       pw[4] = w.p;
       vxw[3] = w.vel[0];
       vyw[3] = w.vel[ND - 1];
@@ -303,7 +352,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       if (r >= y0 + 1) {  // K_y of row r-2 (needs rows r-4..r)
         ky0 = ky1;
         ky1 = ky2;
-        ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
+        ky2 = chi_sel<F>(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2], a.mat);
       }
       if (r < y0 + 3) continue;  // block-uniform: warm-up rows
     }
@@ -318,7 +367,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     sP[s] = p0;
     sV[s] = vx0;
     if (t < 2 * kHalo) {
-      const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
+      const Prim<T, ND> wh = prim_sel<F, T, ND>(uh, a.mat);
 #pragma unroll
       for (int k = 0; k < S; ++k) sU[k][hslot] = uh[k];
       sP[hslot] = wh.p;
@@ -328,10 +377,11 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 
     // ---- phase B: x flattening factors K_x (own column, plus the two
     // columns just outside the lane range)
-    sK[s] = chi_factor(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1]);
+    sK[s] = chi_sel<F>(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1], a.mat);
     if (t < 2) {
       const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
-      sK[hs] = chi_factor(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1], sV[hs + 1]);
+      sK[hs] = chi_sel<F>(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1],
+                         sV[hs + 1], a.mat);
     }
     const int sx = signum(sP[s + 1] - sP[s - 1]);
     T um[S], up[S];
@@ -350,7 +400,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       chi = smin(chi, smin(ky1, ky_nb));
     }
     T R[S], L[S];
-    evolve_faces<T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
+    evolve_sel<F, T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
 #pragma unroll
     for (int k = 0; k < S; ++k) sL[k][t] = L[k];
     __syncthreads();
@@ -361,7 +411,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       T Lr[S];
 #pragma unroll
       for (int k = 0; k < S; ++k) Lr[k] = sL[k][t + 1];
-      numerical_flux<T, ND, 0, FLUX>(R, Lr, dtdh, dhdt, a.mat, G);
+      flux_sel<F, T, ND, 0, FLUX>(R, Lr, dtdh, dhdt, a.mat, G);
 #pragma unroll
       for (int k = 0; k < S; ++k) sG[k][t] = G[k];
     }
@@ -375,7 +425,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k][t - 1] - G[k]) * dtdh;
       store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
       if (a.ws_out) {
-        const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(o, a.mat64));
+        const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
         wsmax = b > wsmax ? b : wsmax;
       }
     }
@@ -390,9 +440,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 
 // One y-sweep tile: columns [bx B, (bx+1) B) of rows [by rows_per_block,
 // (by+1) rows_per_block), B = blockDim.x.  Workgroup-uniform arguments.
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool FAST = false>
 __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int ND = 2, S = 4;
+  constexpr bool F = UseFast<T, FAST>::value;
   constexpr int W = kMaxBlock + 2 * kHalo;
   __shared__ T sP[W];
   __shared__ T sV[W];
@@ -469,7 +520,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
     T uh[S];
     take<T, ND>(H, uh);
     issue_load<T, ND>(a, hcol, min(r + PF, rlast), H);
-    const Prim<T, ND> w = prim_of<T, ND>(Uw[4], a.mat);
+    const Prim<T, ND> w = prim_sel<F, T, ND>(Uw[4], a.mat);
     pw[4] = w.p;
     vyw[3] = w.vel[1];
 
@@ -478,15 +529,16 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       sP[s] = w.p;
       sV[s] = w.vel[0];
       if (t < 2 * kHalo) {
-        const Prim<T, ND> wh = prim_of<T, ND>(uh, a.mat);
+        const Prim<T, ND> wh = prim_sel<F, T, ND>(uh, a.mat);
         sP[hslot] = wh.p;
         sV[hslot] = wh.vel[0];
       }
       __syncthreads();
-      sK[s] = chi_factor(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1]);
+      sK[s] = chi_sel<F>(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1], a.mat);
       if (t < 2) {
         const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
-        sK[hs] = chi_factor(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1], sV[hs + 1]);
+        sK[hs] = chi_sel<F>(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1],
+                         sV[hs + 1], a.mat);
       }
       const int sx = signum(sP[s + 1] - sP[s - 1]);
       __syncthreads();
@@ -497,7 +549,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
     if (r >= y0) {
       ky0 = ky1;
       ky1 = ky2;
-      ky2 = chi_factor(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2]);
+      ky2 = chi_sel<F>(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2], a.mat);
     }
 
     // ---- face states of row jr = r-3
@@ -507,12 +559,12 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
       chi = smin(chi, smin(ky1, ky_nb));
       T R[S], L[S];
-      evolve_faces<T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, a.limiter, a.form, a.mat,
+      evolve_sel<F, T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, a.limiter, a.form, a.mat,
                              R, L);
       if (r >= y0 + 3) {
         // face between rows jr-1 and jr
         T G[S];
-        numerical_flux<T, ND, 1, FLUX>(Rprev, L, dtdh, dhdt, a.mat, G);
+        flux_sel<F, T, ND, 1, FLUX>(Rprev, L, dtdh, dhdt, a.mat, G);
         const int64_t jo = r - 4;
         if (jo >= y0 && c < a.nx) {
           T o[S];
@@ -520,7 +572,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
           for (int k = 0; k < S; ++k) o[k] = Uw[0][k] + (Gprev[k] - G[k]) * dtdh;
           store_cell<T, S>(a.out, jo * a.nx + c, a.es, a.cs, o);
           if (a.ws_out) {
-            const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(o, a.mat64));
+            const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
             wsmax = b > wsmax ? b : wsmax;
           }
         }

@@ -59,10 +59,13 @@ class Scheme:
     gamma: float = 1.4
     p_inf: float = 0.0
     bc: tuple = (N.FL_BC_TRANSMISSIVE, N.FL_BC_TRANSMISSIVE, N.FL_BC_TRANSMISSIVE)
+    # FP64 arithmetic: N.FL_NUMERICS_EXACT (the reference's bits) or
+    # N.FL_NUMERICS_FAST (within 1e-12 relative; an extension)
+    numerics: int = N.FL_NUMERICS_EXACT
 
     def c(self) -> N.Scheme:
         return N.scheme(self.limiter, self.form, self.flux, self.eos, self.gamma, self.p_inf,
-                        self.bc)
+                        self.bc, self.numerics)
 
 
 def autotune_context(ctx: N.Context, U, work, scheme: "Scheme", layout: int, reps: int = 2,

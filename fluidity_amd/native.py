@@ -21,7 +21,8 @@ FL_LAYOUT_AOS, FL_LAYOUT_SOA = 0, 1
 FL_LIMITER_VAN_LEER, FL_LIMITER_SUPERBEE = 0, 1
 FL_FORM_CONSERVATIVE, FL_FORM_PRIMITIVE = 0, 1
 FL_FLUX_HLLC, FL_FLUX_FORCE, FL_FLUX_LAX_FRIEDRICHS, FL_FLUX_RICHTMYER = 0, 1, 2, 3
-FL_EOS_IDEAL, FL_EOS_STIFF = 0, 1
+FL_EOS_IDEAL, FL_EOS_STIFF, FL_EOS_STIFF_CORRECTED = 0, 1, 2
+FL_NUMERICS_EXACT, FL_NUMERICS_FAST = 0, 1
 FL_BC_TRANSMISSIVE, FL_BC_REFLECTIVE, FL_BC_PERIODIC = 0, 1, 2
 
 EXPORTED = [
@@ -70,7 +71,7 @@ class Scheme(C.Structure):
         ("gamma", C.c_double),
         ("p_inf", C.c_double),
         ("bc", C.c_int32 * 3),
-        ("reserved", C.c_int32),
+        ("numerics", C.c_int32),
     ]
 
 
@@ -214,9 +215,11 @@ def check(status: int) -> None:
 
 
 def scheme(limiter=FL_LIMITER_VAN_LEER, form=FL_FORM_CONSERVATIVE, flux=FL_FLUX_HLLC,
-           eos=FL_EOS_IDEAL, gamma=1.4, p_inf=0.0, bc=(0, 0, 0)) -> Scheme:
+           eos=FL_EOS_IDEAL, gamma=1.4, p_inf=0.0, bc=(0, 0, 0),
+           numerics=FL_NUMERICS_EXACT) -> Scheme:
     s = Scheme()
     s.limiter, s.form, s.flux, s.eos = limiter, form, flux, eos
+    s.numerics = numerics
     s.gamma, s.p_inf = gamma, p_inf
     for i in range(3):
         s.bc[i] = bc[i] if i < len(bc) else 0

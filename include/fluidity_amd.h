@@ -44,7 +44,28 @@ typedef enum fl_flux {
   FL_FLUX_LAX_FRIEDRICHS = 2,  /* flux_method/flux_lax_friedrichs.hpp (single grids) */
   FL_FLUX_RICHTMYER = 3        /* flux_method/flux_richtmyer.hpp (single grids) */
 } fl_flux;
-typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1 } fl_eos;
+/* FL_EOS_STIFF is the reference's StiffFluid as it computes: its energy
+ * e = (p + gamma p_inf)/((gamma - 1) rho) and sound speed
+ * sqrt(gamma (p + p_inf)/rho) (material/stiff_fluid.hpp:77-98) with the
+ * ideal-gas pressure (gamma - 1)(E - rho |v|^2/2), which ignores p_inf
+ * (state/state_impl.hpp:117-128): the bit-parity default.
+ * FL_EOS_STIFF_CORRECTED takes the pressure consistent with that energy,
+ * p = (gamma - 1)(E - rho |v|^2/2) - gamma p_inf, and HLLC wave-speed ratios
+ * (p* + p_inf)/(p_K + p_inf): the stiffened gas of physical runs (an
+ * extension). */
+typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1, FL_EOS_STIFF_CORRECTED = 2 } fl_eos;
+/* Arithmetic of the FP64 sweeps.  EXACT keeps the reference's operation
+ * order with correctly rounded division and square root: bit-identical to
+ * the reference's results.  FAST forms each state's 1/rho once, takes
+ * quotients as products with refined reciprocals, uses the closed form of
+ * the van Leer slope (minmod) and contracts multiply-adds; it is held to
+ * 1e-12 relative L-inf of the reference over a 100-step run.  FP32 sweeps
+ * use the fast forms whatever this says.  The single-grid entry points
+ * (fl_split_sweep/solve/step, fl_simulate, fl_ctx_autotune) honour it; the
+ * batched, two-material and adjoint entry points run FP64 exact and reject
+ * FAST with FL_ERR_UNSUPPORTED.  An extension: the reference has one
+ * arithmetic. */
+typedef enum fl_numerics { FL_NUMERICS_EXACT = 0, FL_NUMERICS_FAST = 1 } fl_numerics;
 typedef enum fl_boundary {
   FL_BC_TRANSMISSIVE = 0,  /* solver/boundary_loader.hpp:29-32 */
   FL_BC_REFLECTIVE = 1,
@@ -73,7 +94,7 @@ typedef struct fl_scheme {
   double  gamma;     /* adiabatic index (material/ideal_gas.hpp:118) */
   double  p_inf;     /* stiffened-gas constant (material/stiff_fluid.hpp:132) */
   int32_t bc[3];     /* fl_boundary per dimension, both ends (BoundarySetter) */
-  int32_t reserved;
+  int32_t numerics;  /* fl_numerics (0 = exact) */
 } fl_scheme;
 
 typedef struct fl_sim_params {

@@ -82,22 +82,29 @@ static inline double vss_prim(const double* W, int nd) {
 /* velocity, conservative overload (state/state_impl.hpp:92-98). */
 static inline double vel_cons(const double* U, int d) { return U[2 + d] / U[0]; }
 
+static inline int is_stiff(const orc_scheme* s) {
+  return s->eos == ORC_EOS_STIFF || s->eos == ORC_EOS_STIFF_CORRECTED;
+}
+
 /* pressure, conservative overload (state/state_impl.hpp:117-128).  Uses the
- * adiabatic index only, also for the stiffened gas. */
+ * adiabatic index only, also for the stiffened gas; the corrected stiffened
+ * gas subtracts gamma p_inf (the inverse of StiffFluid::eos). */
 static inline double pres_cons(const double* U, int nd, const orc_scheme* s) {
-  return (s->gamma - 1.0) * (U[1] - 0.5 * U[0] * vss_cons(U, nd));
+  const double p = (s->gamma - 1.0) * (U[1] - 0.5 * U[0] * vss_cons(U, nd));
+  if (s->eos == ORC_EOS_STIFF_CORRECTED) return p - s->gamma * s->p_inf;
+  return p;
 }
 
 /* IdealGas::sound_speed / StiffFluid::sound_speed. */
 static inline double sound_from_p(double p, double rho, const orc_scheme* s) {
-  if (s->eos == ORC_EOS_STIFF)
+  if (is_stiff(s))
     return sqrt(s->gamma * (p + s->p_inf) / rho);   /* stiff_fluid.hpp:95-98 */
   return sqrt(s->gamma * p / rho);                   /* ideal_gas.hpp:75-78 */
 }
 
 /* IdealGas::eos / StiffFluid::eos (specific internal energy). */
 static inline double eos_energy(double p, double rho, const orc_scheme* s) {
-  if (s->eos == ORC_EOS_STIFF)
+  if (is_stiff(s))
     return (p + s->gamma * s->p_inf) / ((s->gamma - 1.0) * rho);
   return p / ((s->gamma - 1.0) * rho);
 }
@@ -224,10 +231,13 @@ static void limit_state(const double* um, const double* u0, const double* up,
 /* numerical fluxes                                                           */
 /* ------------------------------------------------------------------------ */
 
-/* Hllc::Impl::wavespeed (flux_method/flux_hllc.hpp:82-99). */
+/* Hllc::Impl::wavespeed (flux_method/flux_hllc.hpp:82-99); the corrected
+ * stiffened gas takes the ratio of p + p_inf. */
 static inline double hllc_wavespeed(double u, double p, double pstar,
-                                    double sound_speed, double adi_scale) {
+                                    double sound_speed, double adi_scale, const orc_scheme* s) {
   if (pstar <= p) return u + sound_speed;
+  if (s->eos == ORC_EOS_STIFF_CORRECTED)
+    return u + sound_speed * sqrt(1.0 + adi_scale * ((pstar + s->p_inf) / (p + s->p_inf) - 1.0));
   return u + sound_speed * sqrt(1.0 + adi_scale * (pstar / p - 1.0));
 }
 
@@ -255,14 +265,16 @@ static void flux_hllc(const double* ul, const double* ur, int nd,
   const double vl = vel_cons(ul, dim), vr = vel_cons(ur, dim);
 
   const double p_pvrs = 0.5 * ((pl + pr) - (0.25 * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
-  const double p_star = smax(0.0, p_pvrs);
+  /* the corrected stiffened gas bounds p* by -p_inf */
+  const double p_star = s->eos == ORC_EOS_STIFF_CORRECTED ? smax(-s->p_inf, p_pvrs)
+                                                          : smax(0.0, p_pvrs);
 
-  const double wsl = hllc_wavespeed(vl, pl, p_star, -al, adi);
+  const double wsl = hllc_wavespeed(vl, pl, p_star, -al, adi, s);
   double fl[ORC_MAXS];
   orc_flux(ul, nd, 0, s, dim, fl);
   if (0.0 <= wsl) { memcpy(F, fl, sizeof(double) * S); return; }
 
-  const double wsr = hllc_wavespeed(vr, pr, p_star, ar, adi);
+  const double wsr = hllc_wavespeed(vr, pr, p_star, ar, adi, s);
   double fr[ORC_MAXS];
   orc_flux(ur, nd, 0, s, dim, fr);
   if (0.0 >= wsr) { memcpy(F, fr, sizeof(double) * S); return; }
@@ -377,8 +389,10 @@ static double chi_at(const grid_t* g, int64_t x, int64_t y, int d) {
   const double p_pprev = grid_pressure(g, x - 2 * dx, y - 2 * dy);
   const double u_factor = grid_velocity(g, x - dx, y - dy, d) -
                           grid_velocity(g, x + dx, y + dy, d);
-  const int flatten = u_factor > 0.0 &&
-                      (fabs(p_next - p_prev) / smin(p_next, p_prev)) > 0.33;
+  /* the corrected stiffened gas measures the jump against min(p) + p_inf */
+  const double pmin = g->s->eos == ORC_EOS_STIFF_CORRECTED ? smin(p_next, p_prev) + g->s->p_inf
+                                                           : smin(p_next, p_prev);
+  const int flatten = u_factor > 0.0 && (fabs(p_next - p_prev) / pmin) > 0.33;
   if (!flatten) return 1.0;
   const double z = fabs(p_next - p_prev) / fabs(p_nnext - p_pprev);
   const double temp = 1.0 - ((z - 0.75) / (0.85 - 0.75));

@@ -25,7 +25,11 @@ extern "C" {
 enum { ORC_LIM_VANLEER = 0, ORC_LIM_SUPERBEE = 1 };
 enum { ORC_FORM_CONS = 0, ORC_FORM_PRIM = 1 };
 enum { ORC_FLUX_HLLC = 0, ORC_FLUX_FORCE = 1, ORC_FLUX_LF = 2, ORC_FLUX_RICHTMYER = 3 };
-enum { ORC_EOS_IDEAL = 0, ORC_EOS_STIFF = 1 };
+/* ORC_EOS_STIFF: the reference's StiffFluid with its pressure, which ignores
+ * p_inf.  ORC_EOS_STIFF_CORRECTED: the pressure of the stiffened-gas energy,
+ * p = (gamma-1)(E - rho |v|^2/2) - gamma p_inf, and HLLC pressure ratios of
+ * p + p_inf (an extension beside the parity default). */
+enum { ORC_EOS_IDEAL = 0, ORC_EOS_STIFF = 1, ORC_EOS_STIFF_CORRECTED = 2 };
 enum { ORC_BC_TRANSMISSIVE = 0, ORC_BC_REFLECTIVE = 1, ORC_BC_PERIODIC = 2 };
 
 typedef struct orc_scheme {

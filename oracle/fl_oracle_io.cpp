@@ -24,7 +24,7 @@ const char* const kNames[] = {"rho", "p", "v_x", "v_y"};
 // IdealGas::eos (material/ideal_gas.hpp:60-63) / StiffFluid::eos
 // (material/stiff_fluid.hpp:77-80) of a primitive state.
 double eos_of(const double* W, const orc_scheme* s) {
-  if (s->eos == ORC_EOS_STIFF) return (W[1] + s->gamma * s->p_inf) / ((s->gamma - 1.0) * W[0]);
+  if (s->eos == ORC_EOS_STIFF || s->eos == ORC_EOS_STIFF_CORRECTED) return (W[1] + s->gamma * s->p_inf) / ((s->gamma - 1.0) * W[0]);
   return W[1] / ((s->gamma - 1.0) * W[0]);
 }
 

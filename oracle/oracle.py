@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 LIM_VANLEER, LIM_SUPERBEE = 0, 1
 FORM_CONS, FORM_PRIM = 0, 1
 FLUX_HLLC, FLUX_FORCE, FLUX_LF, FLUX_RICHTMYER = 0, 1, 2, 3
-EOS_IDEAL, EOS_STIFF = 0, 1
+EOS_IDEAL, EOS_STIFF, EOS_STIFF_CORRECTED = 0, 1, 2
 BC_TRANSMISSIVE, BC_REFLECTIVE, BC_PERIODIC = 0, 1, 2
 
 

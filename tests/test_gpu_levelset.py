@@ -57,8 +57,17 @@ def _bubble(nx, ny, stiff=False):
     return W.shock_bubble(nx, ny, seed=1002, stiff=stiff)
 
 
+def _eos1(sb):
+    """Equation of state of the bubble: 0 ideal, 1 the reference's stiffened
+    gas, 2 the corrected stiffened gas (W.shock_bubble(..., stiff=True) with
+    .corrected set)."""
+    if not sb.stiff:
+        return 0
+    return 2 if getattr(sb, "corrected", False) else 1
+
+
 def _schemes(O, E, sb):
-    eos1 = 1 if sb.stiff else 0
+    eos1 = _eos1(sb)
     os0 = O.scheme(gamma=sb.gamma0)
     os1 = O.scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
     es0 = E.Scheme(gamma=sb.gamma0)
@@ -202,7 +211,7 @@ def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8, every=1):
 
 
 def _schemes_e(E, sb):
-    eos1 = 1 if sb.stiff else 0
+    eos1 = _eos1(sb)
     return E.Scheme(gamma=sb.gamma0), E.Scheme(gamma=sb.gamma1, eos=eos1, p_inf=sb.p_inf1)
 
 
@@ -342,3 +351,31 @@ def test_mm_simulate_full_size(L, E, oracle):
     area0 = (sb.phi1 <= 0).sum()
     area = (p1 <= 0).sum()
     assert abs(area - area0) / area0 < 0.05
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_mm_simulate_corrected_stiff_bubble(L, E, oracle, layout):
+    """The shock-bubble workload's stiffened-gas variant (gamma 4.4,
+    p_inf 6e8, water-like bubble at pressure equilibrium with the ambient
+    gas) with the corrected stiffened gas: the whole two-material loop
+    (reinitialisation every 5 steps) bit-identical to the oracle's corrected
+    loop, and the bubble's pressure stays at the ambient 1e5 where the
+    reference's pressure would read p + gamma p_inf."""
+    import dataclasses
+    sb = dataclasses.replace(_bubble(256, 128, True))
+    sb.corrected = True
+    os0, os1, _, _ = _schemes(oracle, E, sb)
+    assert os1.eos == 2
+    steps = 10
+    r, U0, U1, p0, p1 = _run_mm(L, E, sb, steps, layout, every=5)
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, os0, os1, 0.5, sb.dh,
+                           max_iters=steps, reinit_every=5)
+    assert r.iters == R[5] == steps
+    assert r.run_time == R[4]
+    _same(U0, R[0])
+    _same(U1, R[1])
+    _same(p0, R[2])
+    _same(p1, R[3])
+    inside = sb.phi1 < -4 * sb.dh
+    p_bubble = np.array([oracle.pressure(u, os1) for u in sb.U1[inside][:50]])
+    np.testing.assert_allclose(p_bubble, 1e5, rtol=1e-6)

@@ -216,13 +216,15 @@ def test_solve_swap_semantics(E, oracle):
     assert _bitwise(_host(new_out), ref)
 
 
-def test_fp32_against_fp64_oracle(E, oracle):
+@pytest.mark.parametrize("flux", [0, 1], ids=["hllc", "force"])
+def test_fp32_against_fp64_oracle(E, oracle, flux):
     """FP32 bulk sweeps, FP64 dt: relative L1 density error <= 1e-5 after
-    100 steps on a 2D Riemann workload."""
+    100 steps on a 2D Riemann workload, with HLLC and with FORCE (the 4096^2
+    workload's two flux variants)."""
     U = cons_field("riemann", (128, 128), seed=1001)
-    os_ = _oscheme(oracle, 0, 0, 0, (0, 0))
+    os_ = _oscheme(oracle, 0, 0, flux, (0, 0))
     ref, rt, it, _ = oracle.simulate(U, os_, 0.9, 1.0 / 128, max_iters=100)
-    solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+    solver = E.SplitSolver(_escheme(E, 0, 0, flux, (0, 0)), layout=1)
     a = _dev(U, dtype=torch.float32, layout=1)
     w = torch.empty_like(a)
     r = solver.simulate(a, w, 0.9, 1.0 / 128, max_iters=100)
@@ -295,23 +297,30 @@ def test_headline_workload_full_size_bitwise(E, oracle):
     assert _bitwise(got, ref), _report(got, ref)
 
 
-def test_fp32_full_size_against_fp64(E):
+@pytest.mark.parametrize("flux", [0, 1], ids=["hllc", "force"])
+def test_fp32_full_size_against_fp64(E, flux):
     """FP32 bulk on the 4096^2 workload, 100 steps, against the FP64 device
-    loop (bit-identical to the oracle): relative L1 density error <= 1e-5."""
+    loop (bit-identical to the oracle): relative L1 density error <= 1e-5;
+    the relative L1 and L-inf error of every field is printed."""
     from fluidity_amd import workloads as W
     wl = W.riemann_2d(n=4096, seed=1001)
     out = {}
     for dt in (torch.float64, torch.float32):
-        solver = E.SplitSolver(_escheme(E, 0, 0, 0, (0, 0)), layout=1)
+        solver = E.SplitSolver(_escheme(E, 0, 0, flux, (0, 0)), layout=1)
         a = _dev(wl.U, dtype=dt, layout=1)
         w = torch.empty_like(a)
         r = solver.simulate(a, w, 0.9, wl.dh, max_iters=100)
         assert r.iters == 100
-        out[dt] = a[0].double()
+        out[dt] = a.double()
         del a, w
-    rel_l1 = float((out[torch.float32] - out[torch.float64]).abs().sum() /
-                   out[torch.float64].abs().sum())
-    print(f"4096^2 fp32 rel L1 density after 100 steps {rel_l1:.3e}", file=sys.stderr)
+    ref, got = out[torch.float64], out[torch.float32]
+    names = ["rho", "E", "m_x", "m_y"]
+    for k in range(4):
+        d = (got[k] - ref[k]).abs()
+        l1 = float(d.sum() / ref[k].abs().sum())
+        linf = float(d.max() / ref[k].abs().max())
+        print(f"4096^2 fp32 {names[k]}: rel L1 {l1:.3e}, rel Linf {linf:.3e}", file=sys.stderr)
+    rel_l1 = float((got[0] - ref[0]).abs().sum() / ref[0].abs().sum())
     assert rel_l1 <= 1e-5
     del out
     torch.cuda.empty_cache()
@@ -429,3 +438,143 @@ def test_simulator_continues_after_iteration_limit(E):
     assert r.iters == 0 and r.run_time == 0.0
     assert sim.params.iters == it and sim.params.run_time == t
     assert torch.equal(sim.state, state)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("shape", [(900,), (52, 60)])
+def test_simulate_corrected_stiff_bitwise(E, oracle, flux, shape):
+    """The corrected stiffened gas (pressure of the stiffened-gas energy,
+    ratios of p + p_inf) through the step loop: bit-identical to the
+    oracle's corrected loop, and different from the reference's pressure."""
+    g, pinf = 4.4, 6.0
+    nd = len(shape)
+    rng = np.random.default_rng(41)
+    n = int(np.prod(shape))
+    Wp = np.zeros((n, 2 + nd))
+    Wp[:, 0] = np.where(rng.uniform(size=n) < 0.5, 1.0, 3.0)
+    Wp[:, 1] = rng.uniform(0.5, 20.0, n)
+    Wp[:, 2:] = rng.uniform(-1.0, 1.0, (n, nd))
+    from fluidity_amd import workloads as W
+    U = W.prim_to_cons_eos(Wp, g, pinf, True).reshape(shape + (2 + nd,))
+    bc = (0,) * nd
+    os_ = _oscheme(oracle, 0, 0, flux, bc, gamma=g, eos=2, p_inf=pinf)
+    ref, rt, it, _ = oracle.simulate(U, os_, 0.8, 1.0 / shape[-1], max_iters=10)
+    solver = E.SplitSolver(_escheme(E, 0, 0, flux, bc, gamma=g, eos=2, p_inf=pinf))
+    a = _dev(U)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.8, 1.0 / shape[-1], max_iters=10)
+    got = _host(a)
+    assert r.iters == it and r.run_time == rt
+    assert _bitwise(got, ref), _report(got, ref)
+    quirk, _, _, _ = oracle.simulate(U, _oscheme(oracle, 0, 0, flux, bc, gamma=g, eos=1,
+                                                 p_inf=pinf), 0.8, 1.0 / shape[-1], max_iters=10)
+    assert np.abs(quirk - ref).max() > 1e-6
+
+
+
+# ---- fast FP64 numerics ---------------------------------------------------------------------
+
+def _rel_linf(got, ref):
+    """Per conserved field, max |got - ref| / max |ref| (fields last)."""
+    S = ref.shape[-1]
+    g = got.reshape(-1, S)
+    r = ref.reshape(-1, S)
+    return [float(np.abs(g[:, k] - r[:, k]).max() / np.abs(r[:, k]).max()) for k in range(S)]
+
+
+FAST_CASES = [
+    # kind, shape, limiter, form, flux, bc, layout
+    ("riemann", (160, 192), 0, 0, 0, (0, 0), 1),
+    ("riemann", (160, 192), 0, 0, 1, (0, 0), 1),
+    ("rough", (96, 80), 1, 0, 0, (1, 2), 0),
+    ("rough", (96, 80), 0, 1, 0, (2, 1), 1),
+    ("rough", (96, 80), 1, 1, 1, (0, 0), 0),
+    ("rough", (2000,), 0, 0, 0, (0,), 0),
+    ("rough", (2000,), 1, 1, 1, (2,), 1),
+]
+
+
+@pytest.mark.parametrize("case", FAST_CASES)
+def test_fast_numerics_against_oracle(E, oracle, case):
+    """FL_NUMERICS_FAST (one reciprocal per density, closed-form van Leer
+    slope, contracted multiply-adds) against the oracle's exact loop: every
+    field within 1e-12 relative L-inf after 100 steps, same step count and
+    run time within 1e-12."""
+    kind, shape, lim, form, flux, bc, layout = case
+    U = cons_field(kind, shape, seed=1001)
+    n = shape[-1]
+    ref, rt, it, _ = oracle.simulate(U, _oscheme(oracle, lim, form, flux, bc), 0.8, 1.0 / n,
+                                     max_iters=100)
+    sch = _escheme(E, lim, form, flux, bc)
+    sch.numerics = 1
+    solver = E.SplitSolver(sch, layout=layout)
+    a = _dev(U, layout=layout)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.8, 1.0 / n, max_iters=100)
+    got = _host(a, layout)
+    assert r.iters == it == 100
+    assert abs(r.run_time - rt) <= 1e-12 * rt
+    err = _rel_linf(got, ref)
+    print(f"fast {case}: rel Linf per field {err}", file=sys.stderr)
+    assert max(err) <= 1e-12, err
+
+
+def test_fast_numerics_1024_against_oracle(E, oracle):
+    """The four-quadrant workload at 1024^2, 100 steps, fast numerics against
+    the oracle (16 host threads): every field within 1e-12 relative L-inf."""
+    from fluidity_amd import workloads as W
+    wl = W.riemann_2d(n=1024, seed=1001)
+    ref, rt, it, _ = oracle.simulate(wl.U, _oscheme(oracle, 0, 0, 0, (0, 0)), 0.9, wl.dh,
+                                     max_iters=100, nthreads=16)
+    sch = _escheme(E, 0, 0, 0, (0, 0))
+    sch.numerics = 1
+    solver = E.SplitSolver(sch, layout=1)
+    a = _dev(wl.U, layout=1)
+    w = torch.empty_like(a)
+    r = solver.simulate(a, w, 0.9, wl.dh, max_iters=100)
+    got = _host(a, 1)
+    assert r.iters == it == 100
+    err = _rel_linf(got, ref)
+    print(f"fast 1024^2 rel Linf per field {err}", file=sys.stderr)
+    assert max(err) <= 1e-12, err
+
+
+@pytest.mark.parametrize("flux", [0, 1], ids=["hllc", "force"])
+def test_fast_numerics_full_size(E, flux):
+    """The 4096^2 workload, 100 steps: fast numerics against the exact device
+    loop (bit-identical to the oracle, test_headline_workload_full_size_bitwise):
+    every field within 1e-12 relative L-inf, and the same run time within
+    1e-12."""
+    from fluidity_amd import workloads as W
+    wl = W.riemann_2d(n=4096, seed=1001)
+    out = {}
+    for numerics in (0, 1):
+        sch = _escheme(E, 0, 0, flux, (0, 0))
+        sch.numerics = numerics
+        solver = E.SplitSolver(sch, layout=1)
+        a = _dev(wl.U, layout=1)
+        w = torch.empty_like(a)
+        r = solver.simulate(a, w, 0.9, wl.dh, max_iters=100)
+        assert r.iters == 100
+        out[numerics] = (a, r.run_time)
+        del w
+    ref, got = out[0][0], out[1][0]
+    err = [float((got[k] - ref[k]).abs().max() / ref[k].abs().max()) for k in range(4)]
+    print(f"fast 4096^2 rel Linf per field {err}", file=sys.stderr)
+    assert abs(out[1][1] - out[0][1]) <= 1e-12 * out[0][1]
+    assert max(err) <= 1e-12, err
+    del out, ref, got
+    torch.cuda.empty_cache()
+
+
+def test_fast_numerics_rejected_where_unsupported(E, N):
+    """The batched, two-material and adjoint paths run the exact numerics
+    and refuse the fast flag with FL_ERR_UNSUPPORTED."""
+    from fluidity_amd import adjoint as AJ
+    sch = _escheme(E, 0, 0, 0, (2, 2))
+    sch.numerics = 1
+    adj = AJ.Adjoint(sch, 0.5, 1.0 / 32)
+    U = _dev(cons_field("smooth", (32, 32), seed=3), layout=1)
+    with pytest.raises(N.FluidityError) as e:
+        adj.forward(U, steps=2, checkpoint_every=1)
+    assert e.value.status == N.FL_ERR_UNSUPPORTED

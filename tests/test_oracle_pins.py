@@ -412,3 +412,63 @@ def test_weno_kink_departure_is_epsilon_sized(oracle):
     phi = np.abs(x - 20.5 * h)
     dm, _ = oracle.ls_weno(phi, h, 20)
     assert 1.0e-11 < dm + 1.0 < 2.5e-11, dm + 1.0
+
+
+# ---- corrected stiffened gas ----------------------------------------------------------------
+
+def _stiff_cons(W_, gamma, p_inf):
+    return W.prim_to_cons_eos(np.asarray(W_, dtype=np.float64), gamma, p_inf, True)
+
+
+def test_stiff_pressure_closed_forms(oracle):
+    """Energy E = rho |v|^2/2 + (p + gamma p_inf)/(gamma - 1) (stiff_fluid.hpp:
+    77-80).  The reference's pressure (gamma - 1)(E - rho |v|^2/2) returns
+    p + gamma p_inf; the corrected one returns p, and its sound speed is
+    sqrt(gamma (p + p_inf)/rho)."""
+    g, pinf = 4.4, 3.0
+    Wp = np.array([1.7, 0.8, 0.3, -0.2])
+    U = _stiff_cons(Wp[None, :], g, pinf)[0]
+    quirk = oracle.scheme(eos=oracle.EOS_STIFF, gamma=g, p_inf=pinf)
+    corr = oracle.scheme(eos=oracle.EOS_STIFF_CORRECTED, gamma=g, p_inf=pinf)
+    assert abs(oracle.pressure(U, quirk) - (0.8 + g * pinf)) < 1e-13
+    assert abs(oracle.pressure(U, corr) - 0.8) < 1e-13
+    assert abs(oracle.sound_speed(U, corr) - np.sqrt(g * (0.8 + pinf) / 1.7)) < 1e-14
+    back = oracle.prim_to_cons(oracle.cons_to_prim(U, corr), corr)
+    np.testing.assert_allclose(back, U, rtol=1e-15)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("nd", [1, 2])
+def test_stiff_corrected_is_shifted_ideal_gas(oracle, flux, nd):
+    """With p~ = p + p_inf and E~ = E - p_inf the stiffened-gas Euler
+    equations are the ideal-gas equations in (rho, rho v, E~) with pressure
+    p~ (the momentum flux shifts by the constant p_inf, which cancels in the
+    update).  The corrected scheme on U therefore equals the ideal-gas scheme
+    on U - (0, p_inf, 0..) shifted back, to rounding; the reference's
+    pressure (which keeps gamma p_inf) does not."""
+    g, pinf = 1.9, 2.5
+    shape = (60,) if nd == 1 else (20, 24)
+    rng = np.random.default_rng(31)
+    n = int(np.prod(shape))
+    Wp = np.zeros((n, 2 + nd))
+    Wp[:, 0] = rng.uniform(0.5, 2.0, n)
+    Wp[:, 1] = rng.uniform(0.2, 2.0, n)
+    Wp[:, 2:] = rng.uniform(-0.5, 0.5, (n, nd))
+    U = _stiff_cons(Wp, g, pinf).reshape(shape + (2 + nd,))
+    Ut = U.copy()
+    Ut[..., 1] -= pinf
+    corr = oracle.scheme(flux=flux, eos=oracle.EOS_STIFF_CORRECTED, gamma=g, p_inf=pinf)
+    ideal = oracle.scheme(flux=flux, gamma=g)
+    dtdh = 0.3 / oracle.max_wavespeed_grid(Ut, ideal)
+    assert abs(oracle.max_wavespeed_grid(U, corr) - oracle.max_wavespeed_grid(Ut, ideal)) < 1e-12
+    a, b = U, Ut
+    for dim in range(nd):
+        a = oracle.sweep(a, corr, dtdh, dim)
+        b = oracle.sweep(b, ideal, dtdh, dim)
+    b[..., 1] += pinf
+    np.testing.assert_allclose(a, b, rtol=0, atol=2e-12)
+    quirk = oracle.scheme(flux=flux, eos=oracle.EOS_STIFF, gamma=g, p_inf=pinf)
+    c = U
+    for dim in range(nd):
+        c = oracle.sweep(c, quirk, dtdh, dim)
+    assert np.abs(c - b).max() > 1e-3
