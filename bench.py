@@ -36,8 +36,21 @@ HEADLINE_TUNING = (128, 27, 64, 64)
 # HBM bytes per launch / algorithmic bytes per launch of the 4096^2 FP64 HLLC
 # sweeps at HEADLINE_TUNING, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
 # calibrated against a plain 512 MiB device copy
-# (profiles/sweeps_4096_f64_hllc.md).
+# (profiles/sweeps_4096_f64_hllc.md).  Profile-derived, not measured by this
+# run: the line says so in roofline.traffic_source.
 MEASURED_TRAFFIC_RATIO = {"x": 1.279, "y": 1.339}
+
+# VALU instructions issued per interior cell by each sweep at HEADLINE_TUNING
+# (rocprofv3 SQ_INSTS_VALU x 64 lanes / cells, profiles/sweeps_4096_f64_hllc.md),
+# and FP64 floating-point operations per interior cell (SQ_INSTS_VALU_ADD_F64
+# + MUL_F64 + TRANS_F64 + 2 FMA_F64, x 64 lanes / cells): profile-derived.
+PROFILED_VALU_PER_CELL = {"x": 531.0, "y": 514.0}
+PROFILED_F64_FLOP_PER_CELL = None
+
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per SIMD every 4 cycles = 64 lanes x 614.4 G = 39.3 T lane
+# instructions/s (FP64 FMA: 16 lanes per SIMD per clock, 78.6 TFLOP/s).
+VALU_PEAK_LANE_INST_S = 256 * 4 * 2.4e9 / 4 * 64
 
 
 def parse():
@@ -66,6 +79,8 @@ def parse():
                     help="precision of --mode sweeps")
     ap.add_argument("--flux", choices=["hllc", "force"], default="hllc",
                     help="flux of --mode sweeps")
+    ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
+                    help="FP64 numerics of --mode sweeps (fast: 1e-12 relative mode)")
     return ap.parse_args()
 
 
@@ -109,8 +124,8 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
 
-    def make_solver(flux, layout=N.FL_LAYOUT_SOA):
-        s = E.SplitSolver(E.Scheme(flux=flux), layout=layout)
+    def make_solver(flux, layout=N.FL_LAYOUT_SOA, numerics=N.FL_NUMERICS_EXACT):
+        s = E.SplitSolver(E.Scheme(flux=flux, numerics=numerics), layout=layout)
         s.set_tuning(x_block=args.x_block, x_rows=args.x_rows, y_block=args.y_block,
                      y_rows=args.y_rows)
         return s
@@ -119,21 +134,22 @@ def main():
 
     U_soa = W.aos_to_soa(wl.U)
 
-    def run_case(flux, dtype, steps, warmup):
-        solver = make_solver(flux)
+    def run_case(flux, dtype, steps, warmup, numerics=N.FL_NUMERICS_EXACT):
+        solver = make_solver(flux, numerics=numerics)
         with torch.cuda.stream(stream):
             a = torch.from_numpy(U_soa).to(device=dev, dtype=dtype)
             w = torch.empty_like(a)
         stream.synchronize()
         explicit = args.x_block or args.x_rows or args.y_block or args.y_rows
-        headline = flux == N.FL_FLUX_HLLC and dtype == torch.float64
+        headline = (flux == N.FL_FLUX_HLLC and dtype == torch.float64
+                    and numerics == N.FL_NUMERICS_EXACT)
         if headline and not explicit and not args.autotune_headline:
             xb, xr, yb, yr = HEADLINE_TUNING
             solver.set_tuning(x_block=xb, x_rows=xr, y_block=yb, y_rows=yr)
-            tuned[(flux, str(dtype))] = list(HEADLINE_TUNING)
+            tuned[(flux, str(dtype), numerics)] = list(HEADLINE_TUNING)
         elif args.autotune and not explicit:
             t = solver.autotune(a, w, reps=2, stream=stream)
-            tuned[(flux, str(dtype))] = [t.x_block, t.x_rows, t.y_block, t.y_rows]
+            tuned[(flux, str(dtype), numerics)] = [t.x_block, t.x_rows, t.y_block, t.y_rows]
         if warmup:
             solver.simulate(a, w, cfl, dh, max_iters=warmup, stream=stream)
         torch.cuda.synchronize()
@@ -168,23 +184,41 @@ def main():
     dom_name, dom_t, dom_k = ("sweep_x_kernel<double,2,HLLC>", t_x, "x") if t_x >= t_y else \
         ("sweep_y_kernel<double,HLLC>", t_y, "y")
     achieved = bytes_per_sweep / dom_t / 1e9
-    profiled = (n == 4096 and tuned.get((N.FL_FLUX_HLLC, str(torch.float64)))
+    profiled = (n == 4096 and tuned.get((N.FL_FLUX_HLLC, str(torch.float64), 0))
                 == list(HEADLINE_TUNING))
     traffic = round(MEASURED_TRAFFIC_RATIO[dom_k] * bytes_per_sweep) if profiled else None
+    copy_gbs = hbm_copy_ceiling(torch, stream)
+    valu_rate = PROFILED_VALU_PER_CELL[dom_k] * cells / dom_t
     roofline = {
-        "bound": "hbm",
+        # The sweeps issue FP64 VALU work at about half the issue peak and move
+        # HBM bytes at about a third of its peak: VALU issue bounds them.
+        "bound": "valu",
         "kernel": dom_name,
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE / algorithmic bytes at these "
+                           "launch shapes (profiles/sweeps_4096_f64_hllc.md), times this "
+                           "launch's algorithmic bytes; not counted in this run")
+        if traffic is not None else None,
         "algorithmic_bytes": int(bytes_per_sweep),
-        "tuning": tuned.get((N.FL_FLUX_HLLC, str(torch.float64))),
+        "hbm_copy_GBps": round(copy_gbs, 1),
+        "frac_of_copy": round(achieved / copy_gbs, 4),
+        "valu_insts_per_cell": PROFILED_VALU_PER_CELL[dom_k] if profiled else None,
+        "valu_achieved_Tinst_s": round(valu_rate / 1e12, 2) if profiled else None,
+        "valu_peak_Tinst_s": round(VALU_PEAK_LANE_INST_S / 1e12, 2),
+        "valu_frac": round(valu_rate / VALU_PEAK_LANE_INST_S, 4) if profiled else None,
+        "fp64_TFLOPs": (round(PROFILED_F64_FLOP_PER_CELL[dom_k] * cells / dom_t / 1e12, 2)
+                        if profiled and PROFILED_F64_FLOP_PER_CELL else None),
+        "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS,
+        "tuning": tuned.get((N.FL_FLUX_HLLC, str(torch.float64), 0)),
         "x_sweep_us": round(t_x * 1e6, 1),
         "y_sweep_us": round(t_y * 1e6, 1),
         "step_bytes_per_cell": 128,
         "step_frac": round(value * 128 / 1e9 / HBM_PEAK_GBS, 4),
+        "step_frac_of_copy": round(value * 128 / 1e9 / copy_gbs, 4),
     }
     del a, w, solver
     torch.cuda.empty_cache()
@@ -192,15 +226,18 @@ def main():
     variants = {}
     if not args.no_variants:
         vsteps = max(10, args.steps // 2)
-        for name, flux, dt, bpc in [("force_f64", N.FL_FLUX_FORCE, torch.float64, 128),
-                                     ("hllc_f32", N.FL_FLUX_HLLC, torch.float32, 64)]:
-            s2, a2, w2, el2, _ = run_case(flux, dt, vsteps, 2)
+        for name, flux, dt, bpc, num in [
+                ("hllc_f64_fast", N.FL_FLUX_HLLC, torch.float64, 128, N.FL_NUMERICS_FAST),
+                ("force_f64", N.FL_FLUX_FORCE, torch.float64, 128, N.FL_NUMERICS_EXACT),
+                ("force_f64_fast", N.FL_FLUX_FORCE, torch.float64, 128, N.FL_NUMERICS_FAST),
+                ("hllc_f32", N.FL_FLUX_HLLC, torch.float32, 64, N.FL_NUMERICS_EXACT)]:
+            s2, a2, w2, el2, _ = run_case(flux, dt, vsteps, 2, num)
             v = vsteps * cells / el2
             variants[name] = {"value": round(v, 1), "unit": "cell-updates/s",
                               "ms_per_step": round(el2 * 1e3 / vsteps, 4),
                               "hbm_frac": round(v * bpc / 1e9 / HBM_PEAK_GBS, 4),
                               "vs_f64_hllc": round(v / value, 3),
-                              "tuning": tuned.get((flux, str(dt)))}
+                              "tuning": tuned.get((flux, str(dt), num))}
             del s2, a2, w2
             torch.cuda.empty_cache()
 
@@ -241,6 +278,28 @@ def main():
         "variants": variants,
     }
     print(json.dumps(line), flush=True)
+
+
+def hbm_copy_ceiling(torch, stream, nbytes=1 << 30, reps=10):
+    """The box's measured device-copy bandwidth: a 1 GiB device-to-device
+    copy (read 1 GiB + write 1 GiB) timed with HIP events on the launch
+    stream, best of `reps`."""
+    dev = torch.device("cuda:0")
+    with torch.cuda.stream(stream):
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            dst.copy_(src)
+            e1.record(stream)
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / best / 1e9
 
 
 def riemann_1d_case(torch, N, W, stream, runs=3):
@@ -494,7 +553,8 @@ def sweeps_mode(args, wl, torch, E, N, W):
     n = args.n
     dt = torch.float64 if args.dtype == "f64" else torch.float32
     flux = N.FL_FLUX_HLLC if args.flux == "hllc" else N.FL_FLUX_FORCE
-    solver = E.SplitSolver(E.Scheme(flux=flux), layout=N.FL_LAYOUT_SOA)
+    num = N.FL_NUMERICS_FAST if args.numerics == "fast" else N.FL_NUMERICS_EXACT
+    solver = E.SplitSolver(E.Scheme(flux=flux, numerics=num), layout=N.FL_LAYOUT_SOA)
     if args.x_block or args.x_rows or args.y_block or args.y_rows:
         solver.set_tuning(x_block=args.x_block, x_rows=args.x_rows, y_block=args.y_block,
                           y_rows=args.y_rows)
@@ -519,6 +579,7 @@ def sweeps_mode(args, wl, torch, E, N, W):
     ty = ev[1].elapsed_time(ev[2]) / args.steps
     grid_bytes = a.numel() * a.element_size()
     print(json.dumps({"mode": "sweeps", "dtype": args.dtype, "flux": args.flux, "n": n,
+                      "numerics": args.numerics,
                       "grid_bytes": grid_bytes, "x_sweep_ms": round(tx, 4),
                       "y_sweep_ms": round(ty, 4),
                       "x_GBps": round(2 * grid_bytes / tx / 1e6, 1),

@@ -35,6 +35,20 @@ int fail(int status, const char* fmt, ...) {
 
 bool valid_block(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0; }
 
+int resolve_tuning(const fl_tuning* t, fl_tuning* out) {
+  fl_tuning n;
+  n.x_block = t && t->x_block ? t->x_block : 256;
+  n.x_rows = t && t->x_rows ? t->x_rows : 32;
+  n.y_block = t && t->y_block ? t->y_block : 256;
+  n.y_rows = t && t->y_rows ? t->y_rows : 64;
+  if (!valid_block(n.x_block) || !valid_block(n.y_block))
+    return fail(FL_ERR_ARGUMENT, "block widths must be multiples of 64 in [64, %d]", fl::kMaxBlock);
+  if (n.x_rows < 1 || n.x_rows > 4096 || n.y_rows < 1 || n.y_rows > 4096)
+    return fail(FL_ERR_ARGUMENT, "rows per workgroup must be in [1, 4096]");
+  *out = n;
+  return FL_OK;
+}
+
 int check_field(const fl_field* f, const char* name) {
   if (!f) return fail(FL_ERR_ARGUMENT, "%s is NULL", name);
   if (!f->data) return fail(FL_ERR_ARGUMENT, "%s->data is NULL", name);
@@ -51,6 +65,10 @@ int check_field(const fl_field* f, const char* name) {
                   name, d, (long long)f->n[d]);
   if (f->ndim == 2 && f->n[1] > (int64_t)65535 * 4096)
     return fail(FL_ERR_ARGUMENT, "%s->n[1] too large", name);
+  // the 2D sweeps address a row with 32-bit byte offsets (4 FP64 components)
+  if (f->ndim == 2 && f->n[0] > ((int64_t)1 << 27))
+    return fail(FL_ERR_ARGUMENT, "%s->n[0] = %lld: 2D rows hold at most 2^27 cells", name,
+                (long long)f->n[0]);
   return FL_OK;
 }
 
@@ -262,29 +280,24 @@ int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out,
 }
 
 // Both materials' sweeps of dimension `dim` in one launch: in[m] -> out[m]
-// with scheme s[m].  The two grids share shape, layout and dtype (FP64); the
-// flux methods must be equal (the launch instantiates one), else each grid
-// is swept on its own.
-template <int FLUX>
-void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<double>& a0,
-                  const fl::SweepArgs<double>& a1, hipStream_t stream) {
+// with scheme s[m].  The two grids share shape, layout and dtype; the flux
+// methods must be equal (the launch instantiates one), else each grid is
+// swept on its own.
+template <typename T, int FLUX>
+void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
+                  const fl::SweepArgs<T>& a1, hipStream_t stream) {
   if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<double, 1, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX>), grid, dim3(B), 0, stream, a0, a1);
   else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<double, 2, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX>), grid, dim3(B), 0, stream, a0, a1);
   else
-    hipLaunchKernelGGL((fl::sweep_y2_kernel<double, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+    hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX>), grid, dim3(B), 0, stream, a0, a1);
 }
 
-int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
-                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream) {
-  if (s[0]->flux != s[1]->flux || in[0]->dtype != FL_F64) {
-    for (int m = 0; m < 2; ++m) {
-      const int rc = launch_sweep(ctx, in[m], out[m], dim, s[m], dtdh, nullptr, nullptr, stream);
-      if (rc) return rc;
-    }
-    return FL_OK;
-  }
+template <typename T>
+int launch_sweep2_t(const fl_context* ctx, const fl_field* const in[2],
+                    const fl_field* const out[2], int dim, const fl_scheme* const s[2],
+                    double dtdh, hipStream_t stream, const fl::StepCtl* ctl) {
   fl_tuning tu = ctx->tuning;
   const fl_field* f = in[0];
   int rows = 0, B;
@@ -305,24 +318,41 @@ int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_f
       grid = dim3((unsigned)((nx + B - 1) / B), (unsigned)((ny + rows - 1) / rows), 2);
     }
   }
-  const fl::SweepArgs<double> a0 = make_args<double>(ctx, in[0], out[0], s[0], dtdh, nullptr,
-                                                     nullptr, f->ndim == 1 ? 1 : rows);
-  const fl::SweepArgs<double> a1 = make_args<double>(ctx, in[1], out[1], s[1], dtdh, nullptr,
-                                                     nullptr, f->ndim == 1 ? 1 : rows);
+  const fl::SweepArgs<T> a0 = make_args<T>(ctx, in[0], out[0], s[0], dtdh, ctl, nullptr,
+                                           f->ndim == 1 ? 1 : rows);
+  const fl::SweepArgs<T> a1 = make_args<T>(ctx, in[1], out[1], s[1], dtdh, ctl, nullptr,
+                                           f->ndim == 1 ? 1 : rows);
   switch (s[0]->flux) {
-    case FL_FLUX_HLLC: launch_flux2<fl::kFluxHllc>(f->ndim, dim, grid, B, a0, a1, stream); break;
-    case FL_FLUX_FORCE: launch_flux2<fl::kFluxForce>(f->ndim, dim, grid, B, a0, a1, stream); break;
-    case FL_FLUX_LAX_FRIEDRICHS:
-      launch_flux2<fl::kFluxLaxFriedrichs>(f->ndim, dim, grid, B, a0, a1, stream);
+    case FL_FLUX_HLLC: launch_flux2<T, fl::kFluxHllc>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    case FL_FLUX_FORCE:
+      launch_flux2<T, fl::kFluxForce>(f->ndim, dim, grid, B, a0, a1, stream);
       break;
-    default: launch_flux2<fl::kFluxRichtmyer>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    case FL_FLUX_LAX_FRIEDRICHS:
+      launch_flux2<T, fl::kFluxLaxFriedrichs>(f->ndim, dim, grid, B, a0, a1, stream);
+      break;
+    default: launch_flux2<T, fl::kFluxRichtmyer>(f->ndim, dim, grid, B, a0, a1, stream); break;
   }
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
-// The max wavespeed over two FP64 grids of one shape (each in its own
-// layout), max-ed into dst (not cleared here).
+int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
+                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream,
+                  const fl::StepCtl* ctl) {
+  // the two-grid kernels run the exact FP64 numerics (FP32 its own forms)
+  if (s[0]->flux != s[1]->flux || (in[0]->dtype == FL_F64 && (fast_numerics(s[0]) ||
+                                                              fast_numerics(s[1])))) {
+    for (int m = 0; m < 2; ++m) {
+      const int rc = launch_sweep(ctx, in[m], out[m], dim, s[m], dtdh, ctl, nullptr, stream);
+      if (rc) return rc;
+    }
+    return FL_OK;
+  }
+  if (in[0]->dtype == FL_F64)
+    return launch_sweep2_t<double>(ctx, in, out, dim, s, dtdh, stream, ctl);
+  return launch_sweep2_t<float>(ctx, in, out, dim, s, dtdh, stream, ctl);
+}
+
 int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
                    const fl_scheme* s1, unsigned long long* dst, hipStream_t stream) {
   int64_t es0, cs0, es1, cs1;
@@ -334,14 +364,26 @@ int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
   if (blocks > fl::kMaxWsBlocks) blocks = fl::kMaxWsBlocks;
   if (blocks < 1) blocks = 1;
   const fl::Material<double> m0 = make_material<double>(s0), m1 = make_material<double>(s1);
-  if (f0->ndim == 1)
-    hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 1>), dim3((unsigned)blocks), dim3(B), 0,
-                       stream, (const double*)f0->data, (const double*)f1->data, es0, cs0, es1,
-                       cs1, n, m0, m1, dst);
-  else
-    hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 2>), dim3((unsigned)blocks), dim3(B), 0,
-                       stream, (const double*)f0->data, (const double*)f1->data, es0, cs0, es1,
-                       cs1, n, m0, m1, dst);
+  const dim3 grid((unsigned)blocks);
+  if (f0->dtype == FL_F64) {
+    if (f0->ndim == 1)
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 1>), grid, dim3(B), 0, stream,
+                         (const double*)f0->data, (const double*)f1->data, es0, cs0, es1, cs1,
+                         n, m0, m1, dst);
+    else
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 2>), grid, dim3(B), 0, stream,
+                         (const double*)f0->data, (const double*)f1->data, es0, cs0, es1, cs1,
+                         n, m0, m1, dst);
+  } else {
+    if (f0->ndim == 1)
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 1>), grid, dim3(B), 0, stream,
+                         (const float*)f0->data, (const float*)f1->data, es0, cs0, es1, cs1,
+                         n, m0, m1, dst);
+    else
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 2>), grid, dim3(B), 0, stream,
+                         (const float*)f0->data, (const float*)f1->data, es0, cs0, es1, cs1,
+                         n, m0, m1, dst);
+  }
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -415,10 +457,7 @@ int fl_ctx_create(int device, fl_context** out) {
   fl_context* c = new fl_context();
   memset(c, 0, sizeof(*c));
   c->device = device;
-  c->tuning.x_block = 256;
-  c->tuning.x_rows = 32;
-  c->tuning.y_block = 256;
-  c->tuning.y_rows = 64;
+  (void)resolve_tuning(nullptr, &c->tuning);
   hipError_t e = hipMalloc((void**)&c->ctl, sizeof(fl::StepCtl));
   if (e == hipSuccess) e = hipMalloc((void**)&c->ws_tmp, sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc((void**)&c->partial, sizeof(double) * kSumBlocks * 4);
@@ -428,6 +467,7 @@ int fl_ctx_create(int device, fl_context** out) {
   c->use_graphs = 1;
   if (e == hipSuccess) e = hipMalloc((void**)&c->eik, 2 * sizeof(fl::ls::EikCtl));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, 2 * sizeof(fl::ls::EikCtl), 0);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
     fl_ctx_destroy(c);
     return fail(FL_ERR_HIP, "context allocation failed: %s", hipGetErrorString(e));
@@ -451,6 +491,7 @@ int fl_ctx_destroy(fl_context* c) {
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->eik) (void)hipFree(c->eik);
   if (c->host_eik) (void)hipHostFree(c->host_eik);
+  if (c->step_ev) (void)hipEventDestroy(c->step_ev);
   delete c;
   return FL_OK;
 }
@@ -462,10 +503,8 @@ int fl_ctx_set_tuning(fl_context* c, const fl_tuning* t) {
   if (t->x_rows) n.x_rows = t->x_rows;
   if (t->y_block) n.y_block = t->y_block;
   if (t->y_rows) n.y_rows = t->y_rows;
-  if (!valid_block(n.x_block) || !valid_block(n.y_block))
-    return fail(FL_ERR_ARGUMENT, "block widths must be multiples of 64 in [64, %d]", fl::kMaxBlock);
-  if (n.x_rows < 1 || n.x_rows > 4096 || n.y_rows < 1 || n.y_rows > 4096)
-    return fail(FL_ERR_ARGUMENT, "rows per workgroup must be in [1, 4096]");
+  int rc = resolve_tuning(&n, &n);
+  if (rc) return rc;
   c->tuning = n;
   return FL_OK;
 }

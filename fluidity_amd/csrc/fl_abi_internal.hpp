@@ -33,6 +33,8 @@ struct fl_context {
   fl::ls::EikCtl* eik;         // device control words of the eikonal loop, 2 jobs
   fl::ls::EikCtl* host_eik;    // pinned host mirror of eik
   fl_tuning tuning;
+  hipEvent_t step_ev;          // fl_mm_simulate: the step's control block is on the host
+  int64_t eik_sweeps;          // sweeps the last eikonal solve took (first batch size)
   int use_graphs;              // fl_simulate replays a captured pair of steps
   hipStream_t cap_stream;      // capture stream of that graph
   hipGraph_t sim_graph;        // a pair of steps (both parities)
@@ -85,6 +87,10 @@ inline fl::Material<T> make_material(const fl_scheme* s) {
 }
 
 double bits_to_double(unsigned long long b);
+// `t` (nullable: all defaults) with zeros replaced by the built-in launch
+// shapes, validated, into *out.
+int resolve_tuning(const fl_tuning* t, fl_tuning* out);
+int check_pair(const fl_field* in, const fl_field* out);
 int check_field(const fl_field* f, const char* name);
 int check_scheme(const fl_scheme* s, int ndim);
 // FL_ERR_UNSUPPORTED unless s asks for the exact numerics.
@@ -96,8 +102,11 @@ int64_t cell_count(const fl_field* f);
 int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out, int dim,
                  const fl_scheme* s, double dtdh, const fl::StepCtl* ctl,
                  unsigned long long* ws_out, hipStream_t stream);
+// Both materials' sweeps of dimension `dim` in one launch; ctl (nullable)
+// as in launch_sweep.
 int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
-                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream);
+                  int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream,
+                  const fl::StepCtl* ctl = nullptr);
 int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
                    const fl_scheme* s1, unsigned long long* dst, hipStream_t stream);
 // atomicMax of the FP64 wavespeed bits of every cell into *dst.

@@ -79,4 +79,5 @@ __global__ void zero_u64_kernel(unsigned long long* p);
 __global__ void step_control_kernel(StepCtl* ctl, int parity, double cfl, double dh,
                                     double sim_time, long long max_iters);
 __global__ void init_control_kernel(StepCtl* ctl);
+__global__ void mm_control_kernel(StepCtl* ctl, double cfl, double dh);
 }  // namespace fl

@@ -89,29 +89,40 @@ __device__ __forceinline__ void norm_at(const double* __restrict__ phi, const Gr
   for (int d = 0; d < ND; ++d) nrm[d] = (mag == 0.0) ? 0.0 : nrm[d] / s;
 }
 
+// The state of cell i of the grid (no ghost mapping), in FP64.
+template <int S>
+__device__ __forceinline__ void load_state_i(const StateView& v, int64_t i, double* u) {
+  if (v.f32) {
+    const float* p = (const float*)v.data + i * v.es;
+#pragma unroll
+    for (int k = 0; k < S; ++k) u[k] = (double)p[k * v.cs];
+  } else {
+    const double* p = v.data + i * v.es;
+#pragma unroll
+    for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
+  }
+}
+
 template <int S>
 __device__ __forceinline__ void load_state(const StateView& v, const Grid& g, int64_t x,
                                            int64_t y, double* u) {
   const int64_t xi = mirror(x, g.nx);
   const int64_t yi = g.nd > 1 ? mirror(y, g.ny) : 0;
-  const double* p = v.data + (yi * g.nx + xi) * v.es;
-#pragma unroll
-  for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
+  load_state_i<S>(v, yi * g.nx + xi, u);
 }
 
-// The state of cell i of the grid (no ghost mapping).
-template <int S>
-__device__ __forceinline__ void load_state_i(const StateView& v, int64_t i, double* u) {
-  const double* p = v.data + i * v.es;
-#pragma unroll
-  for (int k = 0; k < S; ++k) u[k] = p[k * v.cs];
-}
-
+// Stores an FP64 state into cell i (rounded to float on an FP32 grid).
 template <int S>
 __device__ __forceinline__ void store_state(const StateView& v, int64_t i, const double* u) {
-  double* p = v.data + i * v.es;
+  if (v.f32) {
+    float* p = (float*)v.data + i * v.es;
 #pragma unroll
-  for (int k = 0; k < S; ++k) p[k * v.cs] = u[k];
+    for (int k = 0; k < S; ++k) p[k * v.cs] = (float)u[k];
+  } else {
+    double* p = v.data + i * v.es;
+#pragma unroll
+    for (int k = 0; k < S; ++k) p[k * v.cs] = u[k];
+  }
 }
 
 __device__ __forceinline__ int64_t gid() {
@@ -274,6 +285,18 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
       for (int k = -3; k <= 3; ++k) fs[ND - 1][k + 3] = phi[(int64_t)mirror32(y + k, ny) * nx + x];
     }
   }
+  // A stencil whose cells all hold the centre's bit pattern (the clamped
+  // values outside the reinitialised band) has every difference +0: each
+  // one-sided WENO derivative is then 0 / (6 * 10^195) = +0 and the Godunov
+  // sum is +0, so L = v * sqrt(+0) = v * 0 exactly.  A wave whose cells are
+  // all such skips the WENO arithmetic; the others take it.
+  bool flat = isfinite(fs[0][3]);
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      flat = flat && __double_as_longlong(fs[d][k]) == __double_as_longlong(fs[0][3]);
+  if (__all(flat)) return v * 0.0;
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     const double* f = fs[d];
@@ -299,11 +322,14 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
 //  stage 1: out = 0.75 phi_n + 0.25 (s1 - dt L(s1))
 //  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
 // Launched on a (ceil(nx / blockDim.x), min(ny, 65535)) grid; rows stride by
-// gridDim.y.
+// gridDim.y.  dt_dev (nullable): dt in device memory (the step's control
+// block), read instead of dt.
 template <int ND>
 __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
                                  const double* __restrict__ src, const double* __restrict__ V,
-                                 Grid g, double dt, double h1, double* __restrict__ out) {
+                                 Grid g, double dt, const double* dt_dev, double h1,
+                                 double* __restrict__ out) {
+  if (dt_dev) dt = *dt_dev;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t x0 = (int64_t)blockIdx.x * blockDim.x;
   const bool xin = x0 >= 3 && x0 + blockDim.x + 3 <= g.nx;  // workgroup-uniform
@@ -965,7 +991,8 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
   template __global__ void velocity2_kernel<ND>(const double*, const double*, StateView,     \
                                                 StateView, Grid, double, double*, double*); \
   template __global__ void rk3_stage_kernel<ND>(int, const double*, const double*,            \
-                                                const double*, Grid, double, double, double*); \
+                                                const double*, Grid, double, const double*,  \
+                                                double, double*);                            \
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
                                                uint8_t*, int*);                             \

@@ -15,11 +15,15 @@ struct Grid {
   int pad;
 };
 
-// A conservative state grid: element stride and component stride in doubles
-// (AoS: es = S, cs = 1; SoA: es = 1, cs = cells).
+// A conservative state grid: element stride and component stride in
+// elements (AoS: es = S, cs = 1; SoA: es = 1, cs = cells).  f32: the grid
+// holds floats (the FP32 bulk path); the level-set kernels compute in FP64
+// and convert on load and store.
 struct StateView {
   double* data;
   int64_t es, cs;
+  int f32;
+  int pad;
 };
 
 // Control words of the eikonal Jacobi loop: flags[k % 3] records a change in
@@ -65,7 +69,8 @@ __global__ void velocity2_kernel(const double* phi0, const double* phi1, StateVi
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
 template <int ND>
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
-                                 const double* V, Grid g, double dt, double h1, double* out);
+                                 const double* V, Grid g, double dt, const double* dt_dev,
+                                 double h1, double* out);
 __global__ void eik_init_kernel(EikJobs jobs, Grid g);
 // Eikonal tiles: kEikTileW cells x kEikTileH rows (one workgroup each).
 constexpr int kEikTileW = 64;

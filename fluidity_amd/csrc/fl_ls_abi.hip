@@ -42,12 +42,10 @@ int check_dh(double dh) {
   return FL_OK;
 }
 
-// An FP64 state field over grid g.
+// A state field (FP64, or FP32 bulk storage) over grid g.
 int check_state(const fl_field* f, const ls::Grid& g, const char* name) {
   int rc = check_field(f, name);
   if (rc) return rc;
-  if (f->dtype != FL_F64)
-    return fail(FL_ERR_UNSUPPORTED, "%s: the ghost-fluid path runs in FP64 only", name);
   if (f->ndim != g.nd || f->n[0] != g.nx || (g.nd > 1 && f->n[1] != g.ny))
     return fail(FL_ERR_ARGUMENT, "%s does not match the level-set grid", name);
   return FL_OK;
@@ -57,6 +55,8 @@ ls::StateView view_of(const fl_field* f) {
   ls::StateView v;
   v.data = (double*)f->data;
   strides(f, v.es, v.cs);
+  v.f32 = f->dtype == FL_F32 ? 1 : 0;
+  v.pad = 0;
   return v;
 }
 
@@ -108,9 +108,10 @@ int launch_velocity2(double* const* phis, const fl_field& s0, const fl_field& s1
   return FL_OK;
 }
 
-// RK3: phi -> s1 -> s2 -> out, s1/s2 in work.
+// RK3: phi -> s1 -> s2 -> out, s1/s2 in work.  dt_dev (nullable): dt in
+// device memory, used instead of dt.
 int launch_advect(const double* phi, const double* V, const ls::Grid& g, double dt, double dh,
-                  double* out, double* work, hipStream_t st) {
+                  double* out, double* work, hipStream_t st, const double* dt_dev = nullptr) {
   const int64_t cells = g.nx * g.ny;
   double* s1 = work;
   double* s2 = work + cells;
@@ -118,9 +119,11 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
                   (unsigned)(g.ny < 65535 ? g.ny : 65535));
   const double h1 = 1.0 / dh;
   auto k = g.nd == 1 ? ls::rk3_stage_kernel<1> : ls::rk3_stage_kernel<2>;
-  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, h1, s1);
-  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1, V, g, dt, h1, s2);
-  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2, V, g, dt, h1, out);
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 0, phi, phi, V, g, dt, dt_dev, h1, s1);
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 1, phi, (const double*)s1, V, g, dt, dt_dev,
+                     h1, s2);
+  hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, 2, phi, (const double*)s2, V, g, dt, dt_dev,
+                     h1, out);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -193,12 +196,19 @@ int run_reinit(fl_context* c, const ls::EikJobs& jobs, int njobs, const ls::Grid
   // Every sweep strictly lowers some cell of an unconverged job, so the loop
   // ends; the bound only guards the host against a device that never
   // reports convergence.
+  // The first batch is as long as the context's previous solve took (a
+  // level set reinitialised every few steps needs about the same number of
+  // sweeps each time), so that one test usually suffices; sweeps past
+  // convergence return at once.
   const int64_t max_sweeps = cells + 2;
   int64_t launched = 0;
+  int64_t batch = c->eik_sweeps > kEikBatch ? c->eik_sweeps + 1 : kEikBatch;
+  if (batch > max_sweeps) batch = max_sweeps;
   for (;;) {
-    for (int b = 0; b < kEikBatch; ++b, ++launched)
+    for (int64_t b = 0; b < batch; ++b, ++launched)
       hipLaunchKernelGGL(ls::eik_sweep_kernel, sgrid, tblock, 0, st, jobs, g, segs, dh, limit,
                          (int)launched);
+    batch = kEikBatch;
     FL_HIP(hipGetLastError());
     FL_HIP(hipMemcpyAsync(c->host_eik, c->eik, njobs * sizeof(ls::EikCtl),
                           hipMemcpyDeviceToHost, st));
@@ -213,6 +223,10 @@ int run_reinit(fl_context* c, const ls::EikJobs& jobs, int njobs, const ls::Grid
       return fail(FL_ERR_NUMERIC, "eikonal solve did not converge in %lld sweeps",
                   (long long)launched);
   }
+  int64_t took = 0;
+  for (int j = 0; j < njobs; ++j)
+    took = c->host_eik[j].iters > took ? c->host_eik[j].iters : took;
+  c->eik_sweeps = took;
   hipLaunchKernelGGL(ls::eik_finish_kernel, grid, dim3(kBlock), 0, st, jobs, g, limit, clamp);
   FL_HIP(hipGetLastError());
   if (iterations)
@@ -543,6 +557,8 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     if ((rc = check_state(&mats[m].work, g, "mats[m].work"))) return rc;
     if (mats[m].state.layout != mats[m].work.layout || mats[m].state.data == mats[m].work.data)
       return fail(FL_ERR_ARGUMENT, "mats[%d]: work must be a distinct buffer of the same layout", m);
+    if (mats[m].state.dtype != mats[0].state.dtype || mats[m].work.dtype != mats[0].state.dtype)
+      return fail(FL_ERR_ARGUMENT, "mats[%d]: every state and work grid needs one dtype", m);
     if (!mats[m].phi) return fail(FL_ERR_ARGUMENT, "mats[%d].phi is NULL", m);
     if ((rc = check_scheme(&mats[m].scheme, g.nd))) return rc;
     if ((rc = require_exact(&mats[m].scheme, "the two-material step"))) return rc;
@@ -566,6 +582,7 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
   hipStream_t st = (hipStream_t)stream;
   const int64_t cells = g.nx * g.ny;
   const int S = 2 + g.nd;
+  const int64_t esize = mats[0].state.dtype == FL_F64 ? 8 : 4;
   const double dh = p->dh;
   double t = 0.0, dt = 0.0, maxws = 0.0;
   int64_t it = 0;
@@ -581,6 +598,16 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
                               hipMemcpyDeviceToDevice, st));
     return FL_OK;
   };
+  // dt lives on the device: the control kernel turns the maximum wavespeed
+  // into dt, dt/dh, run time and iteration count in the context's control
+  // block, the sweeps read dt/dh and the level-set advection dt from it.  The
+  // host reads a copy of the block once per step, after the step's sweeps are
+  // queued (so the device keeps working while it waits), to test the loop
+  // condition and to stop on a NaN or vacuum maximum, as the reference loop
+  // does, before the level sets move.
+  fl::StepCtl* ctl = c->ctl;
+  hipLaunchKernelGGL(fl::init_control_kernel, dim3(1), dim3(1), 0, st, ctl);
+  FL_HIP(hipGetLastError());
   while (t < p->sim_time && it < p->max_iters) {
     // ghost cells: star states of both materials from the same data, then
     // extrapolation into each material's ghost band
@@ -592,16 +619,28 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     band_seed2(jobs, gw, g, dh, st);
     extrap_sweeps(jobs[0], jobs[1], 2, g, dh, sweeps, st);
     FL_HIP(hipGetLastError());
-    // dt from the maximum wavespeed over both grids
-    hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+    // dt from the maximum wavespeed over both grids (ws_bits[0] is clear)
     if ((rc = launch_max_ws2(&mats[0].state, &mats[1].state, &mats[0].scheme, &mats[1].scheme,
-                             c->ws_tmp, st)))
+                             &ctl->ws_bits[0], st)))
       return rc;
-    FL_HIP(hipMemcpyAsync(&c->host_ctl->ws_bits[0], c->ws_tmp, sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, st));
-    FL_HIP(hipStreamSynchronize(st));
-    maxws = bits_to_double(c->host_ctl->ws_bits[0]);
-    if (!(maxws > 0.0) || isnan(maxws) || isinf(maxws)) {
+    hipLaunchKernelGGL(fl::mm_control_kernel, dim3(1), dim3(1), 0, st, ctl, p->cfl, dh);
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipMemcpyAsync(c->host_ctl, ctl, sizeof(fl::StepCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipEventRecord(c->step_ev, st));
+    // both materials' split solves, each sweep of both grids in one launch;
+    // they read dt/dh from the control block and do nothing when it is not
+    // active
+    {
+      const fl_field* const u[2] = {&mats[0].state, &mats[1].state};
+      const fl_field* const v[2] = {&mats[0].work, &mats[1].work};
+      const fl_scheme* const sc[2] = {&mats[0].scheme, &mats[1].scheme};
+      if ((rc = launch_sweep2(c, u, v, 0, sc, 0.0, st, ctl))) return rc;
+      if (g.nd > 1 && (rc = launch_sweep2(c, v, u, 1, sc, 0.0, st, ctl))) return rc;
+    }
+    FL_HIP(hipEventSynchronize(c->step_ev));
+    const fl::StepCtl hc = *c->host_ctl;
+    maxws = bits_to_double(hc.ws_bits[1]);
+    if (!hc.active) {
       if (result) {
         result->run_time = t;
         result->iters = it;
@@ -613,27 +652,17 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
       return fail(FL_ERR_NUMERIC, "max wavespeed %g at step %lld (NaN or vacuum)", maxws,
                   (long long)it);
     }
-    dt = p->cfl * dh / maxws;
-    const double dtdh = dt / dh;
-    // both materials' split solves, each sweep of both grids in one launch
-    {
-      const fl_field* const u[2] = {&mats[0].state, &mats[1].state};
-      const fl_field* const v[2] = {&mats[0].work, &mats[1].work};
-      const fl_scheme* const sc[2] = {&mats[0].scheme, &mats[1].scheme};
-      if ((rc = launch_sweep2(c, u, v, 0, sc, dtdh, st))) return rc;
-      if (g.nd > 1) {
-        if ((rc = launch_sweep2(c, v, u, 1, sc, dtdh, st))) return rc;
-      } else {
-        for (int m = 0; m < 2; ++m)
-          FL_HIP(hipMemcpyAsync(mats[m].state.data, mats[m].work.data, (size_t)(cells * S * 8),
-                                hipMemcpyDeviceToDevice, st));
-      }
-    }
+    if (g.nd == 1)
+      for (int m = 0; m < 2; ++m)
+        FL_HIP(hipMemcpyAsync(mats[m].state.data, mats[m].work.data, (size_t)(cells * S * esize),
+                              hipMemcpyDeviceToDevice, st));
     // level sets: speeds from the old level sets and new states (both in one
-    // pass), advection, projection, reinitialisation every `every` steps
+    // pass), advection with the device dt, projection, reinitialisation every
+    // `every` steps
     if ((rc = launch_velocity2(cur, mats[0].state, mats[1].state, g, dh, w.V, st))) return rc;
     for (int m = 0; m < 2; ++m)
-      if ((rc = launch_advect(cur[m], w.V[m], g, dt, dh, nxt[m], w.adv, st))) return rc;
+      if ((rc = launch_advect(cur[m], w.V[m], g, 0.0, dh, nxt[m], w.adv, st, &ctl->dt)))
+        return rc;
     if ((rc = launch_project(nxt[0], nxt[1], cells, st))) return rc;
     const bool reinit = (it % every) == every - 1;
     if (reinit) {  // both level sets in one solve, written back into cur[m]
@@ -648,8 +677,9 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
         nxt[m] = tmp;
       }
     }
-    t += dt;
-    it += 1;
+    dt = hc.dt;
+    t = hc.t;
+    it = hc.iters;
   }
   if ((rc = settle())) return rc;
   FL_HIP(hipStreamSynchronize(st));

@@ -2,9 +2,11 @@
 //
 // The same MUSCL-Hancock / HLLC / FORCE formulas as fl_numerics.hpp, written
 // for fewer instructions instead of the reference's rounding:
-//   - each state's 1/rho is formed once (v_rcp_f64 and two Newton steps,
+//   - each state's 1/rho is formed once (v_rcp_f64 and one Newton step,
 //     within about an ulp) and every quotient by rho becomes a product; other
 //     quotients are a * (1/b) the same way (FP32: v_rcp_f32);
+//   - square roots take one Goldschmidt step and one correction after
+//     v_rsq_f64 (FP32: v_sqrt_f32);
 //   - the van Leer slope is its closed form: with r = b/f,
 //     (b + f)/2 * 2 min(r, 1)/(1 + r) equals b when 0 < r <= 1 and f when
 //     r > 1, i.e. minmod(b, f), so it needs no division at all;
@@ -20,14 +22,46 @@
 namespace fl {
 namespace fast {
 
+// Newton steps refining v_rcp_f64: one (about an ulp; tuning builds: 2).
+#ifndef FL_FAST_NEWTON
+#define FL_FAST_NEWTON 1
+#endif
+
 __device__ __forceinline__ double rcp(double b) {
   double r = __builtin_amdgcn_rcp(b);
   double e = __builtin_fma(-b, r, 1.0);
   r = __builtin_fma(r, e, r);
+#if FL_FAST_NEWTON > 1
   e = __builtin_fma(-b, r, 1.0);
-  return __builtin_fma(r, e, r);
+  r = __builtin_fma(r, e, r);
+#endif
+  return r;
 }
 __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+
+// Square root.  FP64 with FL_FAST_SQRT (default): v_rsq_f64, one Goldschmidt
+// step and one residual correction (the IEEE sequence's second correction and
+// its half-inverse-root refinement left out); +-0 and +inf return x.
+// FL_FAST_SQRT=0: the correctly rounded fl::sq.  FP32: v_sqrt_f32.
+#ifndef FL_FAST_SQRT
+#define FL_FAST_SQRT 1
+#endif
+
+__device__ __forceinline__ double sqt(double x) {
+#if FL_FAST_SQRT
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  const double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  const double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return __builtin_amdgcn_class(x, 0x260) ? x : g;
+#else
+  return sq(x);
+#endif
+}
+__device__ __forceinline__ float sqt(float x) { return sq(x); }
 
 template <typename T> __device__ __forceinline__ T div(T a, T b) { return a * rcp(b); }
 
@@ -66,8 +100,8 @@ __device__ __forceinline__ Prim<T, ND> as_prim(const PrimR<T, ND>& w) {
 template <typename T>
 __device__ __forceinline__ T sound_speed(T p, T rinv, const Material<T>& m) {
 #pragma clang fp contract(fast)
-  if (m.stiff) return sq(m.gamma * (p + m.p_inf) * rinv);
-  return sq(m.gamma * p * rinv);
+  if (m.stiff) return fast::sqt(m.gamma * (p + m.p_inf) * rinv);
+  return fast::sqt(m.gamma * p * rinv);
 }
 
 template <typename T, int ND, int D>
@@ -133,7 +167,10 @@ __device__ __forceinline__ T chi_factor(T p_pprev, T p_prev, T p_next, T p_nnext
   const T dp = fabs(p_next - p_prev);
   T pmin = smin(p_next, p_prev);
   if (m.corr) pmin = pmin + m.p_inf;
-  const bool flatten = u_factor > T(0) && fast::div(dp, pmin) > T(0.33);
+  // dp / pmin > 0.33 without the quotient: for pmin > 0 it is dp > 0.33 pmin;
+  // pmin = 0 gives +inf (dp > 0) or NaN (dp = 0); pmin < 0 a quotient <= 0;
+  // a NaN operand fails every comparison.
+  const bool flatten = u_factor > T(0) && pmin >= T(0) && dp > T(0.33) * pmin;
   if (!flatten) return T(1);
   const T z = fast::div(dp, T(fabs(p_nnext - p_pprev)));
   const T temp = T(1) - (z - T(0.75)) * T(10);
@@ -192,8 +229,8 @@ template <typename T>
 __device__ __forceinline__ T hllc_wavespeed(T u, T p, T pstar, T sound, const Material<T>& m) {
 #pragma clang fp contract(fast)
   if (pstar <= p) return u + sound;
-  if (m.corr) return u + sound * sq(T(1) + m.q * (fast::div(pstar + m.p_inf, p + m.p_inf) - T(1)));
-  return u + sound * sq(T(1) + m.q * (fast::div(pstar, p) - T(1)));
+  if (m.corr) return u + sound * fast::sqt(T(1) + m.q * (fast::div(pstar + m.p_inf, p + m.p_inf) - T(1)));
+  return u + sound * fast::sqt(T(1) + m.q * (fast::div(pstar, p) - T(1)));
 }
 
 // HLLC (flux_method/flux_hllc.hpp:82-245), as fl::hllc_flux.
@@ -231,13 +268,16 @@ __device__ __forceinline__ void hllc_flux(const T* ul, const T* ur, const Materi
   T f[S];
   fast::euler_flux<T, ND, D>(st, w, f);
   const T state_factor = ws - w.vel[D];
-  const T scale_factor = fast::div(state_factor, ws - ws_star);
+  // 1/(ws - ws_star) and 1/state_factor from one reciprocal of their product
+  const T rr = fast::rcp((ws - ws_star) * state_factor);
+  const T scale_factor = state_factor * state_factor * rr;
+  const T inv_sf = (ws - ws_star) * rr;
   T us[S];
 #pragma unroll
   for (int k = 0; k < S; ++k) us[k] = st[k] * scale_factor;
   us[2 + D] = ws_star * st[0] * scale_factor;
   us[1] = scale_factor *
-          (st[1] + st[0] * (ws_star - w.vel[D]) * (ws_star + w.p * w.rinv * fast::rcp(state_factor)));
+          (st[1] + st[0] * (ws_star - w.vel[D]) * (ws_star + w.p * w.rinv * inv_sf));
 #pragma unroll
   for (int k = 0; k < S; ++k) F[k] = f[k] + (us[k] - st[k]) * ws;
 }

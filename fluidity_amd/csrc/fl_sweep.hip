@@ -29,7 +29,7 @@ __device__ __forceinline__ void tile_of_block(int64_t& bx, int64_t& by) {
 }
 
 template <typename T, int ND, int FLUX, bool FAST>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWavesOf<T, FAST>::value)))
 sweep_x_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
@@ -37,7 +37,7 @@ sweep_x_kernel(SweepArgs<T> a) {
 }
 
 template <typename T, int FLUX, bool FAST>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWavesOf<T, FAST>::value)))
 sweep_y_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
@@ -188,6 +188,29 @@ __global__ void step_control_kernel(StepCtl* ctl, int parity, double cfl, double
   ctl->active = active ? 1 : 0;
 }
 
+// dt of a two-material step from the maximum wavespeed reduced into
+// ws_bits[0] over both grids after the ghost states were written (the loop of
+// simulator/multimaterial_simulator.hpp, update_time_delta in
+// simulator/parameters.hpp:60-99): dt = cfl dh / max, dt/dh, run time and
+// iteration count.  The maximum moves to ws_bits[1] (reported) and ws_bits[0]
+// is cleared for the next step.  A NaN, infinite or non-positive maximum
+// leaves dt, t and iters unchanged and clears `active`, which the sweeps read.
+__global__ void mm_control_kernel(StepCtl* ctl, double cfl, double dh) {
+  const unsigned long long bits = ctl->ws_bits[0];
+  const double maxws = __longlong_as_double((long long)bits);
+  ctl->ws_bits[1] = bits;
+  ctl->ws_bits[0] = 0ull;
+  const bool ok = maxws > 0.0 && !isnan(maxws) && !isinf(maxws);
+  if (ok) {
+    const double dt = cfl * dh / maxws;
+    ctl->dt = dt;
+    ctl->dtdh = dt / dh;
+    ctl->t += dt;
+    ctl->iters += 1;
+  }
+  ctl->active = ok ? 1 : 0;
+}
+
 __global__ void init_control_kernel(StepCtl* ctl) {
   ctl->ws_bits[0] = 0ull;
   ctl->ws_bits[1] = 0ull;
@@ -255,14 +278,22 @@ FL_INST_SWEEPS2(double, kFluxHllc)
 FL_INST_SWEEPS2(double, kFluxForce)
 FL_INST_SWEEPS2(double, kFluxLaxFriedrichs)
 FL_INST_SWEEPS2(double, kFluxRichtmyer)
-template __global__ void max_wavespeed2_kernel<double, 1>(const double*, const double*, int64_t,
-                                                          int64_t, int64_t, int64_t, int64_t,
-                                                          Material<double>, Material<double>,
-                                                          unsigned long long*);
-template __global__ void max_wavespeed2_kernel<double, 2>(const double*, const double*, int64_t,
-                                                          int64_t, int64_t, int64_t, int64_t,
-                                                          Material<double>, Material<double>,
-                                                          unsigned long long*);
+FL_INST_SWEEPS2(float, kFluxHllc)
+FL_INST_SWEEPS2(float, kFluxForce)
+FL_INST_SWEEPS2(float, kFluxLaxFriedrichs)
+FL_INST_SWEEPS2(float, kFluxRichtmyer)
+
+#define FL_INST_MAXWS2(T)                                                                 \
+  template __global__ void max_wavespeed2_kernel<T, 1>(const T*, const T*, int64_t, int64_t, \
+                                                       int64_t, int64_t, int64_t,            \
+                                                       Material<double>, Material<double>,   \
+                                                       unsigned long long*);                 \
+  template __global__ void max_wavespeed2_kernel<T, 2>(const T*, const T*, int64_t, int64_t, \
+                                                       int64_t, int64_t, int64_t,            \
+                                                       Material<double>, Material<double>,   \
+                                                       unsigned long long*);
+FL_INST_MAXWS2(double)
+FL_INST_MAXWS2(float)
 
 #define FL_SWEEPS(T, FLUX, FAST)                                                 \
   template __global__ void sweep_x_kernel<T, 1, FLUX, FAST>(SweepArgs<T>);       \

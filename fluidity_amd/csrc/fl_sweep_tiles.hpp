@@ -53,6 +53,12 @@
 #define FL_PREFETCH_F32 2
 #endif
 
+// Waves per SIMD of the FP64 sweeps in the fast numerics (fewer live
+// registers than the exact ones).
+#ifndef FL_SWEEP_WAVES_FAST
+#define FL_SWEEP_WAVES_FAST FL_SWEEP_WAVES
+#endif
+
 namespace fl {
 template <typename T> struct SweepWaves { static constexpr int value = FL_SWEEP_WAVES; };
 template <> struct SweepWaves<float> { static constexpr int value = FL_SWEEP_WAVES_F32; };
@@ -60,6 +66,20 @@ template <typename T> struct SweepPrefetch { static constexpr int value = FL_PRE
 template <> struct SweepPrefetch<float> { static constexpr int value = FL_PREFETCH_F32; };
 template <typename T> struct SweepPrefetchX { static constexpr int value = FL_PREFETCH_X; };
 template <> struct SweepPrefetchX<float> { static constexpr int value = FL_PREFETCH_F32; };
+// Rows in flight of the FP64 sweeps in the fast numerics (x, y).
+#ifndef FL_PREFETCH_X_FAST
+#define FL_PREFETCH_X_FAST FL_PREFETCH_X
+#endif
+#ifndef FL_PREFETCH_FAST
+#define FL_PREFETCH_FAST FL_PREFETCH
+#endif
+template <typename T, bool F> struct PrefetchXOf { static constexpr int value = SweepPrefetchX<T>::value; };
+template <> struct PrefetchXOf<double, true> { static constexpr int value = FL_PREFETCH_X_FAST; };
+template <typename T, bool F> struct PrefetchYOf { static constexpr int value = SweepPrefetch<T>::value; };
+template <> struct PrefetchYOf<double, true> { static constexpr int value = FL_PREFETCH_FAST; };
+// Waves per SIMD of a sweep kernel in precision T with fast (F) or exact numerics.
+template <typename T, bool F> struct SweepWavesOf { static constexpr int value = SweepWaves<T>::value; };
+template <> struct SweepWavesOf<double, true> { static constexpr int value = FL_SWEEP_WAVES_FAST; };
 }  // namespace fl
 
 
@@ -116,6 +136,64 @@ __device__ __forceinline__ void issue_load(const SweepArgs<T>& a, int64_t x, int
   if (ND > 1) yi = map_ghost(y, a.ny, a.bcy, fy);
   load_cell<T, 2 + ND>(a.in, yi * a.nx + xi, a.es, a.cs, q.u);
   q.flip = (fx ? 1 : 0) | (fy ? 2 : 0);
+}
+
+// Loads and stores of the 2D sweeps.  The column a lane reads stays fixed
+// along its march and the row is workgroup-uniform, so a cell's address is a
+// scalar row base (plus the component offset) and a per-lane 32-bit byte
+// offset computed once: the loads and stores take the scalar-base address
+// form and spend no vector instructions on addresses.  Rows of 2D grids are
+// below 2^32 bytes (checked on the host).
+struct LaneCol {
+  uint32_t off;  // byte offset of the lane's ghost-mapped column within a row
+  int flip;      // 1: a reflective ghost column (x momentum flips)
+};
+
+template <typename T>
+__device__ __forceinline__ LaneCol lane_col(const SweepArgs<T>& a, int64_t x) {
+  bool fx = false;
+  const int64_t xi = map_ghost(x, a.nx, a.bcx, fx);
+  LaneCol l;
+  l.off = (uint32_t)(xi * a.es * (int64_t)sizeof(T));
+  l.flip = fx ? 1 : 0;
+  return l;
+}
+
+// Byte offset of column x (in the grid) for stores.
+template <typename T>
+__device__ __forceinline__ uint32_t col_off(const SweepArgs<T>& a, int64_t x) {
+  return (uint32_t)(x * a.es * (int64_t)sizeof(T));
+}
+
+// Row index of a 2D grid (ny < 2^31) through the ghost map, in 32 bits so
+// that the workgroup-uniform row arithmetic stays on the scalar unit.
+__device__ __forceinline__ int map_row(int y, int ny, int kind, bool& flip) {
+  y = y < -4 ? -4 : (y > ny + 3 ? ny + 3 : y);
+  flip = false;
+  if (y >= 0 && y < ny) return y;
+  if (kind == kBcPeriodic) return y < 0 ? y + ny : y - ny;
+  flip = (kind == kBcReflective);
+  return y < 0 ? -y - 1 : 2 * ny - 1 - y;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void issue_row(const SweepArgs<T>& a, int64_t y, const LaneCol& lc,
+                                          Pending<T, S>& q) {
+  bool fy = false;
+  const int yi = map_row((int)y, (int)a.ny, a.bcy, fy);
+  const char* row = (const char*)a.in + (int64_t)yi * (a.nx * a.es * (int64_t)sizeof(T));
+#pragma unroll
+  for (int k = 0; k < S; ++k)
+    q.u[k] = *(const T*)(row + k * a.cs * (int64_t)sizeof(T) + lc.off);
+  q.flip = lc.flip | (fy ? 2 : 0);
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void store_row(const SweepArgs<T>& a, int64_t y, uint32_t off,
+                                          const T* u) {
+  char* row = (char*)a.out + y * a.nx * a.es * (int64_t)sizeof(T);
+#pragma unroll
+  for (int k = 0; k < S; ++k) *(T*)(row + k * a.cs * (int64_t)sizeof(T) + off) = u[k];
 }
 
 template <typename T, int ND>
@@ -308,13 +386,20 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   // the compiler can count the loads issued after each one and wait only for
   // the one it needs.
   auto hrow = [&](int64_t r) { return (ND == 2) ? r - 3 : int64_t(0); };
-  constexpr int PF = SweepPrefetchX<T>::value;
+  // 2D: the lane's column and halo column, fixed along the march
+  const LaneCol lc = lane_col(a, c), lh = lane_col(a, hcol);
+  const uint32_t out_off = col_off(a, c);
+  auto issue = [&](int64_t x, int64_t r, const LaneCol& l, Pending<T, S>& q) {
+    if constexpr (ND == 2) issue_row<T, S>(a, r, l, q);
+    else issue_load<T, ND>(a, x, r, q);
+  };
+  constexpr int PF = PrefetchXOf<T, FAST>::value;
   Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
     const int64_t ri = min(rbeg + i, rlast);
-    issue_load<T, ND>(a, c + ri * cstep, ri, Qs[i]);
-    issue_load<T, ND>(a, hcol + ri * cstep, hrow(ri), Hs[i]);
+    issue(c + ri * cstep, ri, lc, Qs[i]);
+    issue(hcol + ri * cstep, hrow(ri), lh, Hs[i]);
   }
 
   // Step i of each group of PF rows owns buffer i.  The inner loop is fully
@@ -338,10 +423,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
     take<T, ND>(Q, Uw[3]);
     const int64_t rn = min(r + PF, rlast);
-    issue_load<T, ND>(a, c + rn * cstep, rn, Q);
+    issue(c + rn * cstep, rn, lc, Q);
     T uh[S];  // halo cell of output row hrow(r)
     take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol + rn * cstep, hrow(rn), H);
+    issue(hcol + rn * cstep, hrow(rn), lh, H);
     {
       const Prim<T, ND> w = prim_sel<F, T, ND>(Uw[3], a.mat);
       pw[4] = w.p;
@@ -423,7 +508,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       T o[S];
 #pragma unroll
       for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k][t - 1] - G[k]) * dtdh;
-      store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
+      if constexpr (ND == 2) store_row<T, S>(a, jo, out_off, o);
+      else store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
       if (a.ws_out) {
         const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
         wsmax = b > wsmax ? b : wsmax;
@@ -489,12 +575,14 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
   const int64_t rbeg = y0 - 4, rlast = yend + 3;
   // All loads are unconditional (see the x sweep); the halo cells of row r are
   // loaded on the same schedule as the row itself.
-  constexpr int PF = SweepPrefetch<T>::value;
+  constexpr int PF = PrefetchYOf<T, FAST>::value;
+  const LaneCol lc = lane_col(a, c), lh = lane_col(a, hcol);
+  const uint32_t out_off = col_off(a, c);
   Pending<T, S> Qs[PF], Hs[PF];
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
-    issue_load<T, ND>(a, c, min(rbeg + i, rlast), Qs[i]);
-    issue_load<T, ND>(a, hcol, min(rbeg + i, rlast), Hs[i]);
+    issue_row<T, S>(a, min(rbeg + i, rlast), lc, Qs[i]);
+    issue_row<T, S>(a, min(rbeg + i, rlast), lh, Hs[i]);
   }
   // Step i of each group of PF rows owns buffer i (see the x sweep).
   for (int64_t r0 = rbeg; r0 <= rlast; r0 += PF) {
@@ -516,10 +604,10 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       cxw[i] = cxw[i + 1];
     }
     take<T, ND>(Q, Uw[4]);
-    issue_load<T, ND>(a, c, min(r + PF, rlast), Q);
+    issue_row<T, S>(a, min(r + PF, rlast), lc, Q);
     T uh[S];
     take<T, ND>(H, uh);
-    issue_load<T, ND>(a, hcol, min(r + PF, rlast), H);
+    issue_row<T, S>(a, min(r + PF, rlast), lh, H);
     const Prim<T, ND> w = prim_sel<F, T, ND>(Uw[4], a.mat);
     pw[4] = w.p;
     vyw[3] = w.vel[1];
@@ -570,7 +658,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
           T o[S];
 #pragma unroll
           for (int k = 0; k < S; ++k) o[k] = Uw[0][k] + (Gprev[k] - G[k]) * dtdh;
-          store_cell<T, S>(a.out, jo * a.nx + c, a.es, a.cs, o);
+          store_row<T, S>(a, jo, out_off, o);
           if (a.ws_out) {
             const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
             wsmax = b > wsmax ? b : wsmax;

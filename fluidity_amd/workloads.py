@@ -223,9 +223,11 @@ class Riemann2D:
     rejected: int
 
 
-def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D:
-    """Four-quadrant Riemann problem on an n x n grid of [0,1]^2 with the
-    split point uniform in [0.3, 0.7]^2."""
+def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4,
+               ny: int | None = None) -> Riemann2D:
+    """Four-quadrant Riemann problem on an n x n grid of [0,1]^2 (ny x n rows
+    when ny is given, the split point in the same fractions of each side)
+    with the split point uniform in [0.3, 0.7]^2."""
     rng = SplitMix64(seed)
     x0 = rng.uniform(0.3, 0.7)
     y0 = rng.uniform(0.3, 0.7)
@@ -237,17 +239,19 @@ def riemann_2d(n: int = 4096, seed: int = 1001, gamma: float = 1.4) -> Riemann2D
         if ok:
             break
         rejected += 1
-    U = _quadrant_grid(n, x0, y0, q, gamma)
+    U = _quadrant_grid(n, x0, y0, q, gamma, ny)
     return Riemann2D(U, q, (x0, y0), gamma, 1.0 / n, rejected)
 
 
-def _quadrant_grid(n: int, x0: float, y0: float, q, gamma: float) -> np.ndarray:
-    """Conservative n x n grid of four quadrant states (Q1 x >= x0, y >= y0,
-    then anticlockwise); cell i is right of the split if i/n >= x0."""
-    x = np.arange(n) / n
-    right = x >= x0
-    top = x >= y0
-    W = np.empty((n, n, 4))
+def _quadrant_grid(n: int, x0: float, y0: float, q, gamma: float,
+                   ny: int | None = None) -> np.ndarray:
+    """Conservative ny x n grid (ny = n by default) of four quadrant states
+    (Q1 x >= x0, y >= y0, then anticlockwise); cell i is right of the split if
+    i/n >= x0, row j above it if j/ny >= y0."""
+    ny = n if ny is None else ny
+    right = np.arange(n) / n >= x0
+    top = np.arange(ny) / ny >= y0
+    W = np.empty((ny, n, 4))
     Q = [np.array(s) for s in q]
     for j_top in (False, True):
         for i_right in (False, True):

@@ -346,7 +346,9 @@ int fl_split_step_vjp(fl_context* ctx, const fl_field* state, const fl_field* la
 /* ---- level set and Riemann ghost fluid -------------------------------------
  * Level-set fields phi are FP64 device arrays of n[0] (1D) or n[0]*n[1] (2D)
  * values, x fastest, interior cells only; phi <= 0 inside the material.  State
- * fields given with them are FP64 (AoS or SoA) of the same shape.  `work`
+ * fields given with them are FP64 or FP32 (AoS or SoA) of the same shape; the
+ * level-set and ghost-fluid arithmetic runs in FP64 either way (FP32 states
+ * are widened on load and rounded on store).  `work`
  * arguments are caller-owned device scratch of the stated size, 256-byte
  * aligned, not aliasing any other argument. */
 
@@ -450,8 +452,15 @@ typedef struct fl_mm_params {
  * ghost cells the failed step had written, as the reference's loop would).  After
  * other errors (a HIP failure, an eikonal solve that does not converge) the
  * states and level sets are unspecified.  Stops when run_time >= sim_time or
- * iters >= max_iters.  work: fl_mm_work_bytes().  Synchronises `stream`
- * every step (and every few eikonal sweeps).  mp may be NULL (defaults). */
+ * iters >= max_iters.  dt is computed on the device (in the context's control
+ * block, which the sweeps and the level-set advection read); the host reads a
+ * copy of that block once per step, after the step's sweeps are queued, to
+ * test the loop condition, and waits for the eikonal solve's convergence once
+ * per reinitialisation (more often only when a solve needs more sweeps than
+ * the previous one).  Both materials' states (and work grids) share one
+ * dtype: FP64, or FP32 bulk storage (FP32 sweeps; dt, the wavespeed maximum,
+ * the level sets, the ghost-fluid states and the eikonal solve in FP64).
+ * work: fl_mm_work_bytes().  mp may be NULL (defaults). */
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n);
 int fl_mm_simulate(fl_context* ctx, fl_material* mats, int nmat, const fl_sim_params* params,
                    const fl_mm_params* mp, void* work, int64_t work_bytes,

@@ -54,6 +54,5 @@ def cons_field(kind: str, shape, seed: int, gamma: float = 1.4) -> np.ndarray:
     if kind == "riemann":
         if len(shape) == 1:
             return W.riemann_1d(n=shape[0], seed=seed, gamma=gamma).U
-        assert shape[0] == shape[1]
-        return W.riemann_2d(n=shape[0], seed=seed, gamma=gamma).U
+        return W.riemann_2d(n=shape[1], seed=seed, gamma=gamma, ny=shape[0]).U
     return np.ascontiguousarray(W.prim_to_cons(prim_field(kind, shape, seed, gamma), gamma))

@@ -185,7 +185,20 @@ def test_finite_differences_with_flow(A, E):
 
 
 def test_finite_differences_static_rho_gamma(A, E):
-    """The literal workload: rho-only and gamma-only directions."""
+    """The literal workload (u = v = 0, p = 1): rho-only directions against
+    central differences, and the gamma direction with an absolute bound.
+
+    The state is an exact steady contact for every gamma: pressure
+    (gamma - 1) E is the same in every cell and the velocity is zero, so each
+    face flux difference cancels and the density never changes; the loss does
+    not depend on gamma and dL/dgamma is 0 exactly.  The central difference
+    is therefore exactly 0, and the adjoint's value may differ from 0 only by
+    the rounding of its gamma terms (which cancel between faces), here held
+    to 1e-12 of loss / gamma.  Directions in momentum and energy start a flow
+    at a state where every limiter ratio and the HLLC star branches sit on
+    their branch points (zero slopes, u_L = u_R = 0), where the scheme is not
+    differentiable: those are checked on the flow workload
+    (test_finite_differences_with_flow)."""
     prob = W.adjoint_problem(1024)
     full = W.smooth_directions(1024, 8, seed=1006)
     dirs = []
@@ -193,7 +206,14 @@ def test_finite_differences_static_rho_gamma(A, E):
         rho_only = np.zeros_like(d)
         rho_only[..., 0] = d[..., 0]
         dirs.append((rho_only, 0.0))
-    dirs.append((np.zeros_like(full[0][0]), 1.0))
     errs = _fd_check(A, E, prob, dirs, 200, 1e-8)
-    print("static workload, rho/gamma directions: rel errs", ["%.3g" % e for e in errs])
-    assert max(errs[:4]) <= 1e-6
+    print("static workload, rho directions: rel errs", ["%.3g" % e for e in errs])
+    assert max(errs) <= 1e-6
+    loss, _, dg, _, _ = _adjoint_run(A, E, prob.U, prob.target, prob.dh, 200, 20)
+    eps = 1e-8
+    lp = _loss_fd(A, E, prob.U, prob.target, prob.gamma + eps, prob.dh, 200)
+    lm = _loss_fd(A, E, prob.U, prob.target, prob.gamma - eps, prob.dh, 200)
+    print(f"static workload: loss {loss:.6g}, dL/dgamma adjoint {dg:.3g}, "
+          f"central difference {(lp - lm) / (2 * eps):.3g}")
+    assert lp == lm == loss
+    assert abs(dg) <= 1e-12 * abs(loss) / prob.gamma

@@ -427,6 +427,7 @@ def test_simulator_continues_after_iteration_limit(E):
     sim = Simulator(dimensions=1)
     sim.configure_resolution(1.0 / 256)
     sim.configure_dimension(0, 0.0, 1.0)
+    sim.configure_sim_time(1.0)      # the reference's default sim_time is 0: no step
     sim.configure_max_iterations(5)
     sim.fill_data([("rho", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.125)),
                    ("p", lambda p: np.where(p[:, 0] < 0.5, 1.0, 0.1))])
