@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libfluidity_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["fl_sweep.hip", "fl_abi.hip", "fl_levelset.hip", "fl_ls_abi.hip", "fl_batch.hip",
-           "fl_batch_abi.hip", "fl_adjoint.hip", "fl_adj_abi.hip", "fl_io.hip"]
+           "fl_batch_abi.hip", "fl_adjoint.hip", "fl_adj_abi.hip", "fl_io.hip", "fl_noalloc.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -31,6 +31,7 @@ EXPORTED = [
     "fl_split_sweep",
     "fl_split_solve", "fl_split_step",
     "fl_max_wavespeed", "fl_set_wavespeeds", "fl_max_element", "fl_simulate",
+    "fl_simulate_noalloc",
     "fl_conservation_sums",
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
     "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate_work_bytes",
@@ -161,6 +162,8 @@ def _load():
     L.fl_split_step.argtypes = [vp, fp, fp, sp, C.c_double, C.POINTER(C.c_double), vp]
     L.fl_max_wavespeed.argtypes = [vp, fp, sp, C.POINTER(C.c_double), vp]
     L.fl_simulate.argtypes = [vp, fp, fp, sp, C.POINTER(SimParams), C.POINTER(SimResult), vp]
+    L.fl_simulate_noalloc.argtypes = [C.POINTER(Tuning), fp, fp, sp, C.POINTER(SimParams),
+                                      C.POINTER(SimResult), vp]
     L.fl_conservation_sums.argtypes = [vp, fp, C.POINTER(C.c_double), vp]
     i64p = C.POINTER(C.c_int64)
     L.fl_ls_masks.argtypes = [vp, C.c_int, i64p, C.c_double, vp, vp]

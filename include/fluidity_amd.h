@@ -198,6 +198,22 @@ int fl_max_element(fl_context* ctx, const double* first, int64_t count, double* 
 int fl_simulate(fl_context* ctx, fl_field* state, fl_field* work, const fl_scheme* scheme,
                 const fl_sim_params* params, fl_sim_result* result, void* stream);
 
+/* fl_simulate without a context and without allocating: for callers that
+ * cannot allocate device or pinned memory (serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The loop, its
+ * arguments and results are fl_simulate's; tuning (nullable: the built-in
+ * launch shapes) gives the launch shapes, and steps are launched one by one
+ * (no graphs).  The step-control block the kernels share lives in a
+ * file-scope device array of 256 bytes, one per GPU, held until the process
+ * ends; the loop needs one such block whatever the grid size, so no grid is
+ * too large for it.  The host reads the block into a stack copy once per 256
+ * steps.  Every call on one GPU uses that GPU's block: calls that overlap on
+ * one GPU (from several host threads or on several streams) race and give
+ * undefined results.  Synchronises `stream`. */
+int fl_simulate_noalloc(const fl_tuning* tuning, fl_field* state, fl_field* work,
+                        const fl_scheme* scheme, const fl_sim_params* params,
+                        fl_sim_result* result, void* stream);
+
 /* fl_simulate captures a pair of steps (control kernel + sweeps, both
  * parities) in a hipGraph on first use for a given grid, scheme, parameters
  * and tuning, and replays it (default); enable = 0 launches every kernel. */
