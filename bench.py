@@ -76,7 +76,7 @@ def parse():
                     help="sweeps: a plain device copy of one grid (HBM calibration for "
                          "counter runs) followed by --steps x sweeps and y sweeps")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
-                    help="precision of --mode sweeps")
+                    help="precision of --mode sweeps and --mode ghost")
     ap.add_argument("--flux", choices=["hllc", "force"], default="hllc",
                     help="flux of --mode sweeps")
     ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
@@ -111,7 +111,8 @@ def main():
         return
     if args.mode == "ghost":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
-        print(json.dumps(ghost_fluid_case(torch, N, W, stream, args.steps, args.warmup)),
+        print(json.dumps(ghost_fluid_case(torch, N, W, stream, args.steps, args.warmup,
+                                          dtype=torch.float32 if args.dtype == "f32" else None)),
               flush=True)
         return
     n = args.n
@@ -242,7 +243,9 @@ def main():
             torch.cuda.empty_cache()
 
         variants["riemann_1d_f64"] = riemann_1d_case(torch, N, W, stream)
-        variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=2)
+        variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5)
+        variants["ghost_fluid_f32"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5,
+                                                       dtype=torch.float32)
         variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
         variants["io_f64"] = io_case(torch, N, W, stream)
         bf = batched_case(torch, N, W, stream, steps=20)
@@ -512,20 +515,25 @@ def io_case(torch, N, W, stream, n=4096, reps=20, text_n=512):
     return res
 
 
-def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024):
+def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype=None):
     """Two-material ghost-fluid steps on the seeded shock-bubble workload
     (ideal-gas bubble, level sets reinitialised every 5 steps in a 6-cell
     band).  A cell-update is one cell of the domain advanced one full
     multimaterial step (both materials' sweeps, ghost states, level-set
-    advection, projection and the share of reinitialisation)."""
+    advection, projection and the share of reinitialisation).  With 5 warm-up
+    steps the first reinitialisation falls in the warm-up call, so the timed
+    steps are the workload's steady state (level sets clamped outside the
+    band, as after every reinitialisation).  dtype float32: FP32 bulk states
+    (FP64 level sets, dt and ghost-fluid arithmetic)."""
     from fluidity_amd import euler as E
     from fluidity_amd import levelset as L
     sb = W.shock_bubble(nx, ny, seed=1002)
     dev = torch.device("cuda:0")
     sim = L.MultimaterialSimulator(layout=N.FL_LAYOUT_SOA)
+    sdt = dtype or torch.float64
     with torch.cuda.stream(stream):
-        U0 = torch.from_numpy(W.aos_to_soa(sb.U0)).to(dev)
-        U1 = torch.from_numpy(W.aos_to_soa(sb.U1)).to(dev)
+        U0 = torch.from_numpy(W.aos_to_soa(sb.U0)).to(dev, dtype=sdt)
+        U1 = torch.from_numpy(W.aos_to_soa(sb.U1)).to(dev, dtype=sdt)
         p0 = torch.from_numpy(sb.phi0).to(dev)
         p1 = torch.from_numpy(sb.phi1).to(dev)
     stream.synchronize()
@@ -542,8 +550,9 @@ def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024):
     cells = nx * ny
     return {"value": round(steps * cells / el, 1), "unit": "cell-updates/s",
             "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps,
-            "workload": f"shock-bubble {nx}x{ny} FP64, 2 materials, Mach {sb.mach:.3f}, "
-                        "reinit every 5 steps, band 6, SoA"}
+            "warmup": warmup,
+            "workload": f"shock-bubble {nx}x{ny} {'FP32 bulk' if dtype else 'FP64'}, 2 materials, "
+                        f"Mach {sb.mach:.3f}, reinit every 5 steps, band 6, SoA"}
 
 
 def sweeps_mode(args, wl, torch, E, N, W):

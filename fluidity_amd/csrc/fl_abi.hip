@@ -167,9 +167,11 @@ template <typename T, int FLUX, bool FAST>
 void launch_flux(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
                  hipStream_t stream) {
   if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX, FAST>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX, FAST>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a);
   else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a);
   else
     hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST>), grid, dim3(B), 0, stream, a);
 }
@@ -183,7 +185,8 @@ int sweep_blocks_per_cu(int dim, int B) {
   const void* k = dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX, FAST>)
                            : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX, FAST>);
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, 0) != hipSuccess) return 0;
+  const size_t lds = dim == 0 ? fl::x_tile_lds_bytes(B, 4, sizeof(T)) : 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, lds) != hipSuccess) return 0;
   return n;
 }
 
@@ -287,9 +290,11 @@ template <typename T, int FLUX>
 void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
                   const fl::SweepArgs<T>& a1, hipStream_t stream) {
   if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a0, a1);
   else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a0, a1);
   else
     hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX>), grid, dim3(B), 0, stream, a0, a1);
 }

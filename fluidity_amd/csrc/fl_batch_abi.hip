@@ -168,7 +168,8 @@ int apply_tuning(fl_batch* b, const fl_tuning& t) {
     const int B = ph ? t.y_block : t.x_block;
     int per_cu = 0;
     FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, phase_kernel_ptr(b->dtype, b->flux, ph), B, 0));
+        &per_cu, phase_kernel_ptr(b->dtype, b->flux, ph), B,
+        ph == 0 ? fl::x_tile_lds_bytes(B, 4, b->dtype == FL_F64 ? 8 : 4) : 0));
     if (per_cu < 1) per_cu = 1;
     b->grid[ph] = per_cu * prop.multiProcessorCount;
   }
@@ -183,7 +184,8 @@ void launch_phases_t(const fl_batch* b, hipStream_t st, int timing, double timin
   const bt::Prob<T>* probs = (const bt::Prob<T>*)b->probs;
   if (only_phase != 1 && b->ntiles[0] > 0)
     hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 0>), dim3(b->grid[0]),
-                       dim3(b->tuning.x_block), 0, st, probs, (const bt::Tile*)b->tiles[0],
+                       dim3(b->tuning.x_block), fl::x_tile_lds_bytes(b->tuning.x_block, 4, sizeof(T)),
+                       st, probs, (const bt::Tile*)b->tiles[0],
                        b->ntiles[0], b->cnt, timing, (T)timing_dtdh);
   if (only_phase != 0 && b->ntiles[1] > 0)
     hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1>), dim3(b->grid[1]),

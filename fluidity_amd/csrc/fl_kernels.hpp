@@ -46,6 +46,12 @@ struct SweepArgs {
 constexpr int kMaxBlock = 256;
 constexpr int kHalo = 3;  // extra cells each side staged in LDS for chi
 
+// Dynamic LDS of an x-sweep tile of B lanes over states of S components of
+// tsize bytes: S + 3 rows of B + 2 kHalo staged columns, 2 S rows of B.
+__host__ __device__ constexpr size_t x_tile_lds_bytes(int B, int S, int tsize) {
+  return (size_t)((S + 3) * (B + 2 * kHalo) + 2 * S * B) * (size_t)tsize;
+}
+
 // Standalone wavespeed max: at most kMaxWsBlocks workgroups of kMaxBlock
 // threads, each thread loading kMaxWsUnroll cells per iteration.
 constexpr int kMaxWsBlocks = 512;

@@ -322,14 +322,12 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
 //  stage 1: out = 0.75 phi_n + 0.25 (s1 - dt L(s1))
 //  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
 // Launched on a (ceil(nx / blockDim.x), min(ny, 65535)) grid; rows stride by
-// gridDim.y.  dt_dev (nullable): dt in device memory (the step's control
-// block), read instead of dt.
+// gridDim.y.
 template <int ND>
-__global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
-                                 const double* __restrict__ src, const double* __restrict__ V,
-                                 Grid g, double dt, const double* dt_dev, double h1,
-                                 double* __restrict__ out) {
-  if (dt_dev) dt = *dt_dev;
+__device__ __forceinline__ void rk3_stage_body(int stage, const double* __restrict__ phi_n,
+                                               const double* __restrict__ src,
+                                               const double* __restrict__ V, const Grid& g,
+                                               double dt, double h1, double* __restrict__ out) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t x0 = (int64_t)blockIdx.x * blockDim.x;
   const bool xin = x0 >= 3 && x0 + blockDim.x + 3 <= g.nx;  // workgroup-uniform
@@ -349,6 +347,26 @@ __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
       out[i] = (fact_13 * phi_n[i]) + (fact_23 * s3);
     }
   }
+}
+
+// dt_dev (nullable): dt in device memory (the step's control block), read
+// instead of dt.
+template <int ND>
+__global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
+                                 const double* __restrict__ src, const double* __restrict__ V,
+                                 Grid g, double dt, const double* dt_dev, double h1,
+                                 double* __restrict__ out) {
+  if (dt_dev) dt = *dt_dev;
+  rk3_stage_body<ND>(stage, phi_n, src, V, g, dt, h1, out);
+}
+
+// The same stage of two level sets' advection in one launch (blockIdx.z
+// selects the job), dt in device memory.
+template <int ND>
+__global__ void rk3_stage2_kernel(int stage, Rk3Job j0, Rk3Job j1, Grid g,
+                                  const double* dt_dev, double h1) {
+  const Rk3Job j = blockIdx.z ? j1 : j0;
+  rk3_stage_body<ND>(stage, j.phi_n, j.src, j.V, g, *dt_dev, h1, j.out);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,6 +1008,8 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
                                                StateView, int, Grid, double, int, double*); \
   template __global__ void velocity2_kernel<ND>(const double*, const double*, StateView,     \
                                                 StateView, Grid, double, double*, double*); \
+  template __global__ void rk3_stage2_kernel<ND>(int, Rk3Job, Rk3Job, Grid, const double*,    \
+                                                 double);                                     \
   template __global__ void rk3_stage_kernel<ND>(int, const double*, const double*,            \
                                                 const double*, Grid, double, const double*,  \
                                                 double, double*);                            \

@@ -67,6 +67,16 @@ template <int ND>
 __global__ void velocity2_kernel(const double* phi0, const double* phi1, StateView u0,
                                  StateView u1, Grid g, double dh, double* V0, double* V1);
 __global__ void project2_kernel(double* phi0, double* phi1, int64_t count);
+// One level set's RK3 stage: phi_n, the stage source, the speed and the output.
+struct Rk3Job {
+  const double* phi_n;
+  const double* src;
+  const double* V;
+  double* out;
+};
+template <int ND>
+__global__ void rk3_stage2_kernel(int stage, Rk3Job j0, Rk3Job j1, Grid g, const double* dt_dev,
+                                  double h1);
 template <int ND>
 __global__ void rk3_stage_kernel(int stage, const double* phi_n, const double* src,
                                  const double* V, Grid g, double dt, const double* dt_dev,

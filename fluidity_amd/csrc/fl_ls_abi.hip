@@ -128,6 +128,30 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
   return FL_OK;
 }
 
+// RK3 of both level sets, each stage of both in one launch: cur[m] -> out[m]
+// with speeds V[m], dt in device memory; work holds 4 cells of scratch.
+int launch_advect2(double* const cur[2], double* const V[2], const ls::Grid& g, double dh,
+                   double* const out[2], double* work, const double* dt_dev, hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
+  double* s1[2] = {work, work + cells};
+  double* s2[2] = {work + 2 * cells, work + 3 * cells};
+  const dim3 grid((unsigned)((g.nx + kBlock - 1) / kBlock),
+                  (unsigned)(g.ny < 65535 ? g.ny : 65535), 2);
+  const double h1 = 1.0 / dh;
+  auto k = g.nd == 1 ? ls::rk3_stage2_kernel<1> : ls::rk3_stage2_kernel<2>;
+  ls::Rk3Job j[3][2];
+  for (int m = 0; m < 2; ++m) {
+    j[0][m] = {cur[m], cur[m], V[m], s1[m]};
+    j[1][m] = {cur[m], s1[m], V[m], s2[m]};
+    j[2][m] = {cur[m], s2[m], V[m], out[m]};
+  }
+  for (int stage = 0; stage < 3; ++stage)
+    hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, st, stage, j[stage][0], j[stage][1], g, dt_dev,
+                       h1);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
 int launch_project(double* phi0, double* phi1, int64_t cells, hipStream_t st) {
   hipLaunchKernelGGL(ls::project2_kernel, grid_for(cells), dim3(kBlock), 0, st, phi0, phi1,
                      cells);
@@ -375,7 +399,7 @@ struct MmWork {
   double* star[2];
   void* extrap[2];  // extrap_work_bytes each
   double* V[2];
-  double* adv;      // 2 * cells
+  double* adv;      // 4 * cells (RK3 stages of both level sets)
   double* tphi[2];
   void* reinit[2];  // reinit_work_bytes each
   uint8_t* has[2];
@@ -399,7 +423,7 @@ MmWork mm_layout(void* base, const ls::Grid& g) {
   w.extrap[1] = take(extrap_work_bytes(g));
   w.V[0] = (double*)take(cells * 8);
   w.V[1] = (double*)take(cells * 8);
-  w.adv = (double*)take(2 * cells * 8);
+  w.adv = (double*)take(4 * cells * 8);
   w.tphi[0] = (double*)take(cells * 8);
   w.tphi[1] = (double*)take(cells * 8);
   w.reinit[0] = take(reinit_work_bytes(g));
@@ -660,9 +684,7 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     // pass), advection with the device dt, projection, reinitialisation every
     // `every` steps
     if ((rc = launch_velocity2(cur, mats[0].state, mats[1].state, g, dh, w.V, st))) return rc;
-    for (int m = 0; m < 2; ++m)
-      if ((rc = launch_advect(cur[m], w.V[m], g, 0.0, dh, nxt[m], w.adv, st, &ctl->dt)))
-        return rc;
+    if ((rc = launch_advect2(cur, w.V, g, dh, nxt, w.adv, &ctl->dt, st))) return rc;
     if ((rc = launch_project(nxt[0], nxt[1], cells, st))) return rc;
     const bool reinit = (it % every) == every - 1;
     if (reinit) {  // both level sets in one solve, written back into cur[m]

@@ -182,9 +182,19 @@ __device__ __forceinline__ void issue_row(const SweepArgs<T>& a, int64_t y, cons
   bool fy = false;
   const int yi = map_row((int)y, (int)a.ny, a.bcy, fy);
   const char* row = (const char*)a.in + (int64_t)yi * (a.nx * a.es * (int64_t)sizeof(T));
+  // The component row bases stay opaque scalars and the lane offset an
+  // opaque 32-bit value, so that the compiler cannot fold them into hoisted
+  // 64-bit per-lane pointers (vector address arithmetic on every row): each
+  // load is a scalar base plus a 32-bit lane offset.
+  uint32_t off = lc.off;
+  asm volatile("" : "+v"(off));
 #pragma unroll
-  for (int k = 0; k < S; ++k)
-    q.u[k] = *(const T*)(row + k * a.cs * (int64_t)sizeof(T) + lc.off);
+  for (int k = 0; k < S; ++k) {
+    const char* base = row + k * a.cs * (int64_t)sizeof(T);
+    asm volatile("" : "+s"(base));
+    q.u[k] = *(const __attribute__((address_space(1))) T*)(
+        (const __attribute__((address_space(1))) char*)base + off);
+  }
   q.flip = lc.flip | (fy ? 2 : 0);
 }
 
@@ -192,8 +202,14 @@ template <typename T, int S>
 __device__ __forceinline__ void store_row(const SweepArgs<T>& a, int64_t y, uint32_t off,
                                           const T* u) {
   char* row = (char*)a.out + y * a.nx * a.es * (int64_t)sizeof(T);
+  asm volatile("" : "+v"(off));
 #pragma unroll
-  for (int k = 0; k < S; ++k) *(T*)(row + k * a.cs * (int64_t)sizeof(T) + off) = u[k];
+  for (int k = 0; k < S; ++k) {
+    char* base = row + k * a.cs * (int64_t)sizeof(T);
+    asm volatile("" : "+s"(base));
+    *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)base + off) =
+        u[k];
+  }
 }
 
 template <typename T, int ND>
@@ -327,13 +343,18 @@ template <typename T, int ND, int FLUX, bool FAST = false>
 __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int S = 2 + ND;
   constexpr bool F = UseFast<T, FAST>::value;
-  constexpr int W = kMaxBlock + 2 * kHalo;
-  __shared__ T sU[S][W];
-  __shared__ T sP[W];
-  __shared__ T sV[W];
-  __shared__ T sK[W];
-  __shared__ T sL[S][kMaxBlock];
-  __shared__ T sG[S][kMaxBlock];
+  // LDS sized for this launch's block width (dynamic shared memory of
+  // x_tile_lds_bytes): states, pressures, x velocities and flattening
+  // factors of the B + 6 staged columns, then the left face states and
+  // fluxes of the B lanes.
+  extern __shared__ __align__(16) unsigned char fl_x_lds[];
+  const int W = (int)blockDim.x + 2 * kHalo;
+  T* const sU = reinterpret_cast<T*>(fl_x_lds);  // [S][W]
+  T* const sP = sU + S * W;
+  T* const sV = sP + W;
+  T* const sK = sV + W;
+  T* const sL = sK + W;                          // [S][B]
+  T* const sG = sL + S * (int)blockDim.x;        // [S][B]
 
   T dtdh;
   if (!sweep_dtdh(a, dtdh)) return;
@@ -448,13 +469,13 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 
     // ---- phase A: stage the row's states, pressures and x velocities
 #pragma unroll
-    for (int k = 0; k < S; ++k) sU[k][s] = U0[k];
+    for (int k = 0; k < S; ++k) sU[k * W + s] = U0[k];
     sP[s] = p0;
     sV[s] = vx0;
     if (t < 2 * kHalo) {
       const Prim<T, ND> wh = prim_sel<F, T, ND>(uh, a.mat);
 #pragma unroll
-      for (int k = 0; k < S; ++k) sU[k][hslot] = uh[k];
+      for (int k = 0; k < S; ++k) sU[k * W + hslot] = uh[k];
       sP[hslot] = wh.p;
       sV[hslot] = wh.vel[0];
     }
@@ -472,8 +493,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     T um[S], up[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) {
-      um[k] = sU[k][s - 1];
-      up[k] = sU[k][s + 1];
+      um[k] = sU[k * W + s - 1];
+      up[k] = sU[k * W + s + 1];
     }
     __syncthreads();
 
@@ -487,7 +508,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     T R[S], L[S];
     evolve_sel<F, T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
 #pragma unroll
-    for (int k = 0; k < S; ++k) sL[k][t] = L[k];
+    for (int k = 0; k < S; ++k) sL[k * B + t] = L[k];
     __syncthreads();
 
     // ---- phase D: face flux between column c and c+1
@@ -495,10 +516,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     if (t < B - 1) {
       T Lr[S];
 #pragma unroll
-      for (int k = 0; k < S; ++k) Lr[k] = sL[k][t + 1];
+      for (int k = 0; k < S; ++k) Lr[k] = sL[k * B + t + 1];
       flux_sel<F, T, ND, 0, FLUX>(R, Lr, dtdh, dhdt, a.mat, G);
 #pragma unroll
-      for (int k = 0; k < S; ++k) sG[k][t] = G[k];
+      for (int k = 0; k < S; ++k) sG[k * B + t] = G[k];
     }
     __syncthreads();
 
@@ -507,7 +528,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     if (t >= 1 && t <= B - 2 && cr < a.nx) {
       T o[S];
 #pragma unroll
-      for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k][t - 1] - G[k]) * dtdh;
+      for (int k = 0; k < S; ++k) o[k] = U0[k] + (sG[k * B + t - 1] - G[k]) * dtdh;
       if constexpr (ND == 2) store_row<T, S>(a, jo, out_off, o);
       else store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
       if (a.ws_out) {

@@ -13,7 +13,8 @@ ghost_fluid/ghost_fluid_extrapolator.hpp:54-202), ``combine_into_first``
 (simulator/multimaterial_simulator.hpp:227-303).
 
 Level sets are float64 device tensors of shape (ny, nx) or (nx,); states are
-float64 grids as in ``euler.grid_field`` (AoS or SoA).
+float64 grids as in ``euler.grid_field`` (AoS or SoA), or float32 grids for
+the FP32 bulk path (the level-set and ghost-fluid arithmetic stays FP64).
 """
 from __future__ import annotations
 

@@ -184,6 +184,39 @@ def test_simulate_2d_scaled_states_bitwise(E, oracle, k):
         assert _bitwise(got, ref), _report(got, ref)
 
 
+def test_division_window_and_edges(E):
+    """The exact FP64 division (fl_numerics.hpp dv: v_rcp_f64, two Newton
+    steps, a residual correction and v_div_fixup, without the IEEE
+    sequence's operand rescaling) through the primitive-planes kernel, whose
+    velocity plane is one quotient m / rho per cell: bit-identical to IEEE
+    m / rho for divisors from 2^-900 to 2^900 with normal quotients and for
+    divisors of 2^1023 and 1.5 * 2^1023, and the recorded outcome outside
+    the window: a denormal divisor (2^-1060, 2^-1030; its reciprocal
+    overflows) gives +inf where IEEE gives 0.5.  No state of a physical run
+    comes near these magnitudes."""
+    from fluidity_amd import simulator as SM
+    rng = np.random.default_rng(11)
+    inside = np.concatenate([2.0 ** rng.uniform(-900, 900, 4000), [2.0 ** -900, 2.0 ** 900,
+                                                                  1.0, 3.0, 0.1, 7.0]])
+    quot = rng.uniform(0.25, 4.0, inside.size) * np.where(rng.random(inside.size) < 0.5, -1, 1)
+    edge_rho = np.array([2.0 ** -1060, 2.0 ** -1030, 2.0 ** 1023, 1.5 * 2.0 ** 1023])
+    rho = np.concatenate([inside, edge_rho])
+    m = np.concatenate([quot * inside, 0.5 * edge_rho])
+    U = np.stack([rho, np.ones_like(rho), m], axis=-1)   # [rho, E, m]: 1D AoS
+    prim = SM.primitive_planes(_dev(U), E.Scheme(), 0)
+    vel = prim[2].cpu().numpy()
+    ieee = m / rho
+    k = inside.size
+    assert _bitwise(vel[:k], ieee[:k]), _report(vel[:k], ieee[:k])
+    got = vel[k:]
+    print("dv outside the window (rho, GPU, IEEE):",
+          [(float(r), float(g), float(q)) for r, g, q in zip(edge_rho, got, ieee[k:])],
+          file=sys.stderr)
+    assert np.all(ieee[k:] == 0.5)
+    assert np.all(got[:2] == np.inf)
+    assert np.all(got[2:] == 0.5)
+
+
 def test_simulate_1d_config0_bitwise(E, oracle):
     """The 1D Riemann workload (4096 cells, run until the fastest wave has
     travelled 0.4) reproduces the oracle bit for bit, including the step
