@@ -434,6 +434,119 @@ double bits_to_double(unsigned long long b) {
 
 using namespace fl::abi;
 
+// ---- persistent 1D step loop -------------------------------------------------
+
+namespace {
+
+constexpr int kPersistBlock = 64;              // lanes per workgroup: one 62-cell segment
+constexpr int64_t kPersistMaxCells = 1 << 16;  // larger lines: steps outlast a launch
+constexpr int64_t kPersistChunk = 1 << 16;     // steps per launch (then the host looks)
+// Polls of a grid barrier before the loop gives up (each poll sleeps about
+// 128 cycles after an uncached load: a fraction of a second, against a few
+// microseconds per step).
+constexpr unsigned int kPersistSpin = 1u << 18;
+
+template <typename T, int FLUX>
+const void* persist_kernel_ptr() {
+  return reinterpret_cast<const void*>(&fl::persist1d_kernel<T, FLUX>);
+}
+
+template <typename T>
+const void* persist_kernel_for(int flux) {
+  switch (flux) {
+    case FL_FLUX_HLLC: return persist_kernel_ptr<T, fl::kFluxHllc>();
+    case FL_FLUX_FORCE: return persist_kernel_ptr<T, fl::kFluxForce>();
+    case FL_FLUX_LAX_FRIEDRICHS: return persist_kernel_ptr<T, fl::kFluxLaxFriedrichs>();
+    default: return persist_kernel_ptr<T, fl::kFluxRichtmyer>();
+  }
+}
+
+// Workgroups of the persistent loop for a line of n cells, or 0 when it does
+// not apply (long lines, fast FP64 numerics, or more segments than the
+// device holds at once: a cooperative launch needs them all resident).
+int persist_blocks(const fl_field* f, const fl_scheme* s) {
+  if (f->ndim != 1 || f->n[0] > kPersistMaxCells) return 0;
+  if (f->dtype == FL_F64 && fast_numerics(s)) return 0;
+  const int64_t nseg = (f->n[0] + kPersistBlock - 3) / (kPersistBlock - 2);
+  const void* k = f->dtype == FL_F64 ? persist_kernel_for<double>(s->flux)
+                                     : persist_kernel_for<float>(s->flux);
+  int per_cu = 0, dev = 0, cus = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+      !coop)
+    return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const size_t lds = fl::x_tile_lds_bytes(kPersistBlock, 3, f->dtype == FL_F64 ? 8 : 4);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kPersistBlock, lds) != hipSuccess)
+    return 0;
+  return (nseg <= (int64_t)per_cu * cus) ? (int)nseg : 0;
+}
+
+// fl_simulate on a short 1D line: the initial maximum, then the step loop in
+// cooperative launches of up to kPersistChunk steps each.
+template <typename T>
+int simulate_persist1d(fl_context* c, fl_field* state, fl_field* work, const fl_scheme* s,
+                       const fl_sim_params* p, int nblocks, fl_sim_result* result,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(fl::zero_u64_kernel, dim3(1), dim3(1), 0, st, c->ws_tmp);
+  int rc = launch_max_ws(state, s, c->ws_tmp, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fl::persist_init_kernel, dim3(1), dim3(1), 0, st, c->pctl,
+                     (const unsigned long long*)c->ws_tmp);
+  FL_HIP(hipGetLastError());
+  fl::SweepArgs<T> a_ab = make_args<T>(c, state, work, s, 0.0, nullptr, nullptr, 1);
+  fl::SweepArgs<T> a_ba = make_args<T>(c, work, state, s, 0.0, nullptr, nullptr, 1);
+  const void* k = persist_kernel_for<T>(s->flux);
+  const size_t lds = fl::x_tile_lds_bytes(kPersistBlock, 3, sizeof(T));
+  double cfl = p->cfl, dh = p->dh, sim_time = p->sim_time;
+  long long max_iters = (long long)p->max_iters;
+  unsigned int spin = kPersistSpin;
+  long long k0 = 0, iters = 0;
+  fl::PersistCtl h;
+  memset(&h, 0, sizeof(h));
+  for (;;) {
+    long long nsteps = max_iters - k0;
+    if (nsteps > kPersistChunk) nsteps = kPersistChunk;
+    if (nsteps > 0) {
+      hipLaunchKernelGGL(fl::persist_bar_reset_kernel, dim3(1), dim3(1), 0, st, c->pctl);
+      void* args[] = {&a_ab, &a_ba, &c->pctl, &cfl, &dh, &sim_time, &max_iters, &k0, &nsteps,
+                      &spin};
+      FL_HIP(hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(kPersistBlock), args, lds, st));
+    }
+    FL_HIP(hipMemcpyAsync(c->host_pctl, c->pctl, sizeof(fl::PersistCtl), hipMemcpyDeviceToHost,
+                          st));
+    FL_HIP(hipStreamSynchronize(st));
+    h = *c->host_pctl;
+    if (h.err)
+      return fail(FL_ERR_HIP, "persistent step loop: a grid barrier timed out at step %lld",
+                  (long long)(h.iters[0] > h.iters[1] ? h.iters[0] : h.iters[1]));
+    iters = h.iters[0] > h.iters[1] ? h.iters[0] : h.iters[1];
+    // the loop stopped early (condition false) or ran the chunk
+    if (nsteps <= 0 || iters < k0 + nsteps) break;
+    k0 += nsteps;
+  }
+  if (iters & 1) {
+    const int64_t bytes = cell_count(state) * 3 * (int64_t)sizeof(T);
+    FL_HIP(hipMemcpyAsync(state->data, work->data, (size_t)bytes, hipMemcpyDeviceToDevice, st));
+    FL_HIP(hipStreamSynchronize(st));
+  }
+  const double t = __builtin_bit_cast(double, h.t[iters & 1]);
+  const double final_ws = bits_to_double(h.ws[iters % 3]);
+  if (result) {
+    result->run_time = t;
+    result->iters = iters;
+    result->last_dt = iters > 0 ? h.dt : 0.0;
+    result->max_wavespeed = final_ws;
+  }
+  if (!(final_ws > 0.0) || isnan(final_ws))
+    return fail(FL_ERR_NUMERIC, "max wavespeed %g after %lld steps (NaN or vacuum)", final_ws,
+                iters);
+  return FL_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* fl_status_string(int status) {
@@ -473,6 +586,8 @@ int fl_ctx_create(int device, fl_context** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->eik, 2 * sizeof(fl::ls::EikCtl));
   if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_eik, 2 * sizeof(fl::ls::EikCtl), 0);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->pctl, sizeof(fl::PersistCtl));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_pctl, sizeof(fl::PersistCtl), 0);
   if (e != hipSuccess) {
     fl_ctx_destroy(c);
     return fail(FL_ERR_HIP, "context allocation failed: %s", hipGetErrorString(e));
@@ -497,6 +612,8 @@ int fl_ctx_destroy(fl_context* c) {
   if (c->eik) (void)hipFree(c->eik);
   if (c->host_eik) (void)hipHostFree(c->host_eik);
   if (c->step_ev) (void)hipEventDestroy(c->step_ev);
+  if (c->pctl) (void)hipFree(c->pctl);
+  if (c->host_pctl) (void)hipHostFree(c->host_pctl);
   delete c;
   return FL_OK;
 }
@@ -755,6 +872,15 @@ int fl_simulate(fl_context* c, fl_field* state, fl_field* work, const fl_scheme*
   if (isnan(p->sim_time)) return fail(FL_ERR_ARGUMENT, "sim_time is NaN");
   DeviceGuard g(c->device);
   hipStream_t st = (hipStream_t)stream;
+
+  if (c->use_graphs) {
+    const int nb = persist_blocks(state, s);
+    if (nb > 0) {
+      if (state->dtype == FL_F64)
+        return simulate_persist1d<double>(c, state, work, s, p, nb, result, st);
+      return simulate_persist1d<float>(c, state, work, s, p, nb, result, st);
+    }
+  }
 
   hipLaunchKernelGGL(fl::init_control_kernel, dim3(1), dim3(1), 0, st, c->ctl);
   rc = launch_max_ws(state, s, &c->ctl->ws_bits[0], st);

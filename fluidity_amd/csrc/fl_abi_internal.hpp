@@ -36,6 +36,9 @@ struct fl_context {
   hipEvent_t step_ev;          // fl_mm_simulate: the step's control block is on the host
   int64_t eik_sweeps;          // sweeps the last eikonal solve took (first batch size)
   int use_graphs;              // fl_simulate replays a captured pair of steps
+                               // (small 1D grids: runs the persistent loop)
+  fl::PersistCtl* pctl;        // device control words of the persistent 1D loop
+  fl::PersistCtl* host_pctl;   // pinned host mirror of pctl
   hipStream_t cap_stream;      // capture stream of that graph
   hipGraph_t sim_graph;        // a pair of steps (both parities)
   hipGraphExec_t sim_exec;

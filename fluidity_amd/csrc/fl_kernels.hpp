@@ -24,6 +24,20 @@ struct StepCtl {
   int pad;
 };
 
+// Control words of the persistent 1D step loop (persist1d_kernel): the
+// fused wavespeed maxima in three slots (step k reads slot k % 3, reduces into
+// (k + 1) % 3 and clears (k + 2) % 3), run time and iteration count in two
+// (step k reads k % 2, writes (k + 1) % 2), the last dt, the grid-barrier
+// counter of the launch and a timeout flag.
+struct PersistCtl {
+  unsigned long long ws[3];
+  unsigned long long t[2];  // run time as bits
+  long long iters[2];
+  double dt;
+  unsigned int bar;
+  int err;
+};
+
 template <typename T>
 struct SweepArgs {
   const T* in;
@@ -86,4 +100,10 @@ __global__ void step_control_kernel(StepCtl* ctl, int parity, double cfl, double
                                     double sim_time, long long max_iters);
 __global__ void init_control_kernel(StepCtl* ctl);
 __global__ void mm_control_kernel(StepCtl* ctl, double cfl, double dh);
+template <typename T, int FLUX>
+__global__ void persist1d_kernel(SweepArgs<T> a_ab, SweepArgs<T> a_ba, PersistCtl* pc,
+                                 double cfl, double dh, double sim_time, long long max_iters,
+                                 long long k0, long long nsteps, unsigned int spin_limit);
+__global__ void persist_init_kernel(PersistCtl* pc, const unsigned long long* ws0);
+__global__ void persist_bar_reset_kernel(PersistCtl* pc);
 }  // namespace fl

@@ -211,6 +211,99 @@ __global__ void mm_control_kernel(StepCtl* ctl, double cfl, double dh) {
   ctl->active = ok ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// persistent 1D step loop
+// ---------------------------------------------------------------------------
+
+// Grid-wide barrier of a cooperative launch: every workgroup adds one to the
+// counter and waits until it reaches `target`.  The wait gives up after
+// spin_limit polls, sets *err and returns false (every workgroup then stops:
+// the counter cannot reach the target of any later barrier either), so the
+// kernel ends even if the barrier never completes.  Release/acquire at agent
+// scope make the stores before the barrier visible to every workgroup after it.
+__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int target,
+                                             unsigned int spin_limit, int* err) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned int spins = 0;
+    int good = 1;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > spin_limit ||
+          __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__device__ __forceinline__ unsigned long long load_u64_agent(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The whole step loop of a 1D grid in one cooperative launch: steps k0 ..
+// k0 + nsteps - 1 while run_time < sim_time and iters < max_iters (the
+// reference's loop condition, simulator/simulator_impl.hpp:235-252).  Every
+// workgroup reads the same control words, so every workgroup computes the
+// same dt = cfl dh / max (update_time_delta, simulator/parameters.hpp:60-99)
+// and stops at the same step.  Workgroup b sweeps segment b of the line
+// (sweep_x_tile, one segment per workgroup), reducing the new wavespeeds into
+// the next slot; a grid barrier ends each step.  Step k reads a_ab (state ->
+// work) for even k and a_ba for odd k.
+template <typename T, int FLUX>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
+persist1d_kernel(SweepArgs<T> a_ab, SweepArgs<T> a_ba, PersistCtl* pc, double cfl, double dh,
+                 double sim_time, long long max_iters, long long k0, long long nsteps,
+                 unsigned int spin_limit) {
+  for (long long k = k0; k < k0 + nsteps; ++k) {
+    const double maxws = __longlong_as_double((long long)load_u64_agent(&pc->ws[k % 3]));
+    const double t = __longlong_as_double((long long)load_u64_agent(&pc->t[k & 1]));
+    const long long it =
+        (long long)load_u64_agent(reinterpret_cast<unsigned long long*>(&pc->iters[k & 1]));
+    if (!(t < sim_time && it < max_iters)) return;  // the same for every workgroup
+    const double dt = cfl * dh / maxws;
+    const double dtdh = dt / dh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      pc->t[(k + 1) & 1] = (unsigned long long)__double_as_longlong(t + dt);
+      pc->iters[(k + 1) & 1] = it + 1;
+      pc->dt = dt;
+      pc->ws[(k + 2) % 3] = 0ull;
+    }
+    SweepArgs<T> a = (k & 1) ? a_ba : a_ab;
+    a.ctl = nullptr;
+    a.dtdh_host = (T)dtdh;
+    a.ws_out = &pc->ws[(k + 1) % 3];
+    sweep_x_tile<T, 1, FLUX>(a, blockIdx.x, 0);
+    if (!grid_barrier(&pc->bar, (unsigned int)((k - k0 + 1) * gridDim.x), spin_limit, &pc->err))
+      return;
+  }
+}
+
+// Slot 0 holds the initial maximum (*ws0), run time and count start at 0.
+__global__ void persist_init_kernel(PersistCtl* pc, const unsigned long long* ws0) {
+  pc->ws[0] = *ws0;
+  pc->ws[1] = 0ull;
+  pc->ws[2] = 0ull;
+  pc->t[0] = 0ull;
+  pc->t[1] = 0ull;
+  pc->iters[0] = 0;
+  pc->iters[1] = 0;
+  pc->dt = 0.0;
+  pc->bar = 0u;
+  pc->err = 0;
+}
+
+__global__ void persist_bar_reset_kernel(PersistCtl* pc) { pc->bar = 0u; }
+
 __global__ void init_control_kernel(StepCtl* ctl) {
   ctl->ws_bits[0] = 0ull;
   ctl->ws_bits[1] = 0ull;
@@ -294,6 +387,24 @@ FL_INST_SWEEPS2(float, kFluxRichtmyer)
                                                        unsigned long long*);
 FL_INST_MAXWS2(double)
 FL_INST_MAXWS2(float)
+
+#define FL_INST_PERSIST(T)                                                                   \
+  template __global__ void persist1d_kernel<T, kFluxHllc>(SweepArgs<T>, SweepArgs<T>,        \
+                                                          PersistCtl*, double, double, double, \
+                                                          long long, long long, long long,     \
+                                                          unsigned int);                       \
+  template __global__ void persist1d_kernel<T, kFluxForce>(SweepArgs<T>, SweepArgs<T>,       \
+                                                           PersistCtl*, double, double, double, \
+                                                           long long, long long, long long,     \
+                                                           unsigned int);                       \
+  template __global__ void persist1d_kernel<T, kFluxLaxFriedrichs>(                          \
+      SweepArgs<T>, SweepArgs<T>, PersistCtl*, double, double, double, long long, long long,  \
+      long long, unsigned int);                                                                \
+  template __global__ void persist1d_kernel<T, kFluxRichtmyer>(                              \
+      SweepArgs<T>, SweepArgs<T>, PersistCtl*, double, double, double, long long, long long,  \
+      long long, unsigned int);
+FL_INST_PERSIST(double)
+FL_INST_PERSIST(float)
 
 #define FL_SWEEPS(T, FLUX, FAST)                                                 \
   template __global__ void sweep_x_kernel<T, 1, FLUX, FAST>(SweepArgs<T>);       \

@@ -93,7 +93,8 @@ class SplitSolver:
         self.ctx.set_tuning(**kw)
 
     def set_graphs(self, enable: bool) -> None:
-        """simulate() replays a hipGraph of a pair of steps (default) or
+        """simulate() replays hipGraphs of steps (default; a 1D line of at
+        most 2^16 cells runs the whole loop in one cooperative launch) or
         launches every kernel."""
         N.check(N.lib.fl_ctx_set_graphs(self.ctx.handle, 1 if enable else 0))
 

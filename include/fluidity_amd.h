@@ -216,7 +216,9 @@ int fl_simulate_noalloc(const fl_tuning* tuning, fl_field* state, fl_field* work
 
 /* fl_simulate captures a pair of steps (control kernel + sweeps, both
  * parities) in a hipGraph on first use for a given grid, scheme, parameters
- * and tuning, and replays it (default); enable = 0 launches every kernel. */
+ * and tuning, and replays it (default); a 1D line of at most 2^16 cells
+ * (exact FP64 or FP32) instead runs the whole loop in one cooperative launch
+ * with a grid barrier per step.  enable = 0 launches every kernel. */
 int fl_ctx_set_graphs(fl_context* ctx, int enable);
 
 /* FP64 sums of each conserved component over the grid (synchronises

@@ -307,6 +307,41 @@ def test_simulate_graph_replay_equals_launches(E, nd, steps):
     assert np.array_equal(res[0][0].view(np.uint64), res[1][0].view(np.uint64))
 
 
+@pytest.mark.parametrize("case", [
+    # n, dtype, flux, bc, layout, max_iters, sim_time
+    (4096, torch.float64, 0, 0, 0, 301, float("inf")),
+    (700, torch.float64, 1, 1, 1, 40, float("inf")),
+    (1000, torch.float64, 0, 2, 0, 10 ** 6, 0.05),
+    (62, torch.float64, 0, 0, 1, 9, float("inf")),
+    (65536, torch.float64, 0, 0, 1, 12, float("inf")),
+    (65537, torch.float64, 0, 0, 1, 12, float("inf")),   # past the persistent loop's limit
+    (3000, torch.float32, 0, 0, 1, 50, float("inf")),
+    (2000, torch.float64, 0, 0, 0, 0, float("inf")),
+])
+def test_simulate_1d_persistent_equals_launches(E, case):
+    """fl_simulate on a 1D line of at most 2^16 cells runs the whole step loop
+    in one cooperative launch (a grid barrier per step, dt computed by every
+    workgroup from the same control words); with graphs off every step is
+    launched from the host.  Both give the same state bit for bit, the same
+    step count, run time, last dt and final maximum wavespeed: odd and even
+    step counts, a run stopped by sim_time, every boundary kind, FP64 and
+    FP32, AoS and SoA, zero steps."""
+    n, dtype, flux, bc, layout, steps, sim_time = case
+    U = cons_field("riemann", (n,), seed=1000)
+    res = []
+    for graphs in (True, False):
+        solver = E.SplitSolver(E.Scheme(flux=flux, bc=(bc, 0, 0)), layout=layout)
+        solver.set_graphs(graphs)
+        a = _dev(U, dtype=dtype, layout=layout)
+        r = solver.simulate(a, torch.empty_like(a), 0.9, 1.0 / n, sim_time=sim_time,
+                            max_iters=steps)
+        res.append((_host(a, layout=layout), r.iters, r.run_time, r.last_dt, r.max_wavespeed))
+    assert res[0][1:] == res[1][1:]
+    if sim_time == float("inf"):
+        assert res[0][1] == steps
+    assert _bitwise(res[0][0], res[1][0]), _report(res[0][0], res[1][0])
+
+
 def test_headline_workload_full_size_bitwise(E, oracle):
     """The 4096^2 FP64 HLLC four-quadrant workload itself (seed 1001), after
     the launch-shape autotuner: 12 steps of the device loop equal 12 steps of
