@@ -811,6 +811,77 @@ __global__ void gfm_star2_kernel(const double* __restrict__ phi0, const double* 
     star_cell<ND>(phi1, p0, u1, u0, e1, e0, g, dh, x, y, i, p1, star1, has1);
 }
 
+// Star states of material m among ms.nmat materials (one thread per cell):
+// at an inside-interfacial cell of m, every other material o in order whose
+// level set is outside-interfacial or on the boundary there gives a star
+// state, a later o overwriting an earlier one, as the reference's loop over
+// the other materials does (ghost_fluid/riemann_ghost_fluid.hpp:56-190).
+template <int ND>
+__global__ void gfm_star_n_kernel(MatSet ms, int m, Grid g, double dh,
+                                  double* __restrict__ star, uint8_t* __restrict__ has_star,
+                                  int* __restrict__ count_reset) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  if (count_reset && i == 0) *count_reset = 0;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  has_star[i] = 0;
+  const double* phi_m = ms.phi[m];
+  const double p = phi_m[i];
+  if (!(mask_value(p, dh) & 8)) return;
+  for (int o = 0; o < ms.nmat; ++o) {
+    if (o == m) continue;
+    star_cell<ND>(phi_m, ms.phi[o][i], ms.u[m], ms.u[o], ms.e[m], ms.e[o], g, dh, x, y, i, p,
+                  star, has_star);
+  }
+}
+
+// set_velocities for level set `target` among ms.nmat materials: V = u . n
+// with u the velocity of the first material whose phi <= 0 and n the normal
+// of level set `target` (velocity_kernel with n materials).
+template <int ND>
+__global__ void velocity_n_kernel(MatSet ms, int target, Grid g, double dh,
+                                  double* __restrict__ V) {
+  const int64_t i = gid();
+  if (i >= g.nx * g.ny) return;
+  const int64_t x = i % g.nx, y = i / g.nx;
+  double v = 0.0;
+  for (int m = 0; m < ms.nmat; ++m) {
+    const double p = ms.phi[m][i];
+    if ((p <= 0.0) || (p <= -0.0)) {
+      double nrm[2];
+      norm_at<ND>(ms.phi[target], g, x, y, dh, nrm);
+      double u[2 + ND];
+      load_state_i<2 + ND>(ms.u[m], i, u);
+      double sum = 0.0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) sum += (u[2 + d] / u[0]) * nrm[d];
+      v = sum;
+      break;
+    }
+  }
+  V[i] = v;
+}
+
+// project_impl for n > 2 level sets (levelset/cuda/levelset_projection.cuh:
+// 87-118): the average of the two smallest values at a cell, from the
+// running pair (max - 1, max), is subtracted from every level set.
+__global__ void project_n_kernel(PhiSet ps, int64_t count) {
+  const int64_t i = gid();
+  if (i >= count) return;
+  double smallest = DBL_MAX - 1.0, next_smallest = DBL_MAX;
+  for (int m = 0; m < ps.nmat; ++m) {
+    const double v = ps.phi[m][i];
+    if (v < smallest) {
+      next_smallest = smallest;
+      smallest = v;
+    } else if (v < next_smallest) {
+      next_smallest = v;
+    }
+  }
+  const double avg = 0.5 * (smallest + next_smallest);
+  for (int m = 0; m < ps.nmat; ++m) ps.phi[m][i] -= avg;
+}
+
 // 0 < p <= upper, as an int so that several can be or-ed without a short
 // circuit.
 __device__ __forceinline__ int in_band(double p, double upper) {
@@ -1013,6 +1084,9 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
   template __global__ void rk3_stage_kernel<ND>(int, const double*, const double*,            \
                                                 const double*, Grid, double, const double*,  \
                                                 double, double*);                            \
+  template __global__ void gfm_star_n_kernel<ND>(MatSet, int, Grid, double, double*,         \
+                                                 uint8_t*, int*);                             \
+  template __global__ void velocity_n_kernel<ND>(MatSet, int, Grid, double, double*);        \
   template __global__ void gfm_star_kernel<ND>(const double*, const double*, StateView,      \
                                                StateView, Eos, Eos, Grid, double, double*,   \
                                                uint8_t*, int*);                             \

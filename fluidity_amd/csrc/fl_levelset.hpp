@@ -59,7 +59,29 @@ struct Eos {
   int corr;   // corrected stiffened gas: pressure minus gamma p_inf
 };
 
+// Materials of an n-material step (n <= kMaxMat): level sets, states and
+// equations of state, passed by value to the kernels that read all of them.
+constexpr int kMaxMat = 8;
+struct MatSet {
+  int nmat;
+  int pad;
+  const double* phi[kMaxMat];
+  StateView u[kMaxMat];
+  Eos e[kMaxMat];
+};
+struct PhiSet {
+  int nmat;
+  int pad;
+  double* phi[kMaxMat];
+};
+
 __global__ void masks_kernel(const double* phi, Grid g, double dh, uint8_t* masks);
+template <int ND>
+__global__ void gfm_star_n_kernel(MatSet ms, int m, Grid g, double dh, double* star,
+                                  uint8_t* has_star, int* count_reset);
+template <int ND>
+__global__ void velocity_n_kernel(MatSet ms, int target, Grid g, double dh, double* V);
+__global__ void project_n_kernel(PhiSet ps, int64_t count);
 template <int ND>
 __global__ void velocity_kernel(const double* phi0, const double* phi1, StateView u0,
                                 StateView u1, int nmat, Grid g, double dh, int target, double* V);

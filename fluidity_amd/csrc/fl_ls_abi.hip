@@ -394,19 +394,21 @@ int launch_combine(const double* phi0, const fl_field* s0, const fl_field* s1,
   return FL_OK;
 }
 
-// Scratch layout of fl_mm_simulate.
+// Scratch layout of fl_mm_simulate for nmat materials: per material a star
+// plane, extrapolation scratch, speed, advection buffer, reinitialisation
+// scratch and star flags, and the RK3 stages of two level sets.
 struct MmWork {
-  double* star[2];
-  void* extrap[2];  // extrap_work_bytes each
-  double* V[2];
-  double* adv;      // 4 * cells (RK3 stages of both level sets)
-  double* tphi[2];
-  void* reinit[2];  // reinit_work_bytes each
-  uint8_t* has[2];
+  double* star[ls::kMaxMat];
+  void* extrap[ls::kMaxMat];  // extrap_work_bytes each
+  double* V[ls::kMaxMat];
+  double* adv;                // 4 * cells (RK3 stages of two level sets)
+  double* tphi[ls::kMaxMat];
+  void* reinit[ls::kMaxMat];  // reinit_work_bytes each
+  uint8_t* has[ls::kMaxMat];
   int64_t bytes;
 };
 
-MmWork mm_layout(void* base, const ls::Grid& g) {
+MmWork mm_layout(void* base, const ls::Grid& g, int nmat) {
   const int64_t cells = g.nx * g.ny;
   const int S = 2 + g.nd;
   char* p = (char*)base;
@@ -417,24 +419,190 @@ MmWork mm_layout(void* base, const ls::Grid& g) {
     return r;
   };
   MmWork w;
-  w.star[0] = (double*)take(cells * S * 8);
-  w.star[1] = (double*)take(cells * S * 8);
-  w.extrap[0] = take(extrap_work_bytes(g));
-  w.extrap[1] = take(extrap_work_bytes(g));
-  w.V[0] = (double*)take(cells * 8);
-  w.V[1] = (double*)take(cells * 8);
+  memset(&w, 0, sizeof(w));
+  for (int m = 0; m < nmat; ++m) w.star[m] = (double*)take(cells * S * 8);
+  for (int m = 0; m < nmat; ++m) w.extrap[m] = take(extrap_work_bytes(g));
+  for (int m = 0; m < nmat; ++m) w.V[m] = (double*)take(cells * 8);
   w.adv = (double*)take(4 * cells * 8);
-  w.tphi[0] = (double*)take(cells * 8);
-  w.tphi[1] = (double*)take(cells * 8);
-  w.reinit[0] = take(reinit_work_bytes(g));
-  w.reinit[1] = take(reinit_work_bytes(g));
-  w.has[0] = (uint8_t*)take(cells);
-  w.has[1] = (uint8_t*)take(cells);
+  for (int m = 0; m < nmat; ++m) w.tphi[m] = (double*)take(cells * 8);
+  for (int m = 0; m < nmat; ++m) w.reinit[m] = take(reinit_work_bytes(g));
+  for (int m = 0; m < nmat; ++m) w.has[m] = (uint8_t*)take(cells);
   w.bytes = off;
   return w;
 }
 
 bool aligned(const void* p) { return ((uintptr_t)p & 255) == 0; }
+
+// Materials' level sets (cur) and states as a MatSet for the n-material kernels.
+ls::MatSet mat_set(double* const* cur, const fl_material* mats, int nmat) {
+  ls::MatSet ms;
+  memset(&ms, 0, sizeof(ms));
+  ms.nmat = nmat;
+  for (int m = 0; m < nmat; ++m) {
+    ms.phi[m] = cur[m];
+    ms.u[m] = view_of(&mats[m].state);
+    ms.e[m] = eos_of(&mats[m].scheme);
+  }
+  return ms;
+}
+
+// fl_mm_simulate's loop for nmat > 2 materials (the two-material loop's
+// order, decisions L7-L9): star states of every material from the same
+// data, each material's extrapolation (two per launch), dt on the device
+// from the maximum over every grid, the split solves (two grids per launch
+// when their flux methods match), every level set's speed, advection (two
+// per launch), the n-level-set projection and reinitialisation (two per
+// solve).
+int mm_simulate_n(fl_context* c, fl_material* mats, int nmat, const fl_sim_params* p,
+                  const ls::Grid& g, int band, int sweeps, int every, const MmWork& w,
+                  fl_sim_result* result, hipStream_t st) {
+  const int64_t cells = g.nx * g.ny;
+  const int S = 2 + g.nd;
+  const int64_t esize = mats[0].state.dtype == FL_F64 ? 8 : 4;
+  const double dh = p->dh;
+  double t = 0.0, dt = 0.0, maxws = 0.0;
+  int64_t it = 0;
+  int rc;
+  double* cur[ls::kMaxMat];
+  double* nxt[ls::kMaxMat];
+  for (int m = 0; m < nmat; ++m) {
+    cur[m] = mats[m].phi;
+    nxt[m] = w.tphi[m];
+  }
+  auto settle = [&]() -> int {
+    for (int m = 0; m < nmat; ++m)
+      if (cur[m] != mats[m].phi)
+        FL_HIP(hipMemcpyAsync(mats[m].phi, cur[m], (size_t)(cells * 8),
+                              hipMemcpyDeviceToDevice, st));
+    return FL_OK;
+  };
+  const dim3 grid = grid_for(cells);
+  fl::StepCtl* ctl = c->ctl;
+  hipLaunchKernelGGL(fl::init_control_kernel, dim3(1), dim3(1), 0, st, ctl);
+  FL_HIP(hipGetLastError());
+  while (t < p->sim_time && it < p->max_iters) {
+    ls::ExtrapJob jobs[ls::kMaxMat];
+    for (int m = 0; m < nmat; ++m)
+      jobs[m] = extrap_job(cur[m], view_of(&mats[m].state), g, w.extrap[m]);
+    const ls::MatSet ms = mat_set(cur, mats, nmat);
+    for (int m = 0; m < nmat; ++m) {
+      if (g.nd == 1)
+        hipLaunchKernelGGL(ls::gfm_star_n_kernel<1>, grid, dim3(kBlock), 0, st, ms, m, g, dh,
+                           w.star[m], w.has[m], jobs[m].count);
+      else
+        hipLaunchKernelGGL(ls::gfm_star_n_kernel<2>, grid, dim3(kBlock), 0, st, ms, m, g, dh,
+                           w.star[m], w.has[m], jobs[m].count);
+    }
+    for (int m = 0; m < nmat; m += 2) {
+      if (m + 1 < nmat) {
+        const MmGhost gw = {{w.star[m], w.star[m + 1]}, {w.has[m], w.has[m + 1]}};
+        band_seed2(jobs + m, gw, g, dh, st);
+        extrap_sweeps(jobs[m], jobs[m + 1], 2, g, dh, sweeps, st);
+      } else {
+        band_seed(jobs[m], w.star[m], w.has[m], g, dh, st);
+        extrap_sweeps(jobs[m], jobs[m], 1, g, dh, sweeps, st);
+      }
+    }
+    FL_HIP(hipGetLastError());
+    for (int m = 0; m < nmat; ++m)
+      if ((rc = launch_max_ws(&mats[m].state, &mats[m].scheme, &ctl->ws_bits[0], st))) return rc;
+    hipLaunchKernelGGL(fl::mm_control_kernel, dim3(1), dim3(1), 0, st, ctl, p->cfl, dh);
+    FL_HIP(hipGetLastError());
+    FL_HIP(hipMemcpyAsync(c->host_ctl, ctl, sizeof(fl::StepCtl), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipEventRecord(c->step_ev, st));
+    for (int m = 0; m < nmat; m += 2) {
+      if (m + 1 < nmat) {
+        const fl_field* const u[2] = {&mats[m].state, &mats[m + 1].state};
+        const fl_field* const v[2] = {&mats[m].work, &mats[m + 1].work};
+        const fl_scheme* const sc[2] = {&mats[m].scheme, &mats[m + 1].scheme};
+        if ((rc = launch_sweep2(c, u, v, 0, sc, 0.0, st, ctl))) return rc;
+        if (g.nd > 1 && (rc = launch_sweep2(c, v, u, 1, sc, 0.0, st, ctl))) return rc;
+      } else {
+        if ((rc = launch_sweep(c, &mats[m].state, &mats[m].work, 0, &mats[m].scheme, 0.0, ctl,
+                               nullptr, st)))
+          return rc;
+        if (g.nd > 1 && (rc = launch_sweep(c, &mats[m].work, &mats[m].state, 1, &mats[m].scheme,
+                                           0.0, ctl, nullptr, st)))
+          return rc;
+      }
+    }
+    FL_HIP(hipEventSynchronize(c->step_ev));
+    const fl::StepCtl hc = *c->host_ctl;
+    maxws = bits_to_double(hc.ws_bits[1]);
+    if (!hc.active) {
+      if (result) {
+        result->run_time = t;
+        result->iters = it;
+        result->last_dt = dt;
+        result->max_wavespeed = maxws;
+      }
+      if ((rc = settle())) return rc;
+      FL_HIP(hipStreamSynchronize(st));
+      return fail(FL_ERR_NUMERIC, "max wavespeed %g at step %lld (NaN or vacuum)", maxws,
+                  (long long)it);
+    }
+    if (g.nd == 1)
+      for (int m = 0; m < nmat; ++m)
+        FL_HIP(hipMemcpyAsync(mats[m].state.data, mats[m].work.data, (size_t)(cells * S * esize),
+                              hipMemcpyDeviceToDevice, st));
+    const ls::MatSet ms2 = mat_set(cur, mats, nmat);
+    for (int m = 0; m < nmat; ++m) {
+      if (g.nd == 1)
+        hipLaunchKernelGGL(ls::velocity_n_kernel<1>, grid, dim3(kBlock), 0, st, ms2, m, g, dh,
+                           w.V[m]);
+      else
+        hipLaunchKernelGGL(ls::velocity_n_kernel<2>, grid, dim3(kBlock), 0, st, ms2, m, g, dh,
+                           w.V[m]);
+    }
+    for (int m = 0; m < nmat; m += 2) {
+      if (m + 1 < nmat) {
+        double* const c2[2] = {cur[m], cur[m + 1]};
+        double* const v2[2] = {w.V[m], w.V[m + 1]};
+        double* const o2[2] = {nxt[m], nxt[m + 1]};
+        if ((rc = launch_advect2(c2, v2, g, dh, o2, w.adv, &ctl->dt, st))) return rc;
+      } else if ((rc = launch_advect(cur[m], w.V[m], g, 0.0, dh, nxt[m], w.adv, st, &ctl->dt))) {
+        return rc;
+      }
+    }
+    {
+      ls::PhiSet ps;
+      memset(&ps, 0, sizeof(ps));
+      ps.nmat = nmat;
+      for (int m = 0; m < nmat; ++m) ps.phi[m] = nxt[m];
+      hipLaunchKernelGGL(ls::project_n_kernel, grid, dim3(kBlock), 0, st, ps, cells);
+      FL_HIP(hipGetLastError());
+    }
+    const bool reinit = (it % every) == every - 1;
+    if (reinit) {  // two level sets per solve, written back into cur[m]
+      for (int m = 0; m < nmat; m += 2) {
+        ls::EikJobs ej;
+        const int nj = m + 1 < nmat ? 2 : 1;
+        ej.j[0] = eik_job(nxt[m], g, cur[m], nullptr, w.reinit[m], c->eik);
+        ej.j[1] = nj == 2 ? eik_job(nxt[m + 1], g, cur[m + 1], nullptr, w.reinit[m + 1], c->eik + 1)
+                          : ej.j[0];
+        if ((rc = run_reinit(c, ej, nj, g, dh, band, nullptr, st))) return rc;
+      }
+    } else {
+      for (int m = 0; m < nmat; ++m) {
+        double* tmp = cur[m];
+        cur[m] = nxt[m];
+        nxt[m] = tmp;
+      }
+    }
+    dt = hc.dt;
+    t = hc.t;
+    it = hc.iters;
+  }
+  if ((rc = settle())) return rc;
+  FL_HIP(hipStreamSynchronize(st));
+  if (result) {
+    result->run_time = t;
+    result->iters = it;
+    result->last_dt = dt;
+    result->max_wavespeed = maxws;
+  }
+  return FL_OK;
+}
 
 }  // namespace
 
@@ -564,7 +732,13 @@ int fl_mm_combine(const double* phi0, const fl_field* state0, const fl_field* st
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n) {
   ls::Grid g;
   if (check_shape(ndim, n, &g)) return -1;
-  return mm_layout(nullptr, g).bytes;
+  return mm_layout(nullptr, g, 2).bytes;
+}
+
+int64_t fl_mm_work_bytes_n(int nmat, int ndim, const int64_t* n) {
+  ls::Grid g;
+  if (nmat < 2 || nmat > ls::kMaxMat || check_shape(ndim, n, &g)) return -1;
+  return mm_layout(nullptr, g, nmat).bytes;
 }
 
 int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_params* p,
@@ -572,11 +746,12 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
                    fl_sim_result* result, void* stream) {
   if (!c) return fail(FL_ERR_ARGUMENT, "context is NULL");
   if (!mats || !p || !work) return fail(FL_ERR_ARGUMENT, "NULL pointer argument");
-  if (nmat != 2) return fail(FL_ERR_UNSUPPORTED, "nmat %d: the projection needs 2 materials", nmat);
+  if (nmat < 2 || nmat > ls::kMaxMat)
+    return fail(FL_ERR_UNSUPPORTED, "nmat %d: 2 to %d materials", nmat, ls::kMaxMat);
   ls::Grid g;
   int rc = check_shape(mats[0].state.ndim, mats[0].state.n, &g);
   if (rc) return rc;
-  for (int m = 0; m < 2; ++m) {
+  for (int m = 0; m < nmat; ++m) {
     if ((rc = check_state(&mats[m].state, g, "mats[m].state"))) return rc;
     if ((rc = check_state(&mats[m].work, g, "mats[m].work"))) return rc;
     if (mats[m].state.layout != mats[m].work.layout || mats[m].state.data == mats[m].work.data)
@@ -598,12 +773,13 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     return fail(FL_ERR_ARGUMENT, "band %d, extrap_sweeps %d or reinit_every %d out of range",
                 band, sweeps, every);
   if (!aligned(work)) return fail(FL_ERR_ARGUMENT, "work must be 256-byte aligned");
-  const MmWork w = mm_layout(work, g);
+  const MmWork w = mm_layout(work, g, nmat);
   if (work_bytes < w.bytes)
     return fail(FL_ERR_ARGUMENT, "work of %lld bytes, need %lld", (long long)work_bytes,
                 (long long)w.bytes);
   DeviceGuard dg(c->device);
   hipStream_t st = (hipStream_t)stream;
+  if (nmat > 2) return mm_simulate_n(c, mats, nmat, p, g, band, sweeps, every, w, result, st);
   const int64_t cells = g.nx * g.ny;
   const int S = 2 + g.nd;
   const int64_t esize = mats[0].state.dtype == FL_F64 ? 8 : 4;

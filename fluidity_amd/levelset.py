@@ -153,7 +153,7 @@ def combine_into_first(phi0, state0, state1, layout: int = N.FL_LAYOUT_AOS,
 
 
 class MultimaterialSimulator:
-    """Two-material Riemann ghost-fluid simulator on one device."""
+    """Riemann ghost-fluid simulator of 2 to 8 materials on one device."""
 
     def __init__(self, device: int = 0, layout: int = N.FL_LAYOUT_AOS):
         self.device = device
@@ -162,17 +162,20 @@ class MultimaterialSimulator:
         self.materials = []
         self._tuned = False
 
+    MAX_MATERIALS = 8
+
     def add_material(self, state: torch.Tensor, phi: torch.Tensor, scheme: Scheme):
-        if len(self.materials) == 2:
-            raise ValueError("two materials are supported")
+        if len(self.materials) == self.MAX_MATERIALS:
+            raise ValueError(f"at most {self.MAX_MATERIALS} materials are supported")
         self.materials.append((state, torch.empty_like(state), phi, scheme))
 
     def simulate_mm(self, cfl: float, dh: float, sim_time: float = float("inf"),
                     max_iters: int = 1, band: int = 6, extrap_sweeps: int = 8,
                     reinit_every: int = 1, stream=None) -> N.SimResult:
-        if len(self.materials) != 2:
-            raise ValueError("add two materials first")
-        mats = (N.Material * 2)()
+        nmat = len(self.materials)
+        if nmat < 2:
+            raise ValueError("add at least two materials first")
+        mats = (N.Material * nmat)()
         for m, (state, work, phi, scheme) in enumerate(self.materials):
             mats[m].state = grid_field(state, self.layout)
             mats[m].work = grid_field(work, self.layout)
@@ -183,11 +186,11 @@ class MultimaterialSimulator:
             autotune_context(self.ctx, state, work, scheme, self.layout, stream=stream)
             self._tuned = True
         nd, n = _shape(self.materials[0][2])
-        wbytes = N.lib.fl_mm_work_bytes(nd, n)
+        wbytes = N.lib.fl_mm_work_bytes_n(nmat, nd, n)
         work = _work(wbytes, self.materials[0][0].device)
         p = N.SimParams(cfl, dh, sim_time, max_iters)
         mp = N.MmParams(band, extrap_sweeps, reinit_every, 0)
         r = N.SimResult()
-        N.check(N.lib.fl_mm_simulate(self.ctx.handle, mats, 2, C.byref(p), C.byref(mp),
+        N.check(N.lib.fl_mm_simulate(self.ctx.handle, mats, nmat, C.byref(p), C.byref(mp),
                                      work.data_ptr(), wbytes, C.byref(r), _stream_handle(stream)))
         return r

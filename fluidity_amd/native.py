@@ -36,7 +36,7 @@ EXPORTED = [
     "fl_ls_masks", "fl_ls_velocity", "fl_ls_advect", "fl_ls_project",
     "fl_ls_reinit_work_bytes", "fl_ls_reinit", "fl_gfm_star", "fl_gfm_extrapolate_work_bytes",
     "fl_gfm_extrapolate",
-    "fl_mm_combine", "fl_mm_work_bytes", "fl_mm_simulate",
+    "fl_mm_combine", "fl_mm_work_bytes", "fl_mm_work_bytes_n", "fl_mm_simulate",
     "fl_batch_create", "fl_batch_destroy", "fl_batch_set_tuning", "fl_batch_get_tuning",
     "fl_batch_autotune", "fl_batch_reset", "fl_batch_step", "fl_batch_finish",
     "fl_batch_conservation_sums",
@@ -47,7 +47,8 @@ EXPORTED = [
 ]
 
 # entry points returning a size rather than a status
-_SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes", "fl_adj_checkpoint_bytes",
+_SIZE_RETURNING = ("fl_ls_reinit_work_bytes", "fl_mm_work_bytes", "fl_mm_work_bytes_n",
+                   "fl_adj_checkpoint_bytes",
                    "fl_gfm_extrapolate_work_bytes",
                    "fl_adj_work_bytes")
 
@@ -177,6 +178,7 @@ def _load():
     L.fl_gfm_extrapolate.argtypes = [vp, fp, vp, vp, C.c_double, C.c_int, vp, vp]
     L.fl_mm_combine.argtypes = [vp, fp, fp, fp, vp]
     L.fl_mm_work_bytes.argtypes = [C.c_int, i64p]
+    L.fl_mm_work_bytes_n.argtypes = [C.c_int, C.c_int, i64p]
     L.fl_mm_simulate.argtypes = [vp, C.POINTER(Material), C.c_int, C.POINTER(SimParams),
                                  C.POINTER(MmParams), vp, C.c_int64, C.POINTER(SimResult), vp]
     L.fl_batch_create.argtypes = [vp, C.POINTER(BatchProblem), C.c_int, C.POINTER(vp), vp]

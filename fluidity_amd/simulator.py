@@ -314,8 +314,9 @@ class MultimaterialSimulator(_Base):
         conservative(primitive state from `elements`: rho=, p=, v_x=, v_y=)
         (simulator/multimaterial_simulation_data.hpp:166-189).  Cells outside,
         which the reference leaves uninitialised, are given the same state."""
-        if len(self.materials) == 2:
-            raise ValueError("two materials are supported")
+        if len(self.materials) == L.MultimaterialSimulator.MAX_MATERIALS:
+            raise ValueError(f"at most {L.MultimaterialSimulator.MAX_MATERIALS} materials "
+                             "are supported")
         names = ELEMENT_NAMES[:2 + self.dimensions]
         for k in elements:
             if k not in names:

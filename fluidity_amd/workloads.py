@@ -406,6 +406,62 @@ def shock_bubble(nx: int = 2048, ny: int = 1024, seed: int = 1002,
                        stiff, mach, r, (cx, cy), xs, dh)
 
 
+@dataclass
+class ShockBubbles:
+    Us: list             # per material (ny, nx, 4) conservative AoS
+    phis: list           # per material (ny, nx) level sets, phi <= 0 inside
+    gammas: list
+    mach: float
+    dh: float
+
+
+def shock_two_bubbles(nx: int = 512, ny: int = 256, seed: int = 1007) -> ShockBubbles:
+    """Three materials: a planar shock (Mach uniform in [1.2, 3]) in gas
+    (gamma 1.4, rho 1, p 1 ahead of it) running in +x onto two bubbles on
+    [0, nx dh] x [0, 1], dh = 1/ny: a light one (gamma 1.67, density ratio
+    0.14) and a heavy one (gamma 1.249, density ratio 3.15), both in pressure
+    equilibrium with the gas, radii uniform in [0.08, 0.15], centres seeded
+    and apart.  Level sets: each bubble's exact signed distance, and for the
+    gas minus the smaller of the two, negative outside both bubbles.  Each
+    material's state is its own phase over the whole grid."""
+    rng = SplitMix64(seed)
+    ra = rng.uniform(0.08, 0.15)
+    rb = rng.uniform(0.08, 0.15)
+    ca = (rng.uniform(0.6, 0.9), rng.uniform(0.2, 0.45))
+    cb = (rng.uniform(1.1, 1.4), rng.uniform(0.55, 0.8))
+    mach = rng.uniform(1.2, 3.0)
+    g0 = 1.4
+    rho_a, p_a = 1.0, 1.0
+    a_a = math.sqrt(g0 * p_a / rho_a)
+    m2 = mach * mach
+    rho_s = rho_a * (g0 + 1.0) * m2 / ((g0 - 1.0) * m2 + 2.0)
+    p_s = p_a * (2.0 * g0 * m2 - (g0 - 1.0)) / (g0 + 1.0)
+    u_s = mach * a_a * (1.0 - rho_a / rho_s)
+    xs = min(ca[0] - ra, cb[0] - rb) - 0.05
+    dh = 1.0 / ny
+    x = (np.arange(nx) + 0.5) * dh
+    y = (np.arange(ny) + 0.5) * dh
+    X, Y = np.meshgrid(x, y)
+    W0 = np.empty((ny, nx, 4))
+    post = X < xs
+    W0[..., 0] = np.where(post, rho_s, rho_a)
+    W0[..., 1] = np.where(post, p_s, p_a)
+    W0[..., 2] = np.where(post, u_s, 0.0)
+    W0[..., 3] = 0.0
+    gammas = [g0, 1.67, 1.249]
+    Us = [prim_to_cons(W0, g0)]
+    for g, ratio in ((1.67, 0.14), (1.249, 3.15)):
+        W = np.zeros((ny, nx, 4))
+        W[..., 0] = ratio * rho_a
+        W[..., 1] = p_a
+        Us.append(prim_to_cons(W, g))
+    phia = np.hypot(X - ca[0], Y - ca[1]) - ra
+    phib = np.hypot(X - cb[0], Y - cb[1]) - rb
+    phis = [-np.minimum(phia, phib), phia, phib]
+    return ShockBubbles([np.ascontiguousarray(U) for U in Us],
+                        [np.ascontiguousarray(p) for p in phis], gammas, mach, dh)
+
+
 def aos_to_soa(U: np.ndarray) -> np.ndarray:
     """(.., S) AoS -> (S, ..) SoA planes."""
     return np.ascontiguousarray(np.moveaxis(U, -1, 0))

@@ -457,12 +457,17 @@ typedef struct fl_mm_params {
   int32_t reserved;
 } fl_mm_params;
 
-/* The two-material loop of MultimaterialSimulator::simulate_mm
- * (simulator/multimaterial_simulator.hpp:227-303, 367-483).  Per step: star
- * states of both materials, then extrapolation of each; dt = cfl dh / max
- * wavespeed over both materials' grids; each material's split solve; level
- * set m advected with u . n_m; projection; reinitialisation of both level
- * sets after every reinit_every-th step.  Each pass covers both materials in
+/* The multimaterial loop of MultimaterialSimulator::simulate_mm
+ * (simulator/multimaterial_simulator.hpp:227-303, 367-483) for nmat = 2..8
+ * materials.  Per step: star states of every material (material m's from
+ * the last other material whose level set is outside-interfacial or on the
+ * boundary at the cell, ghost_fluid/riemann_ghost_fluid.hpp:56-190), then
+ * extrapolation of each; dt = cfl dh / max wavespeed over every material's
+ * grid; each material's split solve; level set m advected with u . n_m;
+ * projection (two level sets: levelset_projection.cuh:54-66; more: the
+ * average of the two smallest values subtracted from each,
+ * levelset_projection.cuh:87-118); reinitialisation of every level set
+ * after every reinit_every-th step.  Each pass covers both materials in
  * one launch (the sweeps too, when both schemes use the same flux method).
  * Between reinitialisations the level sets alternate between phi and a work
  * buffer; they are back in phi on FL_OK and on the FL_ERR_NUMERIC of a NaN or
@@ -478,8 +483,12 @@ typedef struct fl_mm_params {
  * the previous one).  Both materials' states (and work grids) share one
  * dtype: FP64, or FP32 bulk storage (FP32 sweeps; dt, the wavespeed maximum,
  * the level sets, the ghost-fluid states and the eikonal solve in FP64).
- * work: fl_mm_work_bytes().  mp may be NULL (defaults). */
+ * work: fl_mm_work_bytes_n(nmat, ...) (fl_mm_work_bytes() for two).  mp may
+ * be NULL (defaults). */
 int64_t fl_mm_work_bytes(int ndim, const int64_t* n);
+/* fl_mm_simulate's workspace for nmat (2..8) materials (fl_mm_work_bytes is
+ * the nmat = 2 case); -1 for a bad argument. */
+int64_t fl_mm_work_bytes_n(int nmat, int ndim, const int64_t* n);
 int fl_mm_simulate(fl_context* ctx, fl_material* mats, int nmat, const fl_sim_params* params,
                    const fl_mm_params* mp, void* work, int64_t work_bytes,
                    fl_sim_result* result, void* stream);

@@ -93,6 +93,7 @@ void orc_ls_advect(const double* phi, const double* V, const int64_t* n, int nd,
 void orc_ls_weno(const double* phi, int64_t n, double dh, int64_t i, double* dminus,
                  double* dplus);
 void orc_ls_project2(double* phi0, double* phi1, int64_t count);
+void orc_ls_project_n(int nmat, double* const* phis, int64_t count);
 int  orc_ls_reinit(const double* phi, const int64_t* n, int nd, double dh, int band,
                    double* out, uint8_t* band_mask, int64_t* iterations);
 void orc_gfm_star(const double* phi_m, const double* phi_o, const double* U_m,
@@ -108,6 +109,14 @@ int  orc_mm_simulate(double* U0, double* U1, double* phi0, double* phi1, const i
                      double sim_time, int64_t max_iters, int band, int extrap_sweeps,
                      int reinit_every, int nthreads, double* run_time, int64_t* iters,
                      double* dt_hist);
+void orc_gfm_star_n(int m, int nmat, const double* const* phis, const double* const* Us,
+                    const orc_scheme* const* eos, const int64_t* n, int nd, double dh,
+                    double* star, uint8_t* has_star);
+int  orc_mm_simulate_n(int nmat, double* const* Us, double* const* phis, const int64_t* n,
+                       int nd, const orc_scheme* const* ss, double cfl, double dh,
+                       double sim_time, int64_t max_iters, int band, int extrap_sweeps,
+                       int reinit_every, int nthreads, double* run_time, int64_t* iters,
+                       double* dt_hist);
 
 /* ---- initial conditions and result files (fl_oracle_io.cpp) ---- */
 int  orc_write_ascii(const double* U, const int64_t* n, int nd, const orc_scheme* s,

@@ -14,6 +14,7 @@
  *  - L = V |grad phi| .................. scheme/evaluators/levelset_hamiltonian.hpp:63-98
  *  - SSP-RK3 ........................... scheme/updaters/runge_kutta_3.hpp:117-148
  *  - projection (two materials) ........ levelset/cuda/levelset_projection.cuh:54-66
+ *  - projection (n > 2 materials) ...... levelset/cuda/levelset_projection.cuh:87-118
  *  - eikonal sources, Godunov update ... scheme/cuda/fast_iterative_method.cuh:48-85,
  *                                        scheme/schemes/godunov_upwind_scheme.hpp:37-53
  *  - Riemann ghost fluid ............... ghost_fluid/riemann_ghost_fluid.hpp:50-195,
@@ -61,6 +62,14 @@
  *      advects every level set with u . n of the material present, which
  *      flips the sign of the speed across the interface for every level set
  *      but the present material's and holds the interface in place.
+ *  L9. n > 2 materials: material m's ghost-fluid star state at a cell comes
+ *      from the last other material (in material order) whose level set is
+ *      outside-interfacial or on the boundary there, as the reference's
+ *      loop over the other materials overwrites earlier ones
+ *      (riemann_ghost_fluid.hpp:56-190); the level sets are projected by the
+ *      reference's general rule (the average of the two smallest values is
+ *      subtracted from every level set); two materials keep the two-level-set
+ *      projection, as the reference specialises it.
  */
 #include "fl_oracle.h"
 
@@ -275,6 +284,26 @@ void orc_ls_project2(double* phi0, double* phi1, int64_t count) {
   }
 }
 
+/* levelset/cuda/levelset_projection.cuh:87-118 (n > 2 level sets): the
+ * average of the two smallest values at a cell is subtracted from every
+ * level set there.  The running pair starts at (max - 1, max) as there. */
+void orc_ls_project_n(int nmat, double* const* phis, int64_t count) {
+  for (int64_t i = 0; i < count; ++i) {
+    double smallest = DBL_MAX - 1.0, next_smallest = DBL_MAX;
+    for (int m = 0; m < nmat; ++m) {
+      const double v = phis[m][i];
+      if (v < smallest) {
+        next_smallest = smallest;
+        smallest = v;
+      } else if (v < next_smallest) {
+        next_smallest = v;
+      }
+    }
+    const double avg = 0.5 * (smallest + next_smallest);
+    for (int m = 0; m < nmat; ++m) phis[m][i] -= avg;
+  }
+}
+
 /* ---- eikonal reinitialisation (decision L4) --------------------------------- */
 
 /* GodunovUpwindScheme::solve_impl (scheme/schemes/godunov_upwind_scheme.hpp:37-53)
@@ -439,6 +468,60 @@ static void hllc_star_left(const double* ul, const double* ur, int nd, const orc
   out[1] = scale_factor * (ul[1] + ul[0] * (ws_star - vl) * (ws_star + pl / (ul[0] * state_factor)));
 }
 
+/* The star state of material m at cell (x, y) against material o (L5), or 0
+ * where the cell does not qualify: m inside-interfacial with no axis
+ * neighbour on its boundary, o outside-interfacial or on its boundary. */
+static int gfm_star_cell(const double* phi_m, const double* phi_o, const double* U_m,
+                         const double* U_o, const orc_scheme* eos_m, const orc_scheme* eos_o,
+                         const lsgrid_t* gp, double dh, int64_t x, int64_t y, double* out) {
+  const lsgrid_t g = *gp;
+  const int nd = g.nd;
+  const int64_t i = y * g.nx + x;
+  const double p = phi_m[i];
+  const uint8_t mk = orc_ls_mask_value(p, dh);
+  if (!(mk & 8)) return 0;                       /* inside_interfacial_cell */
+  int nb_on_boundary = 0;                         /* neighbour_not_on_boundary */
+  for (int d = 0; d < nd; ++d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const double pm = phi_at(phi_m, &g, x - dx, y - dy), pp = phi_at(phi_m, &g, x + dx, y + dy);
+    if (pm == 0.0 || pp == 0.0) nb_on_boundary = 1;
+  }
+  if (nb_on_boundary) return 0;
+  const uint8_t mo = orc_ls_mask_value(phi_o[i], dh);
+  if (!((mo & 16) || (mo & 4))) return 0;         /* out_ifc_or_boundary */
+  double nrm[2] = {0.0, 0.0};
+  ls_norm(phi_m, &g, x, y, dh, nrm);
+  const double abs_phi = fabs(p);
+  const double dh_three_two = 1.5 * dh;
+  double w[2] = {0.0, 0.0}, wl[8], wr[8];
+  for (int d = 0; d < nd; ++d) w[d] = nrm[d] * (abs_phi - dh_three_two) / dh;
+  interp_state(U_m, &g, x, y, w, wl);
+  for (int d = 0; d < nd; ++d) w[d] = nrm[d] * (abs_phi + dh_three_two) / dh;
+  interp_state(U_o, &g, x, y, w, wr);
+  /* normal frame through the primitive state */
+  double Wl[8], Wr[8], vl[2];
+  orc_cons_to_prim(wl, nd, 0, eos_m, Wl);
+  orc_cons_to_prim(wr, nd, 0, eos_o, Wr);
+  double vl_n = 0.0, vr_n = 0.0;
+  for (int d = 0; d < nd; ++d) {
+    vl[d] = Wl[2 + d];
+    vl_n += Wl[2 + d] * nrm[d];
+    vr_n += Wr[2 + d] * nrm[d];
+  }
+  Wl[2] = vl_n;
+  Wr[2] = vr_n;
+  for (int d = 1; d < nd; ++d) { Wl[2 + d] = 0.0; Wr[2 + d] = 0.0; }
+  orc_prim_to_cons(Wl, nd, 0, eos_m, wl);
+  orc_prim_to_cons(Wr, nd, 0, eos_o, wr);
+  double st[8], Ws[8];
+  hllc_star_left(wl, wr, nd, eos_m, eos_o, st);
+  orc_cons_to_prim(st, nd, 0, eos_m, Ws);
+  const double vs_n = Ws[2];
+  for (int d = 0; d < nd; ++d) Ws[2 + d] = vl[d] + (vs_n - vl_n) * nrm[d];
+  orc_prim_to_cons(Ws, nd, 0, eos_m, out);
+  return 1;
+}
+
 void orc_gfm_star(const double* phi_m, const double* phi_o, const double* U_m,
                   const double* U_o, const orc_scheme* eos_m, const orc_scheme* eos_o,
                   const int64_t* n, int nd, double dh, double* star, uint8_t* has_star) {
@@ -447,50 +530,30 @@ void orc_gfm_star(const double* phi_m, const double* phi_o, const double* U_m,
 #pragma omp parallel for
   for (int64_t i = 0; i < g.nx * g.ny; ++i) {
     const int64_t x = i % g.nx, y = i / g.nx;
+    has_star[i] = (uint8_t)gfm_star_cell(phi_m, phi_o, U_m, U_o, eos_m, eos_o, &g, dh, x, y,
+                                         star + i * S);
+  }
+}
+
+/* Material m's star states among nmat materials (L9): every other material
+ * in order, a later qualifying one overwriting an earlier one. */
+void orc_gfm_star_n(int m, int nmat, const double* const* phis, const double* const* Us,
+                    const orc_scheme* const* eos, const int64_t* n, int nd, double dh,
+                    double* star, uint8_t* has_star) {
+  lsgrid_t g = {n[0], nd > 1 ? n[1] : 1, nd};
+  const int S = 2 + nd;
+#pragma omp parallel for
+  for (int64_t i = 0; i < g.nx * g.ny; ++i) {
+    const int64_t x = i % g.nx, y = i / g.nx;
     has_star[i] = 0;
-    const double p = phi_m[i];
-    const uint8_t mk = orc_ls_mask_value(p, dh);
-    if (!(mk & 8)) continue;                       /* inside_interfacial_cell */
-    int nb_on_boundary = 0;                         /* neighbour_not_on_boundary */
-    for (int d = 0; d < nd; ++d) {
-      const int64_t dx = (d == 0), dy = (d == 1);
-      const double pm = phi_at(phi_m, &g, x - dx, y - dy), pp = phi_at(phi_m, &g, x + dx, y + dy);
-      if (pm == 0.0 || pp == 0.0) nb_on_boundary = 1;
+    double st[8];
+    for (int o = 0; o < nmat; ++o) {
+      if (o == m) continue;
+      if (gfm_star_cell(phis[m], phis[o], Us[m], Us[o], eos[m], eos[o], &g, dh, x, y, st)) {
+        memcpy(star + i * S, st, sizeof(double) * S);
+        has_star[i] = 1;
+      }
     }
-    if (nb_on_boundary) continue;
-    const uint8_t mo = orc_ls_mask_value(phi_o[i], dh);
-    if (!((mo & 16) || (mo & 4))) continue;         /* out_ifc_or_boundary */
-    double nrm[2] = {0.0, 0.0};
-    ls_norm(phi_m, &g, x, y, dh, nrm);
-    const double abs_phi = fabs(p);
-    const double dh_three_two = 1.5 * dh;
-    double w[2], wl[8], wr[8];
-    for (int d = 0; d < nd; ++d) w[d] = nrm[d] * (abs_phi - dh_three_two) / dh;
-    interp_state(U_m, &g, x, y, w, wl);
-    for (int d = 0; d < nd; ++d) w[d] = nrm[d] * (abs_phi + dh_three_two) / dh;
-    interp_state(U_o, &g, x, y, w, wr);
-    /* normal frame through the primitive state */
-    double Wl[8], Wr[8], vl[2];
-    orc_cons_to_prim(wl, nd, 0, eos_m, Wl);
-    orc_cons_to_prim(wr, nd, 0, eos_o, Wr);
-    double vl_n = 0.0, vr_n = 0.0;
-    for (int d = 0; d < nd; ++d) {
-      vl[d] = Wl[2 + d];
-      vl_n += Wl[2 + d] * nrm[d];
-      vr_n += Wr[2 + d] * nrm[d];
-    }
-    Wl[2] = vl_n;
-    Wr[2] = vr_n;
-    for (int d = 1; d < nd; ++d) { Wl[2 + d] = 0.0; Wr[2 + d] = 0.0; }
-    orc_prim_to_cons(Wl, nd, 0, eos_m, wl);
-    orc_prim_to_cons(Wr, nd, 0, eos_o, wr);
-    double st[8], Ws[8];
-    hllc_star_left(wl, wr, nd, eos_m, eos_o, st);
-    orc_cons_to_prim(st, nd, 0, eos_m, Ws);
-    const double vs_n = Ws[2];
-    for (int d = 0; d < nd; ++d) Ws[2 + d] = vl[d] + (vs_n - vl_n) * nrm[d];
-    orc_prim_to_cons(Ws, nd, 0, eos_m, star + i * S);
-    has_star[i] = 1;
   }
 }
 
@@ -629,5 +692,76 @@ done:
   if (run_time) *run_time = t;
   if (iters) *iters = it;
   free(W); free(star0); free(star1); free(has0); free(has1); free(V); free(V1); free(tmp);
+  return rc;
+}
+
+/* MultimaterialSimulator::simulate_mm for nmat >= 2 materials (L7-L9): the
+ * two-material loop above, generalised; nmat == 2 computes exactly what
+ * orc_mm_simulate does. */
+int orc_mm_simulate_n(int nmat, double* const* Us, double* const* phis, const int64_t* n,
+                      int nd, const orc_scheme* const* ss, double cfl, double dh,
+                      double sim_time, int64_t max_iters, int band, int extrap_sweeps,
+                      int reinit_every, int nthreads, double* run_time, int64_t* iters,
+                      double* dt_hist) {
+  const int S = 2 + nd;
+  const int64_t total = n[0] * (nd > 1 ? n[1] : 1);
+  if (nmat < 2 || nmat > 8) return -2;
+  double* W = (double*)malloc(sizeof(double) * total * S);
+  double* stars = (double*)malloc(sizeof(double) * total * S * nmat);
+  uint8_t* has = (uint8_t*)malloc(total * nmat);
+  double* V = (double*)malloc(sizeof(double) * total * nmat);
+  double* tmp = (double*)malloc(sizeof(double) * total);
+  int rc = 0;
+  double t = 0.0;
+  int64_t it = 0;
+  if (!W || !stars || !has || !V || !tmp) { rc = -3; goto done; }
+  while (t < sim_time && it < max_iters) {
+    const double* cphis[8];
+    const double* cUs[8];
+    for (int m = 0; m < nmat; ++m) { cphis[m] = phis[m]; cUs[m] = Us[m]; }
+    for (int m = 0; m < nmat; ++m)
+      orc_gfm_star_n(m, nmat, cphis, cUs, ss, n, nd, dh, stars + m * total * S, has + m * total);
+    for (int m = 0; m < nmat; ++m)
+      orc_gfm_extrapolate(phis[m], Us[m], stars + m * total * S, has + m * total, n, nd, dh,
+                          extrap_sweeps);
+    double maxws = 0.0;
+    for (int m = 0; m < nmat; ++m) {
+      const double ws = orc_max_wavespeed_grid(Us[m], n, nd, ss[m], nthreads);
+      if (ws != ws) { rc = -4; goto done; }
+      maxws = ws > maxws ? ws : maxws;
+    }
+    const double dt = cfl * dh / maxws;
+    const double dtdh = dt / dh;
+    for (int m = 0; m < nmat; ++m) {
+      rc = orc_sweep(Us[m], W, n, nd, ss[m], dtdh, 0, nthreads);
+      if (rc) goto done;
+      if (nd > 1) rc = orc_sweep(W, Us[m], n, nd, ss[m], dtdh, 1, nthreads);
+      else memcpy(Us[m], W, sizeof(double) * total * S);
+      if (rc) goto done;
+    }
+    for (int m = 0; m < nmat; ++m) cUs[m] = Us[m];
+    for (int m = 0; m < nmat; ++m)
+      orc_ls_velocity(nmat, cphis, cUs, n, nd, dh, m, V + m * total);
+    for (int m = 0; m < nmat; ++m) {
+      orc_ls_advect(phis[m], V + m * total, n, nd, dt, dh, tmp);
+      memcpy(phis[m], tmp, sizeof(double) * total);
+    }
+    if (nmat == 2) orc_ls_project2(phis[0], phis[1], total);
+    else orc_ls_project_n(nmat, phis, total);
+    if (it % reinit_every == reinit_every - 1) {
+      for (int m = 0; m < nmat; ++m) {
+        rc = orc_ls_reinit(phis[m], n, nd, dh, band, tmp, NULL, NULL);
+        if (rc) goto done;
+        memcpy(phis[m], tmp, sizeof(double) * total);
+      }
+    }
+    if (dt_hist) dt_hist[it] = dt;
+    t += dt;
+    it += 1;
+  }
+done:
+  if (run_time) *run_time = t;
+  if (iters) *iters = it;
+  free(W); free(stars); free(has); free(V); free(tmp);
   return rc;
 }

@@ -121,6 +121,16 @@ def lib():
                                       C.c_double, C.c_double, C.c_int64, C.c_int, C.c_int,
                                       C.c_int, C.c_int, dp, i64p, dp]
         L.orc_mm_simulate.restype = C.c_int
+        L.orc_mm_simulate_n.argtypes = [C.c_int, C.POINTER(dp), C.POINTER(dp), i64p, C.c_int,
+                                        C.POINTER(sp), C.c_double, C.c_double, C.c_double,
+                                        C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int64), dp]
+        L.orc_mm_simulate_n.restype = C.c_int
+        L.orc_ls_project_n.argtypes = [C.c_int, C.POINTER(dp), C.c_int64]
+        L.orc_ls_project_n.restype = None
+        L.orc_gfm_star_n.argtypes = [C.c_int, C.c_int, C.POINTER(dp), C.POINTER(dp),
+                                     C.POINTER(sp), i64p, C.c_int, C.c_double, dp, u8p]
+        L.orc_gfm_star_n.restype = None
         L.orc_write_ascii.argtypes = [dp, i64p, C.c_int, sp, dp, C.c_double, C.c_double,
                                       C.c_char_p]
         L.orc_write_ascii.restype = C.c_int
@@ -401,6 +411,57 @@ def mm_simulate(U0, U1, phi0, phi1, s0, s1, cfl, dh, sim_time=float("inf"), max_
     if rc:
         raise RuntimeError(f"orc_mm_simulate failed: {rc}")
     return U0, U1, phi0, phi1, rt.value, it.value, dts[: it.value]
+
+
+def mm_simulate_n(Us, phis, schemes, cfl, dh, sim_time=float("inf"), max_iters=1, band=6,
+                  extrap_sweeps=8, reinit_every=1, nthreads=0):
+    """n-material ghost-fluid run (decision L9); returns (Us, phis, run_time,
+    iters, dts) with copies of the inputs advanced."""
+    Us = [np.array(U, dtype=np.float64, order="C") for U in Us]
+    phis = [np.array(p, dtype=np.float64, order="C") for p in phis]
+    nmat = len(Us)
+    nd, n = _ls_dims(phis[0])
+    if max_iters > 10_000_000:
+        raise ValueError("mm_simulate_n needs a bounded max_iters")
+    dts = np.zeros(int(max_iters))
+    rt = C.c_double(0.0)
+    it = C.c_int64(0)
+    dp = C.POINTER(C.c_double)
+    ua = (dp * nmat)(*[_dp(U) for U in Us])
+    pa = (dp * nmat)(*[_dp(p) for p in phis])
+    sa = (C.POINTER(Scheme) * nmat)(*[C.pointer(s) for s in schemes])
+    rc = lib().orc_mm_simulate_n(nmat, ua, pa, _n_ptr(n), nd, sa, cfl, dh, sim_time,
+                                 int(max_iters), band, extrap_sweeps, reinit_every, nthreads,
+                                 C.byref(rt), C.byref(it), _dp(dts))
+    if rc:
+        raise RuntimeError(f"orc_mm_simulate_n failed: {rc}")
+    return Us, phis, rt.value, it.value, dts[: it.value]
+
+
+def ls_project_n(phis):
+    """The n-level-set projection, on copies."""
+    phis = [np.array(p, dtype=np.float64, order="C") for p in phis]
+    dp = C.POINTER(C.c_double)
+    pa = (dp * len(phis))(*[_dp(p) for p in phis])
+    lib().orc_ls_project_n(len(phis), pa, phis[0].size)
+    return phis
+
+
+def gfm_star_n(m, phis, Us, schemes, dh):
+    """Material m's star states among n materials: (star, has_star)."""
+    phis = [np.ascontiguousarray(p, dtype=np.float64) for p in phis]
+    Us = [np.ascontiguousarray(U, dtype=np.float64) for U in Us]
+    nmat = len(phis)
+    nd, n = _ls_dims(phis[0])
+    star = np.zeros(Us[m].shape)
+    has = np.zeros(phis[0].shape, dtype=np.uint8)
+    dp = C.POINTER(C.c_double)
+    pa = (dp * nmat)(*[_dp(p) for p in phis])
+    ua = (dp * nmat)(*[_dp(U) for U in Us])
+    sa = (C.POINTER(Scheme) * nmat)(*[C.pointer(s) for s in schemes])
+    lib().orc_gfm_star_n(m, nmat, pa, ua, sa, _n_ptr(n), nd, dh, _dp(star),
+                         has.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return star, has
 
 
 # ---- initial conditions and result files (fl_oracle_io.cpp) ---------------

@@ -419,3 +419,27 @@ def test_mm_simulate_fp32_bulk(L, E, nx, ny, steps):
     assert np.isfinite(got[1]).all() and np.isfinite(got[2]).all()
     assert abs(got[0].run_time - ref[0].run_time) <= 1e-5 * ref[0].run_time
     assert worst <= 1e-5, worst
+
+
+@pytest.mark.parametrize("layout,every", [(0, 1), (1, 5)])
+def test_mm_simulate_three_materials_bitwise(L, E, oracle, layout, every):
+    """Three materials (a shock onto two bubbles of different gases, seed
+    1007): the n-material loop (star states from the last qualifying other
+    material, n-level-set projection) bit-identical to the oracle's
+    n-material loop, states, level sets, run time and step count."""
+    sbs = W.shock_two_bubbles(192, 96, seed=1007)
+    ss = [oracle.scheme(gamma=g) for g in sbs.gammas]
+    steps = 10
+    Us, phis, rt, it, dts = oracle.mm_simulate_n(sbs.Us, sbs.phis, ss, 0.5, sbs.dh,
+                                                 max_iters=steps, reinit_every=every)
+    sim = L.MultimaterialSimulator(layout=layout)
+    dev_U = [_dev(U, layout) for U in sbs.Us]
+    dev_p = [_dev(p) for p in sbs.phis]
+    for U, p, g in zip(dev_U, dev_p, sbs.gammas):
+        sim.add_material(U, p, E.Scheme(gamma=g))
+    r = sim.simulate_mm(0.5, sbs.dh, max_iters=steps, reinit_every=every)
+    assert r.iters == it == steps
+    assert r.run_time == rt
+    for m in range(3):
+        _same(_host(dev_U[m], layout), Us[m])
+        _same(_host(dev_p[m]), phis[m])

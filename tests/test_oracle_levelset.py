@@ -401,3 +401,111 @@ def test_mm_interface_moves_with_flow(oracle):
     xz = x[i] - p0[i] * dh / (p0[i + 1] - p0[i])
     assert abs(xz - (0.4 + uvel * t)) < 0.05 * dh
     np.testing.assert_allclose(U0[20:100], U[20:100], rtol=1e-12)
+
+
+# ---- n > 2 materials (decision L9) ------------------------------------------
+
+def test_project_n_no_overlap_no_void(oracle):
+    """The n-level-set projection (levelset_projection.cuh:87-118): after
+    subtracting the average of the two smallest values, the smallest is <= 0
+    and every other >= 0 at each cell (no overlap, no void), the order of the
+    level sets at a cell is kept, their differences are kept to rounding, and
+    cells already projected (two smallest opposite, 0 their average) are
+    unchanged bit for bit."""
+    rng = np.random.default_rng(5)
+    phis = [rng.normal(size=4000) for _ in range(4)]
+    out = oracle.ls_project_n(phis)
+    P = np.stack(phis)
+    Q = np.stack(out)
+    order = np.argsort(P, axis=0, kind="stable")
+    srt = np.take_along_axis(Q, order, axis=0)
+    assert (srt[0] <= 0).all() and (srt[1:] >= 0).all()
+    assert np.array_equal(np.argsort(Q, axis=0, kind="stable"), order)
+    np.testing.assert_allclose(Q - Q[0], P - P[0], atol=1e-14)
+    # projected data: the two smallest are -a and +a, so their average is 0
+    a = np.abs(rng.normal(size=500)) + 0.1
+    proj = [-a, a, a + 1.0, a + 2.5]
+    again = oracle.ls_project_n(proj)
+    for p, q in zip(proj, again):
+        assert np.array_equal(p, q)
+
+
+def test_project_n_hand_values(oracle):
+    """Hand-worked cells: (3, -1, 2) -> the two smallest are -1 and 2, their
+    average 0.5, so (2.5, -1.5, 1.5); (-4, -2, 7, 0) -> average -3, so
+    (-1, 1, 10, 3); a tie (1, 1, 5) -> average 1, so (0, 0, 4)."""
+    cells = [([3.0, -1.0, 2.0], [2.5, -1.5, 1.5]),
+             ([-4.0, -2.0, 7.0, 0.0], [-1.0, 1.0, 10.0, 3.0]),
+             ([1.0, 1.0, 5.0], [0.0, 0.0, 4.0])]
+    for vals, want in cells:
+        out = oracle.ls_project_n([np.array([v]) for v in vals])
+        assert [float(q[0]) for q in out] == want
+
+
+def test_gfm_star_n_two_materials_is_pairwise(oracle):
+    """With two materials the n-material star states equal the pairwise
+    ones bit for bit (decision L9 reduces to L5)."""
+    sb = W.shock_bubble(128, 64, seed=1002)
+    s0, s1 = oracle.scheme(gamma=sb.gamma0), oracle.scheme(gamma=sb.gamma1)
+    for m, (pm, po, Um, Uo, sm, so) in enumerate([(sb.phi0, sb.phi1, sb.U0, sb.U1, s0, s1),
+                                                  (sb.phi1, sb.phi0, sb.U1, sb.U0, s1, s0)]):
+        ref, has = oracle.gfm_star(pm, po, Um, Uo, sm, so, sb.dh)
+        got, has_n = oracle.gfm_star_n(m, [sb.phi0, sb.phi1], [sb.U0, sb.U1], [s0, s1], sb.dh)
+        assert np.array_equal(has, has_n)
+        assert np.array_equal(ref[has == 1].view(np.uint64), got[has_n == 1].view(np.uint64))
+
+
+def test_gfm_star_n_last_other_material_wins(oracle):
+    """Three materials where two others qualify at a cell: the star state is
+    the pairwise one against the later material (the reference's loop over
+    the other materials overwrites)."""
+    sbs = W.shock_two_bubbles(96, 48, seed=1007)
+    ss = [oracle.scheme(gamma=g) for g in sbs.gammas]
+    dh = sbs.dh
+    # material 1's level set, with both others outside-interfacial wherever
+    # material 1 is inside-interfacial
+    phi1 = sbs.phis[1]
+    phis = [np.abs(phi1) * 0 + 0.5 * dh, phi1, np.full_like(phi1, 0.5 * dh)]
+    star, has = oracle.gfm_star_n(1, phis, sbs.Us, ss, dh)
+    ref2, has2 = oracle.gfm_star(phis[1], phis[2], sbs.Us[1], sbs.Us[2], ss[1], ss[2], dh)
+    ref0, has0 = oracle.gfm_star(phis[1], phis[0], sbs.Us[1], sbs.Us[0], ss[1], ss[0], dh)
+    assert has.any()
+    assert np.array_equal(has, has2)
+    assert np.array_equal(star[has == 1].view(np.uint64), ref2[has2 == 1].view(np.uint64))
+    assert not np.array_equal(star[has == 1].view(np.uint64), ref0[has0 == 1].view(np.uint64))
+
+
+def test_mm_simulate_n_two_materials_equals_pairwise(oracle):
+    """The n-material loop with two materials computes exactly the
+    two-material loop (same states, level sets, run time, dts)."""
+    sb = W.shock_bubble(96, 48, seed=1002)
+    s0, s1 = oracle.scheme(gamma=sb.gamma0), oracle.scheme(gamma=sb.gamma1)
+    R = oracle.mm_simulate(sb.U0, sb.U1, sb.phi0, sb.phi1, s0, s1, 0.5, sb.dh, max_iters=6,
+                           reinit_every=3)
+    Us, phis, rt, it, dts = oracle.mm_simulate_n([sb.U0, sb.U1], [sb.phi0, sb.phi1], [s0, s1],
+                                                 0.5, sb.dh, max_iters=6, reinit_every=3)
+    assert it == R[5] == 6 and rt == R[4]
+    assert np.array_equal(dts, R[6])
+    for a, b in zip(Us + phis, R[:4]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_mm_simulate_n_three_materials(oracle):
+    """Three materials (shock onto two bubbles of different gases): the loop
+    runs, states stay finite and positive in density where each material is
+    present, the level sets stay projected (exactly one non-positive at each
+    cell away from exact ties), and each bubble keeps its area to 5%."""
+    sbs = W.shock_two_bubbles(128, 64, seed=1007)
+    ss = [oracle.scheme(gamma=g) for g in sbs.gammas]
+    Us, phis, rt, it, dts = oracle.mm_simulate_n(sbs.Us, sbs.phis, ss, 0.5, sbs.dh, max_iters=12,
+                                                 reinit_every=5)
+    assert it == 12 and rt > 0
+    for U in Us:
+        assert np.isfinite(U).all()
+    P = np.stack(phis)
+    inside = (P <= 0).sum(axis=0)
+    assert (inside <= 1).all()
+    assert (inside == 1).mean() > 0.99
+    for m in (1, 2):
+        a0 = (sbs.phis[m] <= 0).sum()
+        assert abs((phis[m] <= 0).sum() - a0) / a0 < 0.05
