@@ -9,6 +9,7 @@ hipcc cross-compiles for gfx950 without a GPU.  Flags that matter:
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -38,15 +39,26 @@ def build(verbose: bool = False, out: str | None = None, defines=()) -> str:
     `defines` are extra -D macros (e.g. FL_SWEEP_WAVES=4) for tuning builds."""
     lib = out or LIB
     tag = f"{os.getpid()}"
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", f".{tag}.o"))
-        cmd = [HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c",
-               os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        cmds.append([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c",
+                     os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    # the sources compile independently: at most `jobs` hipcc processes at once
+    jobs = max(1, int(os.environ.get("MAX_JOBS") or min(8, os.cpu_count() or 1)))
+    try:
+        with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+            for f in [pool.submit(subprocess.check_call, cmd) for cmd in cmds]:
+                f.result()
+    except BaseException:
+        for o in objs:
+            if os.path.exists(o):
+                os.remove(o)
+        raise
     tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -162,28 +162,55 @@ int fit_rows(int rows, int64_t bands, int64_t ny, int block) {
   return rows;
 }
 
+// Kernels with van Leer in conservative form fixed at compile time exist for
+// HLLC and FORCE.
+template <int FLUX>
+constexpr bool has_vlc() { return FLUX == fl::kFluxHllc || FLUX == fl::kFluxForce; }
+inline bool is_vlc(int limiter, int form) {
+  return FL_VLC_KERNELS && limiter == fl::kLimVanLeer && form == fl::kFormCons;
+}
+
+template <typename T, int FLUX, bool FAST, bool VLC>
+void launch_flux_v(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
+                   hipStream_t stream) {
+  if (ndim == 1)
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX, FAST, VLC>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a);
+  else if (dim == 0)
+    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST, VLC>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a);
+  else
+    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST, VLC>), grid, dim3(B), 0, stream, a);
+}
+
 // One sweep kernel of flux FLUX: the 1D sweep, or the x or y sweep of a 2D grid.
 template <typename T, int FLUX, bool FAST>
 void launch_flux(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a,
                  hipStream_t stream) {
-  if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 1, FLUX, FAST>), grid, dim3(B),
-                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a);
-  else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST>), grid, dim3(B),
-                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a);
-  else
-    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST>), grid, dim3(B), 0, stream, a);
+  if constexpr (has_vlc<FLUX>()) {
+    if (is_vlc(a.limiter, a.form)) {
+      launch_flux_v<T, FLUX, FAST, true>(ndim, dim, grid, B, a, stream);
+      return;
+    }
+  }
+  launch_flux_v<T, FLUX, FAST, false>(ndim, dim, grid, B, a, stream);
 }
 
 // The fast FP64 numerics were requested (FP32 sweeps use them always).
 inline bool fast_numerics(const fl_scheme* s) { return s->numerics == FL_NUMERICS_FAST; }
 
 // Workgroups of B lanes of one sweep kernel that a CU holds at once.
+template <typename T, int FLUX, bool FAST, bool VLC>
+const void* sweep_kernel_2d(int dim) {
+  return dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX, FAST, VLC>)
+                  : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX, FAST, VLC>);
+}
+
 template <typename T, int FLUX, bool FAST>
-int sweep_blocks_per_cu(int dim, int B) {
-  const void* k = dim == 0 ? reinterpret_cast<const void*>(&fl::sweep_x_kernel<T, 2, FLUX, FAST>)
-                           : reinterpret_cast<const void*>(&fl::sweep_y_kernel<T, FLUX, FAST>);
+int sweep_blocks_per_cu(int dim, int B, bool vlc) {
+  const void* k = sweep_kernel_2d<T, FLUX, FAST, false>(dim);
+  if constexpr (has_vlc<FLUX>())
+    if (vlc) k = sweep_kernel_2d<T, FLUX, FAST, true>(dim);
   int n = 0;
   const size_t lds = dim == 0 ? fl::x_tile_lds_bytes(B, 4, sizeof(T)) : 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, lds) != hipSuccess) return 0;
@@ -191,21 +218,22 @@ int sweep_blocks_per_cu(int dim, int B) {
 }
 
 template <typename T, bool FAST>
-int sweep_blocks_per_cu_f(int dim, int flux, int B) {
+int sweep_blocks_per_cu_f(int dim, int flux, int B, bool vlc) {
   switch (flux) {
-    case FL_FLUX_HLLC: return sweep_blocks_per_cu<T, fl::kFluxHllc, FAST>(dim, B);
-    case FL_FLUX_FORCE: return sweep_blocks_per_cu<T, fl::kFluxForce, FAST>(dim, B);
+    case FL_FLUX_HLLC: return sweep_blocks_per_cu<T, fl::kFluxHllc, FAST>(dim, B, vlc);
+    case FL_FLUX_FORCE: return sweep_blocks_per_cu<T, fl::kFluxForce, FAST>(dim, B, vlc);
     case FL_FLUX_LAX_FRIEDRICHS:
-      return sweep_blocks_per_cu<T, fl::kFluxLaxFriedrichs, FAST>(dim, B);
-    default: return sweep_blocks_per_cu<T, fl::kFluxRichtmyer, FAST>(dim, B);
+      return sweep_blocks_per_cu<T, fl::kFluxLaxFriedrichs, FAST>(dim, B, vlc);
+    default: return sweep_blocks_per_cu<T, fl::kFluxRichtmyer, FAST>(dim, B, vlc);
   }
 }
 
 template <typename T>
 int sweep_blocks_per_cu_t(int dim, const fl_scheme* s, int B) {
+  const bool vlc = is_vlc(s->limiter, s->form);
   if (std::is_same<T, double>::value && fast_numerics(s))
-    return sweep_blocks_per_cu_f<T, true>(dim, s->flux, B);
-  return sweep_blocks_per_cu_f<T, false>(dim, s->flux, B);
+    return sweep_blocks_per_cu_f<T, true>(dim, s->flux, B, vlc);
+  return sweep_blocks_per_cu_f<T, false>(dim, s->flux, B, vlc);
 }
 
 // Tile heights to time for one 2D sweep: the powers of two in `base`, and,

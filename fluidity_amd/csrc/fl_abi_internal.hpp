@@ -11,6 +11,13 @@
 #include "fl_kernels.hpp"
 #include "fl_levelset.hpp"
 
+// The sweep kernels with van Leer in conservative form fixed at compile time
+// run where a scheme asks for them (tuning builds: FL_VLC_KERNELS=0 always
+// launches the generic kernels).
+#ifndef FL_VLC_KERNELS
+#define FL_VLC_KERNELS 1
+#endif
+
 // What a captured fl_simulate step pair depends on (compared bytewise).
 struct SimGraphKey {
   void* a;

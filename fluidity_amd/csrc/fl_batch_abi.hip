@@ -37,6 +37,7 @@ struct fl_batch {
   hipGraph_t graph;
   hipGraphExec_t exec;
   bool graph_ok;
+  bool vlc;                  // every problem: van Leer, conservative form
 };
 
 namespace {
@@ -126,18 +127,24 @@ std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
   return v;
 }
 
-template <typename T, int FLUX, int PHASE>
-const void* phase_fn() {
-  return reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, PHASE>);
+template <typename T, int FLUX, bool VLC>
+const void* phase_fn(int phase) {
+  return phase ? reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 1, VLC>)
+               : reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 0, VLC>);
 }
 
-const void* phase_kernel_ptr(int dtype, int flux, int phase) {
-  if (dtype == FL_F64) {
-    if (flux == FL_FLUX_HLLC) return phase ? phase_fn<double, fl::kFluxHllc, 1>() : phase_fn<double, fl::kFluxHllc, 0>();
-    return phase ? phase_fn<double, fl::kFluxForce, 1>() : phase_fn<double, fl::kFluxForce, 0>();
-  }
-  if (flux == FL_FLUX_HLLC) return phase ? phase_fn<float, fl::kFluxHllc, 1>() : phase_fn<float, fl::kFluxHllc, 0>();
-  return phase ? phase_fn<float, fl::kFluxForce, 1>() : phase_fn<float, fl::kFluxForce, 0>();
+template <typename T, bool VLC>
+const void* phase_fn_f(int flux, int phase) {
+  return flux == FL_FLUX_HLLC ? phase_fn<T, fl::kFluxHllc, VLC>(phase)
+                              : phase_fn<T, fl::kFluxForce, VLC>(phase);
+}
+
+const void* phase_kernel_ptr(const fl_batch* b, int phase) {
+  if (b->dtype == FL_F64)
+    return b->vlc ? phase_fn_f<double, true>(b->flux, phase)
+                  : phase_fn_f<double, false>(b->flux, phase);
+  return b->vlc ? phase_fn_f<float, true>(b->flux, phase)
+                : phase_fn_f<float, false>(b->flux, phase);
 }
 
 // Uploads the problem table and tile lists of tuning t; sizes the persistent
@@ -168,7 +175,7 @@ int apply_tuning(fl_batch* b, const fl_tuning& t) {
     const int B = ph ? t.y_block : t.x_block;
     int per_cu = 0;
     FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, phase_kernel_ptr(b->dtype, b->flux, ph), B,
+        &per_cu, phase_kernel_ptr(b, ph), B,
         ph == 0 ? fl::x_tile_lds_bytes(B, 4, b->dtype == FL_F64 ? 8 : 4) : 0));
     if (per_cu < 1) per_cu = 1;
     b->grid[ph] = per_cu * prop.multiProcessorCount;
@@ -178,19 +185,28 @@ int apply_tuning(fl_batch* b, const fl_tuning& t) {
   return FL_OK;
 }
 
-template <typename T, int FLUX>
-void launch_phases_t(const fl_batch* b, hipStream_t st, int timing, double timing_dtdh,
+template <typename T, int FLUX, bool VLC>
+void launch_phases_v(const fl_batch* b, hipStream_t st, int timing, double timing_dtdh,
                      int only_phase) {
   const bt::Prob<T>* probs = (const bt::Prob<T>*)b->probs;
   if (only_phase != 1 && b->ntiles[0] > 0)
-    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 0>), dim3(b->grid[0]),
+    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 0, VLC>), dim3(b->grid[0]),
                        dim3(b->tuning.x_block), fl::x_tile_lds_bytes(b->tuning.x_block, 4, sizeof(T)),
                        st, probs, (const bt::Tile*)b->tiles[0],
                        b->ntiles[0], b->cnt, timing, (T)timing_dtdh);
   if (only_phase != 0 && b->ntiles[1] > 0)
-    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1>), dim3(b->grid[1]),
+    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1, VLC>), dim3(b->grid[1]),
                        dim3(b->tuning.y_block), 0, st, probs, (const bt::Tile*)b->tiles[1],
                        b->ntiles[1], b->cnt, timing, (T)timing_dtdh);
+}
+
+template <typename T, int FLUX>
+void launch_phases_t(const fl_batch* b, hipStream_t st, int timing, double timing_dtdh,
+                     int only_phase) {
+  if (b->vlc)
+    launch_phases_v<T, FLUX, true>(b, st, timing, timing_dtdh, only_phase);
+  else
+    launch_phases_v<T, FLUX, false>(b, st, timing, timing_dtdh, only_phase);
 }
 
 // only_phase: -1 both phases, else just that one.
@@ -295,6 +311,10 @@ int fl_batch_create(fl_context* c, const fl_batch_problem* problems, int nprob, 
   b->nprob = nprob;
   b->dtype = dtype;
   b->flux = flux;
+  b->vlc = FL_VLC_KERNELS != 0;
+  for (int p = 0; p < nprob; ++p)
+    b->vlc = b->vlc && problems[p].scheme.limiter == FL_LIMITER_VAN_LEER &&
+             problems[p].scheme.form == FL_FORM_CONSERVATIVE;
   b->problems.assign(problems, problems + nprob);
   b->probs = nullptr;
   b->ctls = b->host_ctls = nullptr;

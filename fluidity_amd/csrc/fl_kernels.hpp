@@ -75,8 +75,8 @@ constexpr int kMaxWsUnroll = 4;
 
 // Kernels defined in fl_sweep.hip (explicitly instantiated there).
 namespace fl {
-template <typename T, int ND, int FLUX, bool FAST> __global__ void sweep_x_kernel(SweepArgs<T> a);
-template <typename T, int FLUX, bool FAST> __global__ void sweep_y_kernel(SweepArgs<T> a);
+template <typename T, int ND, int FLUX, bool FAST, bool VLC> __global__ void sweep_x_kernel(SweepArgs<T> a);
+template <typename T, int FLUX, bool FAST, bool VLC> __global__ void sweep_y_kernel(SweepArgs<T> a);
 template <typename T, int ND, int FLUX>
 __global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int FLUX> __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);

@@ -28,20 +28,22 @@ __device__ __forceinline__ void tile_of_block(int64_t& bx, int64_t& by) {
 #endif
 }
 
-template <typename T, int ND, int FLUX, bool FAST>
+// VLC: van Leer in conservative form, fixed at compile time (the launcher
+// picks these kernels when the scheme says so).
+template <typename T, int ND, int FLUX, bool FAST, bool VLC>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWavesOf<T, FAST>::value)))
 sweep_x_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_x_tile<T, ND, FLUX, FAST>(a, bx, by);
+  sweep_x_tile<T, ND, FLUX, FAST, VLC>(a, bx, by);
 }
 
-template <typename T, int FLUX, bool FAST>
+template <typename T, int FLUX, bool FAST, bool VLC>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWavesOf<T, FAST>::value)))
 sweep_y_kernel(SweepArgs<T> a) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_y_tile<T, FLUX, FAST>(a, bx, by);
+  sweep_y_tile<T, FLUX, FAST, VLC>(a, bx, by);
 }
 
 // The same sweeps over two grids of one shape in one launch (the two
@@ -406,10 +408,11 @@ FL_INST_MAXWS2(float)
 FL_INST_PERSIST(double)
 FL_INST_PERSIST(float)
 
-#define FL_SWEEPS(T, FLUX, FAST)                                                 \
-  template __global__ void sweep_x_kernel<T, 1, FLUX, FAST>(SweepArgs<T>);       \
-  template __global__ void sweep_x_kernel<T, 2, FLUX, FAST>(SweepArgs<T>);       \
-  template __global__ void sweep_y_kernel<T, FLUX, FAST>(SweepArgs<T>);
+#define FL_SWEEPS_V(T, FLUX, FAST, VLC)                                          \
+  template __global__ void sweep_x_kernel<T, 1, FLUX, FAST, VLC>(SweepArgs<T>);  \
+  template __global__ void sweep_x_kernel<T, 2, FLUX, FAST, VLC>(SweepArgs<T>);  \
+  template __global__ void sweep_y_kernel<T, FLUX, FAST, VLC>(SweepArgs<T>);
+#define FL_SWEEPS(T, FLUX, FAST) FL_SWEEPS_V(T, FLUX, FAST, false)
 
 #define FL_INST_SWEEPS(T)                                                      \
   FL_SWEEPS(T, kFluxHllc, false)                                                          \
@@ -434,5 +437,12 @@ FL_SWEEPS(double, kFluxHllc, true)
 FL_SWEEPS(double, kFluxForce, true)
 FL_SWEEPS(double, kFluxLaxFriedrichs, true)
 FL_SWEEPS(double, kFluxRichtmyer, true)
+// van Leer, conservative form: HLLC and FORCE
+FL_SWEEPS_V(double, kFluxHllc, false, true)
+FL_SWEEPS_V(double, kFluxForce, false, true)
+FL_SWEEPS_V(double, kFluxHllc, true, true)
+FL_SWEEPS_V(double, kFluxForce, true, true)
+FL_SWEEPS_V(float, kFluxHllc, false, true)
+FL_SWEEPS_V(float, kFluxForce, false, true)
 
 }  // namespace fl

@@ -339,10 +339,14 @@ __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
 // is rows_per_block consecutive segments of B-2 cells, starting at segment
 // bx rows_per_block, marched like rows.  Every thread of the workgroup must
 // call it with the same arguments.
-template <typename T, int ND, int FLUX, bool FAST = false>
+template <typename T, int ND, int FLUX, bool FAST = false, bool VLC = false>
 __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int S = 2 + ND;
   constexpr bool F = UseFast<T, FAST>::value;
+  // VLC: the scheme is known to limit with van Leer in conservative form, so
+  // the per-component limiter dispatch folds away at compile time.
+  const int limiter = VLC ? int(kLimVanLeer) : a.limiter;
+  const int form = VLC ? int(kFormCons) : a.form;
   // LDS sized for this launch's block width (dynamic shared memory of
   // x_tile_lds_bytes): states, pressures, x velocities and flattening
   // factors of the B + 6 staged columns, then the left face states and
@@ -506,7 +510,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       chi = smin(chi, smin(ky1, ky_nb));
     }
     T R[S], L[S];
-    evolve_sel<F, T, ND, 0>(um, U0, up, chi, half_dtdh, a.limiter, a.form, a.mat, R, L);
+    evolve_sel<F, T, ND, 0>(um, U0, up, chi, half_dtdh, limiter, form, a.mat, R, L);
 #pragma unroll
     for (int k = 0; k < S; ++k) sL[k * B + t] = L[k];
     __syncthreads();
@@ -547,10 +551,12 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
 
 // One y-sweep tile: columns [bx B, (bx+1) B) of rows [by rows_per_block,
 // (by+1) rows_per_block), B = blockDim.x.  Workgroup-uniform arguments.
-template <typename T, int FLUX, bool FAST = false>
+template <typename T, int FLUX, bool FAST = false, bool VLC = false>
 __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, int64_t by) {
   constexpr int ND = 2, S = 4;
   constexpr bool F = UseFast<T, FAST>::value;
+  const int limiter = VLC ? int(kLimVanLeer) : a.limiter;  // as in the x sweep
+  const int form = VLC ? int(kFormCons) : a.form;
   constexpr int W = kMaxBlock + 2 * kHalo;
   __shared__ T sP[W];
   __shared__ T sV[W];
@@ -668,7 +674,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
       chi = smin(chi, smin(ky1, ky_nb));
       T R[S], L[S];
-      evolve_sel<F, T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, a.limiter, a.form, a.mat,
+      evolve_sel<F, T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, limiter, form, a.mat,
                              R, L);
       if (r >= y0 + 3) {
         // face between rows jr-1 and jr

@@ -77,6 +77,30 @@ def test_batch_f64_bitwise(B, oracle, graph, layout):
     assert failed == sum(not np.isfinite(r[0]).all() for r in ref)
 
 
+def test_batch_mixed_limiters_bitwise(B, oracle):
+    """Problems with Superbee or primitive-form limiting beside van Leer
+    conservative ones: the batch then runs the kernels that read the limiter
+    and form per problem; every problem bit-identical to the oracle."""
+    from fluidity_amd.euler import Scheme
+    specs = _specs()
+    steps = 11
+    probs = B.problems_from_workload(specs, torch.float64, 1, max_iters=steps)
+    choices = [(0, 0), (1, 0), (0, 1), (1, 1)]
+    for i, p in enumerate(probs):
+        lim, form = choices[i % 4]
+        p["scheme"] = Scheme(flux=p["scheme"].flux, gamma=specs[i].gamma, limiter=lim, form=form)
+    bt = B.Batch(probs)
+    bt.step(steps)
+    res = bt.finish(allow_failed=True)
+    for i, sp in enumerate(specs):
+        lim, form = choices[i % 4]
+        s = oracle.scheme(limiter=lim, form=form, gamma=sp.gamma, bc=(0, 0))
+        U, rt, it, _ = oracle.simulate(sp.U, s, 0.9, sp.dh, max_iters=steps)
+        assert res[i].iters == it, i
+        assert _same_float(res[i].run_time, rt), i
+        assert _bits(_host(bt.state(i), sp.nd), U), f"problem {i} limiter {lim} form {form}"
+
+
 def test_batch_stops_per_problem(B, oracle):
     """Each problem stops at its own sim_time; later steps leave it alone."""
     specs = _specs()
