@@ -180,7 +180,8 @@ void launch_flux_v(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& 
     hipLaunchKernelGGL((fl::sweep_x_kernel<T, 2, FLUX, FAST, VLC>), grid, dim3(B),
                        fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a);
   else
-    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST, VLC>), grid, dim3(B), 0, stream, a);
+    hipLaunchKernelGGL((fl::sweep_y_kernel<T, FLUX, FAST, VLC>), grid, dim3(B),
+                       fl::y_tile_lds_bytes(B, sizeof(T)), stream, a);
 }
 
 // One sweep kernel of flux FLUX: the 1D sweep, or the x or y sweep of a 2D grid.
@@ -212,7 +213,8 @@ int sweep_blocks_per_cu(int dim, int B, bool vlc) {
   if constexpr (has_vlc<FLUX>())
     if (vlc) k = sweep_kernel_2d<T, FLUX, FAST, true>(dim);
   int n = 0;
-  const size_t lds = dim == 0 ? fl::x_tile_lds_bytes(B, 4, sizeof(T)) : 0;
+  const size_t lds =
+      dim == 0 ? fl::x_tile_lds_bytes(B, 4, sizeof(T)) : fl::y_tile_lds_bytes(B, sizeof(T));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, B, lds) != hipSuccess) return 0;
   return n;
 }
@@ -314,17 +316,30 @@ int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out,
 // with scheme s[m].  The two grids share shape, layout and dtype; the flux
 // methods must be equal (the launch instantiates one), else each grid is
 // swept on its own.
+template <typename T, int FLUX, bool VLC>
+void launch_flux2_v(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
+                    const fl::SweepArgs<T>& a1, hipStream_t stream) {
+  if (ndim == 1)
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX, VLC>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a0, a1);
+  else if (dim == 0)
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX, VLC>), grid, dim3(B),
+                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a0, a1);
+  else
+    hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX, VLC>), grid, dim3(B),
+                       fl::y_tile_lds_bytes(B, sizeof(T)), stream, a0, a1);
+}
+
 template <typename T, int FLUX>
 void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
                   const fl::SweepArgs<T>& a1, hipStream_t stream) {
-  if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX>), grid, dim3(B),
-                       fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a0, a1);
-  else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX>), grid, dim3(B),
-                       fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a0, a1);
-  else
-    hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX>), grid, dim3(B), 0, stream, a0, a1);
+  if constexpr (has_vlc<FLUX>()) {
+    if (is_vlc(a0.limiter, a0.form) && is_vlc(a1.limiter, a1.form)) {
+      launch_flux2_v<T, FLUX, true>(ndim, dim, grid, B, a0, a1, stream);
+      return;
+    }
+  }
+  launch_flux2_v<T, FLUX, false>(ndim, dim, grid, B, a0, a1, stream);
 }
 
 template <typename T>

@@ -176,7 +176,8 @@ int apply_tuning(fl_batch* b, const fl_tuning& t) {
     int per_cu = 0;
     FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, phase_kernel_ptr(b, ph), B,
-        ph == 0 ? fl::x_tile_lds_bytes(B, 4, b->dtype == FL_F64 ? 8 : 4) : 0));
+        ph == 0 ? fl::x_tile_lds_bytes(B, 4, b->dtype == FL_F64 ? 8 : 4)
+                : fl::y_tile_lds_bytes(B, b->dtype == FL_F64 ? 8 : 4)));
     if (per_cu < 1) per_cu = 1;
     b->grid[ph] = per_cu * prop.multiProcessorCount;
   }
@@ -196,7 +197,8 @@ void launch_phases_v(const fl_batch* b, hipStream_t st, int timing, double timin
                        b->ntiles[0], b->cnt, timing, (T)timing_dtdh);
   if (only_phase != 0 && b->ntiles[1] > 0)
     hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1, VLC>), dim3(b->grid[1]),
-                       dim3(b->tuning.y_block), 0, st, probs, (const bt::Tile*)b->tiles[1],
+                       dim3(b->tuning.y_block), fl::y_tile_lds_bytes(b->tuning.y_block, sizeof(T)),
+                       st, probs, (const bt::Tile*)b->tiles[1],
                        b->ntiles[1], b->cnt, timing, (T)timing_dtdh);
 }
 

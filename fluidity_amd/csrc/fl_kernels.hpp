@@ -66,6 +66,13 @@ __host__ __device__ constexpr size_t x_tile_lds_bytes(int B, int S, int tsize) {
   return (size_t)((S + 3) * (B + 2 * kHalo) + 2 * S * B) * (size_t)tsize;
 }
 
+// Dynamic LDS of a y-sweep tile of B lanes (2D states, 4 components of tsize
+// bytes): 3 rows of B + 2 kHalo columns, a ring of kYRing rows of 4 x B.
+constexpr int kYRing = 5;
+__host__ __device__ constexpr size_t y_tile_lds_bytes(int B, int tsize) {
+  return (size_t)(3 * (B + 2 * kHalo) + kYRing * 4 * B) * (size_t)tsize;
+}
+
 // Standalone wavespeed max: at most kMaxWsBlocks workgroups of kMaxBlock
 // threads, each thread loading kMaxWsUnroll cells per iteration.
 constexpr int kMaxWsBlocks = 512;
@@ -77,9 +84,10 @@ constexpr int kMaxWsUnroll = 4;
 namespace fl {
 template <typename T, int ND, int FLUX, bool FAST, bool VLC> __global__ void sweep_x_kernel(SweepArgs<T> a);
 template <typename T, int FLUX, bool FAST, bool VLC> __global__ void sweep_y_kernel(SweepArgs<T> a);
-template <typename T, int ND, int FLUX>
+template <typename T, int ND, int FLUX, bool VLC>
 __global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
-template <typename T, int FLUX> __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
+template <typename T, int FLUX, bool VLC>
+__global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int ND>
 __global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es0, int64_t cs0,
                                       int64_t es1, int64_t cs1, int64_t ncells,

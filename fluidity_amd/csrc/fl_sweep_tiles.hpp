@@ -557,17 +557,27 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
   constexpr bool F = UseFast<T, FAST>::value;
   const int limiter = VLC ? int(kLimVanLeer) : a.limiter;  // as in the x sweep
   const int form = VLC ? int(kFormCons) : a.form;
-  constexpr int W = kMaxBlock + 2 * kHalo;
-  __shared__ T sP[W];
-  __shared__ T sV[W];
-  __shared__ T sK[W];
+  const int B = blockDim.x;
+  const int W = B + 2 * kHalo;
+  // LDS sized for this launch's block width (dynamic shared memory of
+  // y_tile_lds_bytes): pressures, x velocities and flattening factors of the
+  // B + 6 columns of the newest row, then a ring of the last 5 rows of this
+  // workgroup's states, per wave [slot][component][lane of 64] (so that the
+  // component offsets are immediates).  A lane reads back only its
+  // own column of the ring, so the ring needs no barriers; it holds the rows
+  // the face states of row r - 3 need while rows r - 1 and r finish their
+  // flattening factors, which would otherwise stay in registers.
+  extern __shared__ __align__(16) unsigned char fl_y_lds[];
+  T* const sP = reinterpret_cast<T*>(fl_y_lds);
+  T* const sV = sP + W;
+  T* const sK = sV + W;
+  T* const ring = sK + W;                        // [B / 64][kYRing][S][64]
 
   T dtdh;
   if (!sweep_dtdh(a, dtdh)) return;
   const T half_dtdh = T(0.5) * dtdh;
   const T dhdt = T(1) / dtdh;
 
-  const int B = blockDim.x;
   const int t = threadIdx.x;
   const int s = t + kHalo;
   const int64_t x0 = bx * B;
@@ -583,9 +593,8 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
   const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - kHalo + hslot;
 
-  // Windows along y: U rows r-4..r, p rows r-4..r, u_y rows r-3..r,
-  // K_y rows r-4..r-2, x flattening chi_x rows r-3..r.
-  T Uw[5][S];
+  // Windows along y: p rows r-4..r, u_y rows r-3..r, K_y rows r-4..r-2,
+  // x flattening chi_x rows r-3..r; the states of rows r-4..r in the ring.
   T pw[5], vyw[4], cxw[4];
   T ky0 = T(1), ky1 = T(1), ky2 = T(1);  // named scalars (see the x sweep)
 #pragma unroll
@@ -611,6 +620,9 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
     issue_row<T, S>(a, min(rbeg + i, rlast), lc, Qs[i]);
     issue_row<T, S>(a, min(rbeg + i, rlast), lh, Hs[i]);
   }
+  // ring slot of row r: (r - rbeg) mod kYRing, advanced per row
+  int slot = 0;
+  T* const ring_t = ring + (t >> 6) * (kYRing * S * 64) + (t & 63);
   // Step i of each group of PF rows owns buffer i (see the x sweep).
   for (int64_t r0 = rbeg; r0 <= rlast; r0 += PF) {
 #pragma unroll
@@ -620,22 +632,21 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
     Pending<T, S>& Q = Qs[step];
     Pending<T, S>& H = Hs[step];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int k = 0; k < S; ++k) Uw[i][k] = Uw[i + 1][k];
-      pw[i] = pw[i + 1];
-    }
+    for (int i = 0; i < 4; ++i) pw[i] = pw[i + 1];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       vyw[i] = vyw[i + 1];
       cxw[i] = cxw[i + 1];
     }
-    take<T, ND>(Q, Uw[4]);
+    T un[S];
+    take<T, ND>(Q, un);
     issue_row<T, S>(a, min(r + PF, rlast), lc, Q);
     T uh[S];
     take<T, ND>(H, uh);
     issue_row<T, S>(a, min(r + PF, rlast), lh, H);
-    const Prim<T, ND> w = prim_sel<F, T, ND>(Uw[4], a.mat);
+#pragma unroll
+    for (int k = 0; k < S; ++k) ring_t[(slot * S + k) * 64] = un[k];
+    const Prim<T, ND> w = prim_sel<F, T, ND>(un, a.mat);
     pw[4] = w.p;
     vyw[3] = w.vel[1];
 
@@ -667,15 +678,24 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
       ky2 = chi_sel<F>(pw[0], pw[1], pw[3], pw[4], vyw[0], vyw[2], a.mat);
     }
 
-    // ---- face states of row jr = r-3
+    // ---- face states of row jr = r-3 (rows r-4..r-2 from the ring)
     if (r >= y0 + 2) {
+      const int s4 = slot == kYRing - 1 ? 0 : slot + 1;  // row r-4
+      const int s3 = s4 == kYRing - 1 ? 0 : s4 + 1;      // row r-3
+      const int s2 = s3 == kYRing - 1 ? 0 : s3 + 1;      // row r-2
+      T um[S], u0[S], up[S];
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        um[k] = ring_t[(s4 * S + k) * 64];
+        u0[k] = ring_t[(s3 * S + k) * 64];
+        up[k] = ring_t[(s2 * S + k) * 64];
+      }
       const int sy = signum(pw[2] - pw[0]);
       T chi = smin(Limits<T>::max(), cxw[0]);
       const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
       chi = smin(chi, smin(ky1, ky_nb));
       T R[S], L[S];
-      evolve_sel<F, T, ND, 1>(Uw[0], Uw[1], Uw[2], chi, half_dtdh, limiter, form, a.mat,
-                             R, L);
+      evolve_sel<F, T, ND, 1>(um, u0, up, chi, half_dtdh, limiter, form, a.mat, R, L);
       if (r >= y0 + 3) {
         // face between rows jr-1 and jr
         T G[S];
@@ -684,7 +704,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
         if (jo >= y0 && c < a.nx) {
           T o[S];
 #pragma unroll
-          for (int k = 0; k < S; ++k) o[k] = Uw[0][k] + (Gprev[k] - G[k]) * dtdh;
+          for (int k = 0; k < S; ++k) o[k] = um[k] + (Gprev[k] - G[k]) * dtdh;
           store_row<T, S>(a, jo, out_off, o);
           if (a.ws_out) {
             const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
@@ -697,6 +717,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
 #pragma unroll
       for (int k = 0; k < S; ++k) Rprev[k] = R[k];
     }
+    slot = slot == kYRing - 1 ? 0 : slot + 1;
   }
   }
   if (a.ws_out) block_atomic_max(wsmax, a.ws_out);
