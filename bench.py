@@ -7,6 +7,11 @@ A step is one full time step of the simulator loop
 the previous step's last sweep, computed on the device), the x sweep and the
 y sweep.  Inputs are resident in HBM before the timed region starts.
 
+The headline runs the FP64 step in the fast numerics (FP64 arithmetic
+throughout, held to 1e-12 relative L-inf of the reference's results by the
+GPU tests); the bit-exact FP64 numerics run beside it as the variant
+hllc_f64_exact (--numerics exact makes it the headline).
+
 Prints ONE JSON line.  Only the cpu_baseline leg loads the CPU oracle.
 """
 from __future__ import annotations
@@ -25,27 +30,30 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E bandwidth
 FP64_PEAK_TFLOPS = 78.6
 
-# Launch shapes (x lanes, x rows, y lanes, y rows) of the headline step.  The
-# kernel trace and the HBM counters in profiles/sweeps_4096_f64_hllc.md were
-# taken with these shapes; the autotuner's candidates run within a few percent
-# of each other on this grid (--autotune-headline times them instead); 27-row
-# x tiles make 4.9 rounds of the workgroups the device holds at once, 32-row
-# tiles 4.1 rounds and a 3% slower x sweep.
-HEADLINE_TUNING = (128, 27, 64, 64)
+# Launch shapes (x lanes, x rows, y lanes, y rows) of the headline step per
+# FP64 numerics.  The kernel trace and the HBM counters in
+# profiles/sweeps_4096_f64_hllc.md were taken with these shapes (the
+# autotuner's choices on this grid; --autotune-headline times the candidates
+# instead).
+HEADLINE_TUNING = {"fast": (128, 90, 64, 86), "exact": (128, 27, 64, 64)}
 
-# HBM bytes per launch / algorithmic bytes per launch of the 4096^2 FP64 HLLC
-# sweeps at HEADLINE_TUNING, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
-# calibrated against a plain 512 MiB device copy
-# (profiles/sweeps_4096_f64_hllc.md).  Profile-derived, not measured by this
-# run: the line says so in roofline.traffic_source.
-MEASURED_TRAFFIC_RATIO = {"x": 1.279, "y": 1.339}
-
-# VALU instructions issued per interior cell by each sweep at HEADLINE_TUNING
-# (rocprofv3 SQ_INSTS_VALU x 64 lanes / cells, profiles/sweeps_4096_f64_hllc.md),
-# and FP64 floating-point operations per interior cell (SQ_INSTS_VALU_ADD_F64
-# + MUL_F64 + TRANS_F64 + 2 FMA_F64, x 64 lanes / cells): profile-derived.
-PROFILED_VALU_PER_CELL = {"x": 531.0, "y": 514.0}
-PROFILED_F64_FLOP_PER_CELL = None
+# Per FP64 numerics, at HEADLINE_TUNING, from profiles/sweeps_4096_f64_hllc.md
+# (profile-derived, not measured by this run; the line says so in
+# roofline.traffic_source):
+#   traffic_ratio: HBM bytes / algorithmic bytes per launch (rocprofv3
+#     FETCH_SIZE and WRITE_SIZE, calibrated against a plain device copy);
+#   valu_per_cell: VALU instructions per interior cell (SQ_INSTS_VALU x 64
+#     lanes / cells);
+#   f64_flop_per_cell: FP64 operations per interior cell
+#     (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells).
+PROFILE = {
+    "fast": {"traffic_ratio": {"x": 1.279, "y": 1.339},
+             "valu_per_cell": {"x": 411.0, "y": 406.0},
+             "f64_flop_per_cell": {"x": 279.0, "y": 270.0}},
+    "exact": {"traffic_ratio": {"x": 1.279, "y": 1.339},
+              "valu_per_cell": {"x": 510.0, "y": 509.0},
+              "f64_flop_per_cell": {"x": 421.0, "y": 424.0}},
+}
 
 # VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
 # instruction per SIMD every 4 cycles = 64 lanes x 614.4 G = 39.3 T lane
@@ -79,8 +87,9 @@ def parse():
                     help="precision of --mode sweeps and --mode ghost")
     ap.add_argument("--flux", choices=["hllc", "force"], default="hllc",
                     help="flux of --mode sweeps")
-    ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
-                    help="FP64 numerics of --mode sweeps (fast: 1e-12 relative mode)")
+    ap.add_argument("--numerics", choices=["exact", "fast"], default="fast",
+                    help="FP64 numerics of the headline step and of --mode sweeps (fast: "
+                         "within 1e-12 relative L-inf of the reference; exact: its bits)")
     return ap.parse_args()
 
 
@@ -142,12 +151,12 @@ def main():
             w = torch.empty_like(a)
         stream.synchronize()
         explicit = args.x_block or args.x_rows or args.y_block or args.y_rows
-        headline = (flux == N.FL_FLUX_HLLC and dtype == torch.float64
-                    and numerics == N.FL_NUMERICS_EXACT)
+        headline = flux == N.FL_FLUX_HLLC and dtype == torch.float64
         if headline and not explicit and not args.autotune_headline:
-            xb, xr, yb, yr = HEADLINE_TUNING
+            shape = HEADLINE_TUNING["fast" if numerics == N.FL_NUMERICS_FAST else "exact"]
+            xb, xr, yb, yr = shape
             solver.set_tuning(x_block=xb, x_rows=xr, y_block=yb, y_rows=yr)
-            tuned[(flux, str(dtype), numerics)] = list(HEADLINE_TUNING)
+            tuned[(flux, str(dtype), numerics)] = list(shape)
         elif args.autotune and not explicit:
             t = solver.autotune(a, w, reps=2, stream=stream)
             tuned[(flux, str(dtype), numerics)] = [t.x_block, t.x_rows, t.y_block, t.y_rows]
@@ -162,7 +171,9 @@ def main():
         return solver, a, w, el, r
 
     # ---- headline: FP64 HLLC
-    solver, a, w, elapsed, res = run_case(N.FL_FLUX_HLLC, torch.float64, args.steps, args.warmup)
+    head_num = N.FL_NUMERICS_FAST if args.numerics == "fast" else N.FL_NUMERICS_EXACT
+    solver, a, w, elapsed, res = run_case(N.FL_FLUX_HLLC, torch.float64, args.steps, args.warmup,
+                                          head_num)
     value = args.steps * cells / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
 
@@ -182,14 +193,16 @@ def main():
     t_x = ev[0].elapsed_time(ev[1]) / reps * 1e-3
     t_y = ev[1].elapsed_time(ev[2]) / reps * 1e-3
     bytes_per_sweep = cells * 64.0            # read 32 B + write 32 B per cell (FP64, 2D)
-    dom_name, dom_t, dom_k = ("sweep_x_kernel<double,2,HLLC>", t_x, "x") if t_x >= t_y else \
-        ("sweep_y_kernel<double,HLLC>", t_y, "y")
+    kind = "fast" if head_num == N.FL_NUMERICS_FAST else "exact"
+    dom_name, dom_t, dom_k = (f"sweep_x_kernel<double,2,HLLC,{kind},van Leer>", t_x, "x") \
+        if t_x >= t_y else (f"sweep_y_kernel<double,HLLC,{kind},van Leer>", t_y, "y")
     achieved = bytes_per_sweep / dom_t / 1e9
-    profiled = (n == 4096 and tuned.get((N.FL_FLUX_HLLC, str(torch.float64), 0))
-                == list(HEADLINE_TUNING))
-    traffic = round(MEASURED_TRAFFIC_RATIO[dom_k] * bytes_per_sweep) if profiled else None
+    profiled = (n == 4096 and tuned.get((N.FL_FLUX_HLLC, str(torch.float64), head_num))
+                == list(HEADLINE_TUNING[kind]))
+    prof = PROFILE[kind]
+    traffic = round(prof["traffic_ratio"][dom_k] * bytes_per_sweep) if profiled else None
     copy_gbs = hbm_copy_ceiling(torch, stream)
-    valu_rate = PROFILED_VALU_PER_CELL[dom_k] * cells / dom_t
+    valu_rate = prof["valu_per_cell"][dom_k] * cells / dom_t
     roofline = {
         # The sweeps issue FP64 VALU work at about half the issue peak and move
         # HBM bytes at about a third of its peak: VALU issue bounds them.
@@ -207,14 +220,15 @@ def main():
         "algorithmic_bytes": int(bytes_per_sweep),
         "hbm_copy_GBps": round(copy_gbs, 1),
         "frac_of_copy": round(achieved / copy_gbs, 4),
-        "valu_insts_per_cell": PROFILED_VALU_PER_CELL[dom_k] if profiled else None,
+        "valu_insts_per_cell": prof["valu_per_cell"][dom_k] if profiled else None,
         "valu_achieved_Tinst_s": round(valu_rate / 1e12, 2) if profiled else None,
         "valu_peak_Tinst_s": round(VALU_PEAK_LANE_INST_S / 1e12, 2),
         "valu_frac": round(valu_rate / VALU_PEAK_LANE_INST_S, 4) if profiled else None,
-        "fp64_TFLOPs": (round(PROFILED_F64_FLOP_PER_CELL[dom_k] * cells / dom_t / 1e12, 2)
-                        if profiled and PROFILED_F64_FLOP_PER_CELL else None),
+        "fp64_TFLOPs": (round(prof["f64_flop_per_cell"][dom_k] * cells / dom_t / 1e12, 2)
+                        if profiled else None),
         "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS,
-        "tuning": tuned.get((N.FL_FLUX_HLLC, str(torch.float64), 0)),
+        "numerics": kind,
+        "tuning": tuned.get((N.FL_FLUX_HLLC, str(torch.float64), head_num)),
         "x_sweep_us": round(t_x * 1e6, 1),
         "y_sweep_us": round(t_y * 1e6, 1),
         "step_bytes_per_cell": 128,
@@ -225,15 +239,22 @@ def main():
     torch.cuda.empty_cache()
 
     variants = {}
+    # the FP64 step in the other numerics: the bit-exact one beside a fast
+    # headline (or the fast one beside an exact headline)
+    other = N.FL_NUMERICS_EXACT if head_num == N.FL_NUMERICS_FAST else N.FL_NUMERICS_FAST
+    exact_value = value if head_num == N.FL_NUMERICS_EXACT else None
     if not args.no_variants:
         vsteps = max(10, args.steps // 2)
         for name, flux, dt, bpc, num in [
-                ("hllc_f64_fast", N.FL_FLUX_HLLC, torch.float64, 128, N.FL_NUMERICS_FAST),
+                ("hllc_f64_exact" if other == N.FL_NUMERICS_EXACT else "hllc_f64_fast",
+                 N.FL_FLUX_HLLC, torch.float64, 128, other),
                 ("force_f64", N.FL_FLUX_FORCE, torch.float64, 128, N.FL_NUMERICS_EXACT),
                 ("force_f64_fast", N.FL_FLUX_FORCE, torch.float64, 128, N.FL_NUMERICS_FAST),
                 ("hllc_f32", N.FL_FLUX_HLLC, torch.float32, 64, N.FL_NUMERICS_EXACT)]:
-            s2, a2, w2, el2, _ = run_case(flux, dt, vsteps, 2, num)
+            s2, a2, w2, el2, _ = run_case(flux, dt, vsteps, max(2, args.warmup // 2), num)
             v = vsteps * cells / el2
+            if name == "hllc_f64_exact":
+                exact_value = v
             variants[name] = {"value": round(v, 1), "unit": "cell-updates/s",
                               "ms_per_step": round(el2 * 1e3 / vsteps, 4),
                               "hbm_frac": round(v * bpc / 1e9 / HBM_PEAK_GBS, 4),
@@ -241,6 +262,7 @@ def main():
                               "tuning": tuned.get((flux, str(dt), num))}
             del s2, a2, w2
             torch.cuda.empty_cache()
+        variants["hllc_f32"]["vs_f64_exact"] = round(variants["hllc_f32"]["value"] / exact_value, 3)
 
         variants["riemann_1d_f64"] = riemann_1d_case(torch, N, W, stream)
         variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5)
@@ -250,7 +272,9 @@ def main():
         variants["io_f64"] = io_case(torch, N, W, stream)
         bf = batched_case(torch, N, W, stream, steps=20)
         for k, v in bf.items():
+            # against the headline FP64 step and against the bit-exact one
             v["vs_f64_hllc"] = round(v["value"] / value, 3)
+            v["vs_f64_exact"] = round(v["value"] / exact_value, 3)
             variants[k] = v
 
     cpu = None
@@ -275,7 +299,9 @@ def main():
         "data": "synthetic (seeded four-quadrant Riemann data, seed 1001)",
         "config": {"workload": f"2D Euler {n}x{n}, gamma 1.4, MUSCL-Hancock + van Leer + HLLC, "
                                "dimensionally split x then y, transmissive, CFL 0.9",
-                   "nx": n, "ny": n, "layout": "SoA", "parallelism": "single GPU"},
+                   "nx": n, "ny": n, "layout": "SoA", "parallelism": "single GPU",
+                   "numerics": ("FP64 fast: within 1e-12 relative L-inf of the reference"
+                                if kind == "fast" else "FP64 exact: the reference's bits")},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
@@ -568,7 +594,7 @@ def sweeps_mode(args, wl, torch, E, N, W):
         solver.set_tuning(x_block=args.x_block, x_rows=args.x_rows, y_block=args.y_block,
                           y_rows=args.y_rows)
     else:
-        xb, xr, yb, yr = HEADLINE_TUNING
+        xb, xr, yb, yr = HEADLINE_TUNING[args.numerics]
         solver.set_tuning(x_block=xb, x_rows=xr, y_block=yb, y_rows=yr)
     a = torch.from_numpy(W.aos_to_soa(wl.U)).to(device="cuda:0", dtype=dt)
     w = torch.empty_like(a)

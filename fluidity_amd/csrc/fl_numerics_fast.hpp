@@ -65,6 +65,13 @@ __device__ __forceinline__ float sqt(float x) { return sq(x); }
 
 template <typename T> __device__ __forceinline__ T div(T a, T b) { return a * rcp(b); }
 
+// Minimum and maximum as single v_min/v_max instructions.  They differ from
+// the comparison-based fl::smin/smax only in which operand a NaN or a signed
+// zero leaves: a NaN operand yields the other one, as std::min(max, NaN) and
+// std::max(0, NaN) do where the reference uses them to clamp.
+template <typename T> __device__ __forceinline__ T mn(T a, T b) { return __builtin_fmin(a, b); }
+template <typename T> __device__ __forceinline__ T mx(T a, T b) { return __builtin_fmax(a, b); }
+
 // Primitive quantities and 1/rho of one conservative state.
 template <typename T, int ND> struct PrimR {
   T vel[ND];
@@ -120,7 +127,7 @@ template <typename T, int ND>
 __device__ __forceinline__ T max_wavespeed(const PrimR<T, ND>& w, const Material<T>& m) {
   T mv = fabs(w.vel[0]);
 #pragma unroll
-  for (int i = 1; i < ND; ++i) mv = smax(mv, T(fabs(w.vel[i])));
+  for (int i = 1; i < ND; ++i) mv = fast::mx(mv, T(fabs(w.vel[i])));
   return mv + fast::sound_speed(w.p, w.rinv, m);
 }
 
@@ -132,7 +139,7 @@ __device__ __forceinline__ T superbee(T a, T b) {
   if (r <= T(0) || b == T(0)) return T(0);
   if (r <= T(0.5)) return T(2) * r;
   if (r <= T(1)) return T(1);
-  return smin(fast::div(T(2), T(1) + r), smin(r, T(2)));
+  return fast::mn(fast::div(T(2), T(1) + r), fast::mn(r, T(2)));
 }
 
 // The limited slope (back + fwrd)/2 * limit(back, fwrd) of one component,
@@ -165,7 +172,7 @@ __device__ __forceinline__ T chi_factor(T p_pprev, T p_prev, T p_next, T p_nnext
 #pragma clang fp contract(fast)
   const T u_factor = v_prev - v_next;
   const T dp = fabs(p_next - p_prev);
-  T pmin = smin(p_next, p_prev);
+  T pmin = fast::mn(p_next, p_prev);
   if (m.corr) pmin = pmin + m.p_inf;
   // dp / pmin > 0.33 without the quotient: for pmin > 0 it is dp > 0.33 pmin;
   // pmin = 0 gives +inf (dp > 0) or NaN (dp = 0); pmin < 0 a quotient <= 0;
@@ -174,7 +181,7 @@ __device__ __forceinline__ T chi_factor(T p_pprev, T p_prev, T p_next, T p_nnext
   if (!flatten) return T(1);
   const T z = fast::div(dp, T(fabs(p_nnext - p_pprev)));
   const T temp = T(1) - (z - T(0.75)) * T(10);
-  return smin(T(1), smax(T(0), temp));
+  return fast::mn(T(1), fast::mx(T(0), temp));
 }
 
 // Both evolved face states of one cell (reconstruction/muscl_reconstructor.hpp:
@@ -245,7 +252,7 @@ __device__ __forceinline__ void hllc_flux(const T* ul, const T* ur, const Materi
   const T ar = fast::sound_speed(wr.p, wr.rinv, m);
   const T vl = wl.vel[D], vr = wr.vel[D];
   const T p_pvrs = T(0.5) * ((wl.p + wr.p) - (T(0.25) * (vr - vl) * (ur[0] + ul[0]) * (al + ar)));
-  const T p_star = m.corr ? smax(-m.p_inf, p_pvrs) : smax(T(0), p_pvrs);
+  const T p_star = m.corr ? fast::mx(-m.p_inf, p_pvrs) : fast::mx(T(0), p_pvrs);
 
   const T wsl = fast::hllc_wavespeed(vl, wl.p, p_star, -al, m);
   if (T(0) <= wsl) {

@@ -280,6 +280,26 @@ __device__ __forceinline__ double cell_wavespeed64(const T* u, const Material<do
 template <typename T, bool FAST>
 struct UseFast { static constexpr bool value = FAST || std::is_same<T, float>::value; };
 
+// Halo lanes of a tile of B lanes: the first kHalo lanes stage the left halo
+// cells (index 0..2, LDS slots 0..2), the last kHalo lanes the right ones
+// (index 3..5, slots B+3..B+5), so that the first and the last wave of a
+// workgroup share the halo work.  The other lanes load one of the six halo
+// cells as well (unused), which keeps the halo loads unconditional.
+__device__ __forceinline__ int halo_index(int t, int B) {
+  return t < kHalo ? t : (t >= B - kHalo ? t - (B - 2 * kHalo) : t % (2 * kHalo));
+}
+__device__ __forceinline__ bool halo_lane(int t, int B) { return t < kHalo || t >= B - kHalo; }
+// the lanes that compute the flattening factor just outside the lane range
+__device__ __forceinline__ bool halo_chi_lane(int t, int B) { return t == 0 || t == B - 1; }
+
+// min of the flattening factors: the reference's comparison form, or one
+// v_min in the fast numerics
+template <bool F, typename T>
+__device__ __forceinline__ T min_sel(T a, T b) {
+  if constexpr (F) return fast::mn(a, b);
+  else return smin(a, b);
+}
+
 template <bool F, typename T, int ND>
 __device__ __forceinline__ Prim<T, ND> prim_sel(const T* U, const Material<T>& m) {
   if constexpr (F) return fast::as_prim(fast::prim_of<T, ND>(U, m));
@@ -377,11 +397,9 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
   const int64_t nseg1 = (a.nx + B - 3) / (B - 2) - seg0;
   const int64_t segs = (ND == 2) ? 1 : min((int64_t)a.rows_per_block, nseg1);
 
-  // Halo lanes: t < 6 stage columns x0-4..x0-2 (slots 0..2) and
+  // Halo lanes (halo_index) stage columns x0-4..x0-2 (slots 0..2) and
   // x0+B-1..x0+B+1 (slots B+3..B+5) of the output row.
-  // Every lane loads one of the six halo cells (lane t takes cell t mod 6), so
-  // the halo loads are unconditional; only lanes t < 6 stage them in LDS.
-  const int hl = t % (2 * kHalo);
+  const int hl = halo_index(t, B);
   const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - 1 - kHalo + hslot;
   // column of this lane and its halo cell for row (1D: segment) r
@@ -476,7 +494,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     for (int k = 0; k < S; ++k) sU[k * W + s] = U0[k];
     sP[s] = p0;
     sV[s] = vx0;
-    if (t < 2 * kHalo) {
+    if (halo_lane(t, B)) {
       const Prim<T, ND> wh = prim_sel<F, T, ND>(uh, a.mat);
 #pragma unroll
       for (int k = 0; k < S; ++k) sU[k * W + hslot] = uh[k];
@@ -488,7 +506,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     // ---- phase B: x flattening factors K_x (own column, plus the two
     // columns just outside the lane range)
     sK[s] = chi_sel<F>(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1], a.mat);
-    if (t < 2) {
+    if (halo_chi_lane(t, B)) {
       const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
       sK[hs] = chi_sel<F>(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1],
                          sV[hs + 1], a.mat);
@@ -503,11 +521,11 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
     __syncthreads();
 
     // ---- phase C: chi (muscl_reconstructor.hpp:106-114) and face states
-    T chi = smin(Limits<T>::max(), smin(sK[s], sK[s + sx]));
+    T chi = min_sel<F>(Limits<T>::max(), min_sel<F>(sK[s], sK[s + sx]));
     if (ND == 2) {
       const int sy = signum(pw[2] - pw[0]);
       const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
-      chi = smin(chi, smin(ky1, ky_nb));
+      chi = min_sel<F>(chi, min_sel<F>(ky1, ky_nb));
     }
     T R[S], L[S];
     evolve_sel<F, T, ND, 0>(um, U0, up, chi, half_dtdh, limiter, form, a.mat, R, L);
@@ -585,11 +603,9 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
   const int64_t y0 = by * a.rows_per_block;
   const int64_t yend = min(y0 + (int64_t)a.rows_per_block, a.ny);
 
-  // Halo lanes stage columns x0-3..x0-1 (slots 0..2) and x0+B..x0+B+2
-  // (slots B+3..B+5).
-  // Every lane loads one of the six halo cells (lane t takes cell t mod 6);
-  // only lanes t < 6 stage them in LDS.
-  const int hl = t % (2 * kHalo);
+  // Halo lanes (halo_index) stage columns x0-3..x0-1 (slots 0..2) and
+  // x0+B..x0+B+2 (slots B+3..B+5).
+  const int hl = halo_index(t, B);
   const int hslot = (hl < 3) ? hl : B + hl;
   const int64_t hcol = x0 - kHalo + hslot;
 
@@ -654,21 +670,21 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
     if (r >= y0 - 1) {  // block-uniform
       sP[s] = w.p;
       sV[s] = w.vel[0];
-      if (t < 2 * kHalo) {
+      if (halo_lane(t, B)) {
         const Prim<T, ND> wh = prim_sel<F, T, ND>(uh, a.mat);
         sP[hslot] = wh.p;
         sV[hslot] = wh.vel[0];
       }
       __syncthreads();
       sK[s] = chi_sel<F>(sP[s - 2], sP[s - 1], sP[s + 1], sP[s + 2], sV[s - 1], sV[s + 1], a.mat);
-      if (t < 2) {
+      if (halo_chi_lane(t, B)) {
         const int hs = (t == 0) ? kHalo - 1 : B + kHalo;
         sK[hs] = chi_sel<F>(sP[hs - 2], sP[hs - 1], sP[hs + 1], sP[hs + 2], sV[hs - 1],
                          sV[hs + 1], a.mat);
       }
       const int sx = signum(sP[s + 1] - sP[s - 1]);
       __syncthreads();
-      cxw[3] = smin(sK[s], sK[s + sx]);
+      cxw[3] = min_sel<F>(sK[s], sK[s + sx]);
     }
 
     // ---- y flattening K_y of row r-2
@@ -691,9 +707,9 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
         up[k] = ring_t[(s2 * S + k) * 64];
       }
       const int sy = signum(pw[2] - pw[0]);
-      T chi = smin(Limits<T>::max(), cxw[0]);
+      T chi = min_sel<F>(Limits<T>::max(), cxw[0]);
       const T ky_nb = sy > 0 ? ky2 : (sy < 0 ? ky0 : ky1);
-      chi = smin(chi, smin(ky1, ky_nb));
+      chi = min_sel<F>(chi, min_sel<F>(ky1, ky_nb));
       T R[S], L[S];
       evolve_sel<F, T, ND, 1>(um, u0, up, chi, half_dtdh, limiter, form, a.mat, R, L);
       if (r >= y0 + 3) {
