@@ -47,12 +47,12 @@ HEADLINE_TUNING = {"fast": (128, 90, 64, 86), "exact": (128, 27, 64, 64)}
 #   f64_flop_per_cell: FP64 operations per interior cell
 #     (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells).
 PROFILE = {
-    "fast": {"traffic_ratio": {"x": 1.279, "y": 1.339},
-             "valu_per_cell": {"x": 411.0, "y": 406.0},
-             "f64_flop_per_cell": {"x": 279.0, "y": 270.0}},
-    "exact": {"traffic_ratio": {"x": 1.279, "y": 1.339},
-              "valu_per_cell": {"x": 510.0, "y": 509.0},
-              "f64_flop_per_cell": {"x": 421.0, "y": 424.0}},
+    "fast": {"traffic_ratio": {"x": 1.228, "y": 1.316},
+             "valu_per_cell": {"x": 396.2, "y": 390.2},
+             "f64_flop_per_cell": {"x": 283.8, "y": 277.4}},
+    "exact": {"traffic_ratio": {"x": 1.328, "y": 1.339},
+              "valu_per_cell": {"x": 510.0, "y": 503.2},
+              "f64_flop_per_cell": {"x": 433.4, "y": 433.7}},
 }
 
 # VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
