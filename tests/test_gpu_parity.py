@@ -541,6 +541,29 @@ def test_simulate_corrected_stiff_bitwise(E, oracle, flux, shape):
 
 
 
+@pytest.mark.parametrize("prec", ["f64", "f64fast", "f32"])
+def test_step_loop_independent_of_x_tile_width(E, prec):
+    """x-sweep tiles of 64, 128 and 256 lanes (one, two and four waves per
+    workgroup) and several heights give the same bits over a full step loop
+    (x and y sweeps, fused CFL), FP64 exact and fast and FP32."""
+    U = cons_field("riemann", (150, 200), seed=77)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    outs = []
+    for xb, xr in ((64, 5), (64, 33), (128, 7), (256, 16)):
+        sch = _escheme(E, 0, 0, 0, (0, 0))
+        sch.numerics = 1 if prec == "f64fast" else 0
+        solver = E.SplitSolver(sch, layout=1)
+        solver.set_tuning(x_block=xb, x_rows=xr, y_block=64, y_rows=16)
+        a = _dev(U, dtype=dt, layout=1)
+        w = torch.empty_like(a)
+        r = solver.simulate(a, w, 0.9, 1.0 / 200, max_iters=25)
+        assert r.iters == 25
+        outs.append((_host(a, 1), r.run_time))
+    for got, rt in outs[1:]:
+        assert rt == outs[0][1]
+        assert _bitwise(got, outs[0][0])
+
+
 # ---- fast FP64 numerics ---------------------------------------------------------------------
 
 def _rel_linf(got, ref):
