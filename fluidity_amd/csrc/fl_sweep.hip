@@ -228,18 +228,23 @@ __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int tar
   __shared__ int ok;
   __syncthreads();
   if (threadIdx.x == 0) {
+    // the release fence publishes this workgroup's stores and atomics; the
+    // polls are relaxed agent-scope loads (they see the counter at the
+    // device's coherence point) and one acquire fence follows the last, so
+    // no poll invalidates the caches; the error flag is read every 64 polls
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned int spins = 0;
     int good = 1;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++spins > spin_limit ||
-          __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      ++spins;
+      if (spins > spin_limit ||
+          ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         good = 0;
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     ok = good;
@@ -266,7 +271,17 @@ __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(
 persist1d_kernel(SweepArgs<T> a_ab, SweepArgs<T> a_ba, PersistCtl* pc, double cfl, double dh,
                  double sim_time, long long max_iters, long long k0, long long nsteps,
                  unsigned int spin_limit) {
+  const int64_t col = (int64_t)blockIdx.x * (blockDim.x - 2) - 1 + threadIdx.x;
+  const int64_t ncell = a_ab.nx;
   for (long long k = k0; k < k0 + nsteps; ++k) {
+    // This workgroup's cells of the step's input, read while the control
+    // words are on their way (after the barrier they come from device
+    // memory): the tile's own loads of them then hit the caches.
+    const SweepArgs<T>& ain = (k & 1) ? a_ba : a_ab;
+    const int64_t pc_col = col < 0 ? 0 : (col >= ncell ? ncell - 1 : col);
+    T warm[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) warm[q] = ain.in[pc_col * ain.es + q * ain.cs];
     const double maxws = __longlong_as_double((long long)load_u64_agent(&pc->ws[k % 3]));
     const double t = __longlong_as_double((long long)load_u64_agent(&pc->t[k & 1]));
     const long long it =
@@ -284,6 +299,7 @@ persist1d_kernel(SweepArgs<T> a_ab, SweepArgs<T> a_ba, PersistCtl* pc, double cf
     a.ctl = nullptr;
     a.dtdh_host = (T)dtdh;
     a.ws_out = &pc->ws[(k + 1) % 3];
+    asm volatile("" ::"v"(warm[0]), "v"(warm[1]), "v"(warm[2]));
     sweep_x_tile<T, 1, FLUX>(a, blockIdx.x, 0);
     if (!grid_barrier(&pc->bar, (unsigned int)((k - k0 + 1) * gridDim.x), spin_limit, &pc->err))
       return;
