@@ -74,7 +74,11 @@ struct AdjLayout {
   double* V0;        // same: x velocity
   double* V1;        // same: y velocity
   double* chi;       // (nx+2)(ny+2): flattening factor within 1 ghost cell
-  double* partials;  // 2 sweeps * 2 * kPartialBlocks
+  double* chi_g;     // faces * 2: chi adjoints of each face's two cells
+  int* rec_k;        // (nx+2)(ny+2): the factor each chi minimum selected
+  double* rec_v;     // (nx+2)(ny+2) * 4: its pressure adjoints
+  int* any;          // 1: some chi position has a record
+  double* partials;  // 4 slots * 2 * kPartialBlocks: x, y faces; x, y chi
   aj::StepRec* rec;  // K
   double* acc;       // 2
   fl::StepCtl* ctl;  // 1
@@ -103,8 +107,13 @@ AdjLayout adj_layout(void* base, const fl_field* f, int K) {
   L.P = (double*)take(padded * 8);
   L.V0 = (double*)take(padded * 8);
   L.V1 = (double*)take(padded * 8);
-  L.chi = (double*)take((nx + 2) * (f->ndim > 1 ? ny + 2 : 1) * 8);
-  L.partials = (double*)take(2 * 2 * aj::kPartialBlocks * 8);
+  const int64_t cpad = (nx + 2) * (f->ndim > 1 ? ny + 2 : 1);
+  L.chi = (double*)take(cpad * 8);
+  L.chi_g = (double*)take(nf * 2 * 8);
+  L.rec_k = (int*)take(cpad * 4);
+  L.rec_v = (double*)take(cpad * 4 * 8);
+  L.any = (int*)take(4);
+  L.partials = (double*)take(4 * 2 * aj::kPartialBlocks * 8);
   L.rec = (aj::StepRec*)take(sizeof(aj::StepRec) * K);
   L.acc = (double*)take(2 * 8);
   L.ctl = (fl::StepCtl*)take(sizeof(fl::StepCtl));
@@ -179,10 +188,12 @@ int forward_steps(const fl_context* c, const fl_field* f, const fl_scheme* s, do
   return FL_OK;
 }
 
+// Adjoint of one sweep: lam_out -> lam_in; the faces' (dt/dh, gamma) sums
+// into `partials`, the flattening factor's gamma sums into `chi_partials`.
 template <int ND, int D>
 void sweep_adj(const fl_field* f, const fl_scheme* s, const double* Uin, const double* lam_out,
                double* lam_in, const aj::StepRec* rec, const AdjLayout& L, double* partials,
-               hipStream_t st) {
+               double* chi_partials, hipStream_t st) {
   const aj::AGrid g = agrid(f, Uin, s);
   const fl::Material<double> m = mat64(s);
   const int64_t n = D == 0 ? g.nx : g.ny;
@@ -193,14 +204,24 @@ void sweep_adj(const fl_field* f, const fl_scheme* s, const double* Uin, const d
   hipLaunchKernelGGL(aj::prim_pad_kernel<ND>, dim3((unsigned)ceil_div(padded, 256)), dim3(256), 0,
                      st, g, m, L.P, L.V0, L.V1);
   hipLaunchKernelGGL(aj::chi_pad_kernel<ND>, dim3((unsigned)ceil_div(cpad, 256)), dim3(256), 0,
-                     st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1, L.chi);
+                     st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1, L.chi,
+                     L.any);
   hipLaunchKernelGGL((aj::face_adj_kernel<ND, D>), dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0,
                      st, g, lam_out, m, s->limiter, rec, (const double*)L.chi, L.contrib,
-                     L.face_g);
+                     L.face_g, L.chi_g);
   hipLaunchKernelGGL(aj::partial_sum_kernel, dim3(aj::kPartialBlocks), dim3(256), 0, st,
                      (const double*)L.face_g, nf, partials);
-  hipLaunchKernelGGL((aj::gather_adj_kernel<ND, D>), dim3((unsigned)ceil_div(g.nx * g.ny, 256)),
+  const int64_t cells = g.nx * g.ny;
+  hipLaunchKernelGGL((aj::gather_adj_kernel<ND, D>), dim3((unsigned)ceil_div(cells, 256)),
                      dim3(256), 0, st, g, lam_out, lam_in, (const double*)L.contrib);
+  // the flattening factor: the selected factors' pressure adjoints where
+  // 0 < chi < 1, then into lambda and the gamma sum
+  hipLaunchKernelGGL((aj::chi_adj_kernel<ND, D>), dim3((unsigned)ceil_div(cpad, 256)), dim3(256),
+                     0, st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1,
+                     (const double*)L.chi, (const double*)L.chi_g, L.rec_k, L.rec_v, L.any);
+  hipLaunchKernelGGL(aj::chi_gather_kernel<ND>, dim3(aj::kPartialBlocks), dim3(256), 0, st, g, m,
+                     (const int*)L.rec_k, (const double*)L.rec_v, (const int*)L.any, lam_in,
+                     chi_partials);
 }
 
 // Backward through step k of the segment: lam_next -> lam_prev.
@@ -211,16 +232,19 @@ int backward_step(const fl_field* f, const fl_scheme* s, const AdjLayout& L, int
   const double* Uk = L.U + k * cells * S;
   const aj::StepRec* rec = L.rec + k;
   const fl::Material<double> m = mat64(s);
+  // partial slots (2 * kPartialBlocks each): faces of the sweeps, then their
+  // flattening-factor terms
+  auto slot = [&](int i) { return L.partials + 2 * aj::kPartialBlocks * i; };
   if (f->ndim == 1) {
-    sweep_adj<1, 0>(f, s, Uk, lam_next, lam_prev, rec, L, L.partials, st);
+    sweep_adj<1, 0>(f, s, Uk, lam_next, lam_prev, rec, L, slot(0), slot(1), st);
     hipLaunchKernelGGL(aj::finalize_kernel<1>, dim3(1), dim3(256), 0, st, agrid(f, Uk, s), m, rec,
-                       (const double*)L.partials, 1, lam_prev, L.acc);
+                       (const double*)L.partials, 2, lam_prev, L.acc);
   } else {
     const double* Xk = L.X + k * cells * S;
-    sweep_adj<2, 1>(f, s, Xk, lam_next, L.lam[2], rec, L, L.partials + 2 * aj::kPartialBlocks, st);
-    sweep_adj<2, 0>(f, s, Uk, L.lam[2], lam_prev, rec, L, L.partials, st);
+    sweep_adj<2, 1>(f, s, Xk, lam_next, L.lam[2], rec, L, slot(1), slot(3), st);
+    sweep_adj<2, 0>(f, s, Uk, L.lam[2], lam_prev, rec, L, slot(0), slot(2), st);
     hipLaunchKernelGGL(aj::finalize_kernel<2>, dim3(1), dim3(256), 0, st, agrid(f, Uk, s), m, rec,
-                       (const double*)L.partials, 2, lam_prev, L.acc);
+                       (const double*)L.partials, 4, lam_prev, L.acc);
   }
   FL_HIP(hipGetLastError());
   return FL_OK;

@@ -76,10 +76,12 @@ template <int ND>
 __global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __restrict__ P,
                                                       const double* __restrict__ V0,
                                                       const double* __restrict__ V1,
-                                                      double* __restrict__ chi) {
+                                                      double* __restrict__ chi,
+                                                      int* __restrict__ any) {
   const int64_t px = g.nx + 8;
   const int64_t cx = g.nx + 2, cy = ND > 1 ? g.ny + 2 : 1;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) *any = 0;  // chi_adj_kernel's record flag, read by chi_gather_kernel
   if (q >= cx * cy) return;
   const int64_t x = q % cx - 1, y = ND > 1 ? q / cx - 1 : 0;
   auto at = [&](const double* A, int64_t xx, int64_t yy) {
@@ -108,7 +110,8 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
                                                        const StepRec* __restrict__ rec,
                                                        const double* __restrict__ chi_pad,
                                                        double* __restrict__ contrib,
-                                                       double* __restrict__ face_g) {
+                                                       double* __restrict__ face_g,
+                                                       double* __restrict__ chi_g) {
   constexpr int S = 2 + ND;
   const int64_t n = D == 0 ? g.nx : g.ny;          // cells along the sweep
   const int64_t lines = D == 0 ? g.ny : g.nx;
@@ -186,11 +189,13 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int k = 0; k < S; ++k) c[s][k] = 0.0;
-  double g_half = 0.0;
+  double g_half = 0.0, g_chi_a = 0.0, g_chi_b = 0.0;
   evolve_adj<ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, m, gR, zero, c[0], c[1], c[2],
-                    g_half, gm);
+                    g_half, g_chi_a, gm);
   evolve_adj<ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, m, zero, gL, c[1], c[2], c[3],
-                    g_half, gm);
+                    g_half, g_chi_b, gm);
+  chi_g[2 * f] = g_chi_a;      // of cell i (its right face state)
+  chi_g[2 * f + 1] = g_chi_b;  // of cell i + 1 (its left face state)
   // contributions as planes (slot, component) x faces: coalesced both here and
   // in the gather
   const int64_t nf = (n + 1) * lines;
@@ -252,6 +257,212 @@ __global__ void __launch_bounds__(256) gather_adj_kernel(AGrid g, const double* 
   double* o = lam_in + cell * g.es;
 #pragma unroll
   for (int k = 0; k < S; ++k) o[k * g.cs] = acc[k];
+}
+
+// ---- the flattening factor's adjoint ----------------------------------------
+// chi_v of a cell (chi_pad_kernel) is the smallest of the flattening factors
+// K_d(j) of the cell and of its neighbour on the higher-pressure side in each
+// dimension d; where the selected factor is flattened and not clamped,
+// K = 1 - (z - 0.75) / 0.1 with z = |p(j+1) - p(j-1)| / |p(j+2) - p(j-2)|
+// along d carries the gradient (muscl_reconstructor.hpp:59-87); the
+// flattening test, the neighbour and the minimum are branches.
+
+// At each chi position (the grid padded by one cell) whose chi lies strictly
+// between 0 and 1 (elsewhere the selected factor is clamped or not
+// flattened and carries no gradient): the adjoint of chi, the sum of what the
+// position's right face state (face i+1/2) and left face state (face i-1/2)
+// passed back in sweep dimension D; which factor the minimum selected,
+// rec_k = 4 d + (j - c along d) + 1 for K_d(j) (-1: none); and the adjoints
+// of the pressures at j-2, j-1, j+1, j+2 along d (rec_v, 4 per position).
+// *any is set when some position has a record.
+template <int ND, int D>
+__global__ void __launch_bounds__(256) chi_adj_kernel(AGrid g, const double* __restrict__ P,
+                                                      const double* __restrict__ V0,
+                                                      const double* __restrict__ V1,
+                                                      const double* __restrict__ chi,
+                                                      const double* __restrict__ chi_g,
+                                                      int* __restrict__ rec_k,
+                                                      double* __restrict__ rec_v,
+                                                      int* __restrict__ any) {
+  const int64_t px = g.nx + 8;
+  const int64_t cx = g.nx + 2, cy = ND > 1 ? g.ny + 2 : 1;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cx * cy) return;
+  const int64_t x = q % cx - 1, y = ND > 1 ? q / cx - 1 : 0;
+  const double cv = chi[q];
+  double G = 0.0;
+  if (cv > 0.0 && cv < 1.0) {
+    const int64_t n = D == 0 ? g.nx : g.ny;
+    const int64_t lines = D == 0 ? g.ny : g.nx;
+    const int64_t i = D == 0 ? x : y;     // along the sweep: -1 .. n
+    const int64_t line = D == 0 ? y : x;  // faces exist on interior lines only
+    if (line >= 0 && line < lines) {
+      auto face = [&](int64_t ii) {
+        return D == 0 ? line * (n + 1) + (ii + 1) : (ii + 1) * lines + line;
+      };
+      if (i <= n - 1) G += chi_g[2 * face(i)];
+      if (i >= 0) G += chi_g[2 * face(i - 1) + 1];
+    }
+  }
+  auto at = [&](const double* A, int64_t xx, int64_t yy) {
+    return A[(ND > 1 ? (yy + 4) * px : 0) + xx + 4];
+  };
+  const Material<double> no_shift{};
+  auto chi_d = [&](int64_t xx, int64_t yy, int d) {
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const double* V = d == 0 ? V0 : V1;
+    return chi_factor<double>(at(P, xx - 2 * dx, yy - 2 * dy), at(P, xx - dx, yy - dy),
+                              at(P, xx + dx, yy + dy), at(P, xx + 2 * dx, yy + 2 * dy),
+                              at(V, xx - dx, yy - dy), at(V, xx + dx, yy + dy), no_shift);
+  };
+  int best = -1;
+  if (G != 0.0) {
+    // the selection of chi_pad_kernel: smin(chi, smin(K(c), K(c + s)))
+    double c = DBL_MAX;
+    for (int d = 0; d < ND; ++d) {
+      const int64_t dx = (d == 0), dy = (d == 1);
+      const int sign = signum(at(P, x + dx, y + dy) - at(P, x - dx, y - dy));
+      const double k0 = chi_d(x, y, d);
+      const double k1 = chi_d(x + sign * dx, y + sign * dy, d);
+      double mv = k0;
+      int sel = 4 * d + 1;
+      if (k1 < k0) {
+        mv = k1;
+        sel = 4 * d + sign + 1;
+      }
+      if (mv < c) {
+        c = mv;
+        best = sel;
+      }
+    }
+  }
+  if (best >= 0) {
+    const int d = best / 4, jo = best % 4 - 1;
+    const int64_t dx = (d == 0), dy = (d == 1);
+    const int64_t jx = x + jo * dx, jy = y + jo * dy;
+    const double* V = d == 0 ? V0 : V1;
+    const double p_pp = at(P, jx - 2 * dx, jy - 2 * dy), p_p = at(P, jx - dx, jy - dy);
+    const double p_n = at(P, jx + dx, jy + dy), p_nn = at(P, jx + 2 * dx, jy + 2 * dy);
+    const double u_factor = at(V, jx - dx, jy - dy) - at(V, jx + dx, jy + dy);
+    const double dp = fabs(p_n - p_p);
+    const double pmin = smin(p_n, p_p);
+    const bool flatten = u_factor > 0.0 && dv(dp, pmin) > 0.33;
+    const double den = fabs(p_nn - p_pp);
+    if (flatten && den > 0.0) {
+      const double z = dv(dp, den);
+      const double temp = 1.0 - dv(z - 0.75, 0.85 - 0.75);
+      if (temp > 0.0 && temp < 1.0) {
+        const double g_z = -G / (0.85 - 0.75);
+        const double g_dp = g_z / den;
+        const double g_den = -g_z * z / den;
+        const double sn = (double)signum(p_n - p_p), snn = (double)signum(p_nn - p_pp);
+        rec_v[4 * q + 0] = -g_den * snn;  // p(j-2)
+        rec_v[4 * q + 1] = -g_dp * sn;    // p(j-1)
+        rec_v[4 * q + 2] = g_dp * sn;     // p(j+1)
+        rec_v[4 * q + 3] = g_den * snn;   // p(j+2)
+        rec_k[q] = best;
+        *any = 1;
+        return;
+      }
+    }
+  }
+  rec_k[q] = -1;
+}
+
+// Positions within 4 ghost cells of a line of n cells whose ghost rule maps
+// to cell x (x itself first), at most 3.
+__device__ __forceinline__ int alias_positions(int64_t x, int64_t n, int bc, int64_t* pos) {
+  int k = 0;
+  pos[k++] = x;
+  int64_t cand[2];
+  if (bc == kBcPeriodic) {
+    cand[0] = x - n;
+    cand[1] = x + n;
+  } else {
+    cand[0] = -x - 1;
+    cand[1] = 2 * n - 1 - x;
+  }
+  for (int a = 0; a < 2; ++a) {
+    const int64_t c = cand[a];
+    if (c < -4 || c > n + 3 || (c >= 0 && c < n)) continue;
+    bool flip;
+    if (ghost_index(c, n, bc, flip) == x) pos[k++] = c;
+  }
+  return k;
+}
+
+// lambda += (dp/dU) times the pressure adjoints that the chi positions'
+// selected factors pass to each cell (at the cell and at its ghost aliases,
+// in a fixed order), and the gamma adjoint of the same terms (dp/dgamma)
+// summed per workgroup into partials[2 b + 1] (partials[2 b] = 0).
+// kPartialBlocks workgroups of 256 threads stride over the cells, so the sums
+// have a fixed order; with no record at all (*any == 0) they only write
+// zeros.
+template <int ND>
+__global__ void __launch_bounds__(256) chi_gather_kernel(AGrid g, Material<double> m,
+                                                         const int* __restrict__ rec_k,
+                                                         const double* __restrict__ rec_v,
+                                                         const int* __restrict__ any,
+                                                         double* __restrict__ lam,
+                                                         double* __restrict__ partials) {
+  constexpr int S = 2 + ND;
+  __shared__ double red[256];
+  double acc = 0.0;
+  const int64_t cx = g.nx + 2;
+  if (*any) {
+    for (int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x; cell < g.nx * g.ny;
+         cell += (int64_t)kPartialBlocks * 256) {
+      const int64_t x = cell % g.nx, y = cell / g.nx;
+      int64_t ax[3], ay[3] = {0, 0, 0};
+      const int nax = alias_positions(x, g.nx, g.bcx, ax);
+      const int nay = ND > 1 ? alias_positions(y, g.ny, g.bcy, ay) : 1;
+      double gp = 0.0;
+      for (int a = 0; a < nax; ++a)
+        for (int b = 0; b < nay; ++b)
+          for (int d = 0; d < ND; ++d) {
+            const int64_t dx = (d == 0), dy = (d == 1);
+            // a record at position c selecting K_d(c + jo) reaches the
+            // pressure at c + jo + k, k in {-2, -1, 1, 2}: here t = jo + k
+            for (int t = -3; t <= 3; ++t) {
+              const int64_t qx = ax[a] - t * dx, qy = ay[b] - t * dy;
+              if (qx < -1 || qx > g.nx || (ND > 1 && (qy < -1 || qy > g.ny))) continue;
+              const int64_t qc = (ND > 1 ? (qy + 1) * cx : 0) + qx + 1;
+              const int r = rec_k[qc];
+              if (r < 0 || r / 4 != d) continue;
+              const int k = t - (r % 4 - 1);
+              if (k < -2 || k > 2 || k == 0) continue;
+              gp += rec_v[4 * qc + (k < 0 ? k + 2 : k + 1)];
+            }
+          }
+      if (gp != 0.0) {
+        double u[S];
+        const double* p = g.U + cell * g.es;
+#pragma unroll
+        for (int k = 0; k < S; ++k) u[k] = p[k * g.cs];
+        // p = (gamma - 1) (E - rho |v|^2 / 2), v = m / rho
+        const Prim<double, ND> w = prim_of<double, ND>(u, m);
+        double v2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) v2 += w.vel[i] * w.vel[i];
+        double* l = lam + cell * g.es;
+        l[0] += gp * (m.gm1 * 0.5 * v2);
+        l[1 * g.cs] += gp * m.gm1;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) l[(2 + i) * g.cs] -= gp * (m.gm1 * w.vel[i]);
+        acc += gp * (u[1] - 0.5 * u[0] * v2);
+      }
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = 0.0;
+    partials[2 * blockIdx.x + 1] = red[0];
+  }
 }
 
 // Per-block sums of the per-face (dt/dh, gamma) adjoints; kPartialBlocks
@@ -369,9 +580,12 @@ __global__ void acc_clear_kernel(double* acc) {
 #define FL_ADJ_INST(ND, D)                                                                  \
   template __global__ void face_adj_kernel<ND, D>(AGrid, const double*, Material<double>,    \
                                                   int, const StepRec*, const double*,        \
-                                                  double*, double*);                         \
+                                                  double*, double*, double*);                \
   template __global__ void gather_adj_kernel<ND, D>(AGrid, const double*, double*,           \
-                                                    const double*);
+                                                    const double*);                          \
+  template __global__ void chi_adj_kernel<ND, D>(AGrid, const double*, const double*,         \
+                                                 const double*, const double*, const double*, \
+                                                 int*, double*, int*);
 
 FL_ADJ_INST(1, 0)
 FL_ADJ_INST(2, 0)
@@ -379,9 +593,13 @@ FL_ADJ_INST(2, 1)
 template __global__ void prim_pad_kernel<1>(AGrid, Material<double>, double*, double*, double*);
 template __global__ void prim_pad_kernel<2>(AGrid, Material<double>, double*, double*, double*);
 template __global__ void chi_pad_kernel<1>(AGrid, const double*, const double*, const double*,
-                                           double*);
+                                           double*, int*);
 template __global__ void chi_pad_kernel<2>(AGrid, const double*, const double*, const double*,
-                                           double*);
+                                           double*, int*);
+template __global__ void chi_gather_kernel<1>(AGrid, Material<double>, const int*, const double*,
+                                              const int*, double*, double*);
+template __global__ void chi_gather_kernel<2>(AGrid, Material<double>, const int*, const double*,
+                                              const int*, double*, double*);
 template __global__ void argmax_kernel<1>(AGrid, Material<double>, StepRec*);
 template __global__ void argmax_kernel<2>(AGrid, Material<double>, StepRec*);
 template __global__ void finalize_kernel<1>(AGrid, Material<double>, const StepRec*,

@@ -142,13 +142,14 @@ __device__ __forceinline__ void limit_adj(int limiter, double L, double R, doubl
 // Adjoint of evolve_faces<double, ND, D> with conservative-form limiting.
 // Given the adjoints of the right and left evolved face states (gR, gL; either
 // may be all zeros), adds to the adjoints of the stencil cells (gum, gu0,
-// gup), of half_dtdh and of gamma.
+// gup), of half_dtdh, of the flattening factor chi and of gamma.
 template <int ND, int D>
 __device__ __forceinline__ void evolve_adj(const double* um, const double* u0, const double* up,
                                            double chi, double half_dtdh, int limiter,
                                            const Material<double>& m, const double* gR,
                                            const double* gL, double* gum, double* gu0,
-                                           double* gup, double& g_half_dtdh, GMat& gm) {
+                                           double* gup, double& g_half_dtdh, double& g_chi,
+                                           GMat& gm) {
   constexpr int S = 2 + ND;
   const double cscale = chi * 0.5;
   double plus[S], minus[S], back[S], fwrd[S], lim0[S], phi[S];
@@ -183,6 +184,8 @@ __device__ __forceinline__ void evolve_adj(const double* um, const double* u0, c
   for (int k = 0; k < S; ++k) {
     gu0[k] += g_plus[k] + g_minus[k];
     const double g_delta = g_plus[k] - g_minus[k];
+    // delta = (lim0 phi) (chi / 2)
+    g_chi += g_delta * (lim0[k] * phi[k]) * 0.5;
     const double g_lim = g_delta * cscale;
     const double g_lim0 = g_lim * phi[k];
     const double g_phi = g_lim * lim0[k];

@@ -29,11 +29,19 @@ template <int ND>
 __global__ void prim_pad_kernel(AGrid g, Material<double> m, double* P, double* V0, double* V1);
 template <int ND>
 __global__ void chi_pad_kernel(AGrid g, const double* P, const double* V0, const double* V1,
-                               double* chi);
+                               double* chi, int* any);
 template <int ND, int D>
 __global__ void face_adj_kernel(AGrid g, const double* lam, Material<double> m, int limiter,
                                 const StepRec* rec, const double* chi_pad, double* contrib,
-                                double* face_g);
+                                double* face_g, double* chi_g);
+template <int ND, int D>
+__global__ void chi_adj_kernel(AGrid g, const double* P, const double* V0, const double* V1,
+                               const double* chi, const double* chi_g, int* rec_k,
+                               double* rec_v, int* any);
+template <int ND>
+__global__ void chi_gather_kernel(AGrid g, Material<double> m, const int* rec_k,
+                                  const double* rec_v, const int* any, double* lam,
+                                  double* partials);
 template <int ND, int D>
 __global__ void gather_adj_kernel(AGrid g, const double* lam_out, double* lam_in,
                                   const double* contrib);

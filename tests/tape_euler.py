@@ -8,8 +8,9 @@ flux_method/flux_hllc.hpp:56-245, dt from simulator/parameters.hpp:60-63)
 with vectorised torch operations in the same operation order, so that its
 values equal the oracle's; torch.autograd then differentiates it.  Branches
 are torch.where selections (the derivative of the branch taken, with the
-unselected branch fed safe operands); the flattening factor chi is computed
-without gradient, as the adjoint holds it constant.
+unselected branch fed safe operands), the flattening factor chi included
+(its flattening test and the choice of the neighbour and of the minimum are
+branches; the clamped ratio carries the gradient).
 
 Grids are lists of component tensors [rho, E, m_x, (m_y)], each (ny, nx) in
 2D or (nx,) in 1D; bc is one of 0 transmissive, 1 reflective, 2 periodic per
@@ -96,54 +97,62 @@ def van_leer(L, R):
 def _chi_min(P, gamma, nd):
     """chi_v of MHReconstructor::evolve for every cell of the grid padded by
     one ghost layer in each dimension; returns a tensor shaped like the grid
-    padded by 1 everywhere.  No gradient."""
-    with torch.no_grad():
-        vel, p = prim([c.detach() for c in P], float(gamma), nd)
-        full = p.shape                                     # padded by PAD everywhere
+    padded by 1 everywhere."""
+    vel, p = prim(P, gamma, nd)
+    full = p.shape                                     # padded by PAD everywhere
 
-        def at(t, offs):
-            """t at offset offs (tuple per dim, x first) for every cell of the
-            grid padded by 1."""
-            sl = t
-            for d in range(nd):
-                axis = t.dim() - 1 - d
-                n = full[axis] - 2 * PAD
-                sl = sl.narrow(axis, PAD - 1 + offs[d], n + 2)
-            return sl
-
-        def chi_at(offs, d):
-            e = [0] * nd
-            e[d] = 1
-
-            def o(k):
-                return tuple(offs[i] + k * e[i] for i in range(nd))
-            p_next, p_prev = at(p, o(1)), at(p, o(-1))
-            p_nnext, p_pprev = at(p, o(2)), at(p, o(-2))
-            u_factor = at(vel[d], o(-1)) - at(vel[d], o(1))
-            dp = torch.abs(p_next - p_prev)
-            pmin = torch.where(p_prev < p_next, p_prev, p_next)
-            flatten = (u_factor > 0) & ((dp / pmin) > 0.33)
-            z = dp / torch.abs(p_nnext - p_pprev)
-            temp = 1.0 - ((z - 0.75) / (0.85 - 0.75))
-            lo = torch.where(0.0 < temp, temp, torch.zeros_like(temp))
-            c = torch.where(lo < 1.0, lo, torch.ones_like(lo))
-            return torch.where(flatten, c, torch.ones_like(c))
-
-        zero = tuple([0] * nd)
-        chi = torch.full(at(p, zero).shape, torch.finfo(torch.float64).max, dtype=torch.float64)
+    def at(t, offs):
+        """t at offset offs (tuple per dim, x first) for every cell of the
+        grid padded by 1."""
+        sl = t
         for d in range(nd):
-            e = [0] * nd
-            e[d] = 1
-            plus = tuple(e)
-            minus = tuple(-x for x in e)
-            sign = torch.sign(at(p, plus) - at(p, minus))
-            c0 = chi_at(zero, d)
-            cp = chi_at(plus, d)
-            cm = chi_at(minus, d)
-            cn = torch.where(sign > 0, cp, torch.where(sign < 0, cm, c0))
-            m = torch.where(cn < c0, cn, c0)                 # smin(chi(c), chi(c+s))
-            chi = torch.where(m < chi, m, chi)               # smin(chi_v, .)
-        return chi
+            axis = t.dim() - 1 - d
+            n = full[axis] - 2 * PAD
+            sl = sl.narrow(axis, PAD - 1 + offs[d], n + 2)
+        return sl
+
+    def chi_at(offs, d):
+        e = [0] * nd
+        e[d] = 1
+
+        def o(k):
+            return tuple(offs[i] + k * e[i] for i in range(nd))
+        p_next, p_prev = at(p, o(1)), at(p, o(-1))
+        p_nnext, p_pprev = at(p, o(2)), at(p, o(-2))
+        u_factor = at(vel[d], o(-1)) - at(vel[d], o(1))
+        dp = torch.abs(p_next - p_prev)
+        pmin = torch.where(p_prev < p_next, p_prev, p_next)
+        with torch.no_grad():
+            flatten = (u_factor > 0) & ((dp / pmin) > 0.33)
+        # the ratio only where it is used and finite (elsewhere 0 / 1, and
+        # +inf where the reference divides by a zero difference): no
+        # infinite or NaN operand whose zero gradient would turn into NaN
+        den = torch.abs(p_nnext - p_pprev)
+        ok = flatten & (den > 0)
+        z = torch.where(ok, torch.where(ok, dp, torch.zeros_like(dp))
+                        / torch.where(ok, den, torch.ones_like(den)),
+                        torch.where(flatten, torch.full_like(dp, float("inf")),
+                                    torch.zeros_like(dp)))
+        temp = 1.0 - ((z - 0.75) / (0.85 - 0.75))
+        lo = torch.where(0.0 < temp, temp, torch.zeros_like(temp))
+        c = torch.where(lo < 1.0, lo, torch.ones_like(lo))
+        return torch.where(flatten, c, torch.ones_like(c))
+
+    zero = tuple([0] * nd)
+    chi = torch.full(at(p, zero).shape, torch.finfo(torch.float64).max, dtype=torch.float64)
+    for d in range(nd):
+        e = [0] * nd
+        e[d] = 1
+        plus = tuple(e)
+        minus = tuple(-x for x in e)
+        sign = torch.sign(at(p, plus) - at(p, minus))
+        c0 = chi_at(zero, d)
+        cp = chi_at(plus, d)
+        cm = chi_at(minus, d)
+        cn = torch.where(sign > 0, cp, torch.where(sign < 0, cm, c0))
+        m = torch.where(cn < c0, cn, c0)                 # smin(chi(c), chi(c+s))
+        chi = torch.where(m < chi, m, chi)               # smin(chi_v, .)
+    return chi
 
 
 def hllc(ul, ur, gamma, D, nd):

@@ -88,6 +88,55 @@ def test_step_vjp_matches_tape(A, E, nd, bc):
     assert abs(dg - dg_ref) <= 1e-11 * max(abs(dg_ref), scale)
 
 
+def _shock_state(nd):
+    """Smeared compressive pressure jumps (along x, and y in 2D) whose
+    flattening factors lie strictly between 0 and 1 at many cells, with small
+    noise so that no two cells tie for the maximum wavespeed."""
+    ny, nx = (12, 20) if nd == 2 else (1, 40)
+    x = np.arange(nx)[None, :] + np.zeros((ny, 1))
+    y = np.arange(ny)[:, None] + np.zeros((1, nx))
+    xc = 9.3 if nd == 2 else 19.3
+    p = 1.5 + 0.5 * np.tanh((x - xc) / 0.8)
+    rho = 1.0 + 0.3 * np.tanh((x - xc) / 0.8)
+    u = -0.4 * np.tanh((x - xc) / 0.8)
+    v = np.zeros_like(u)
+    if nd == 2:
+        p = p + 0.4 * np.tanh((y - 5.6) / 0.7)
+        u = u + 0.02 * np.sin(y)
+        v = -0.3 * np.tanh((y - 5.6) / 0.7)
+    U = np.zeros((ny, nx, 2 + nd))
+    U[..., 0] = rho
+    U[..., 2] = rho * u
+    if nd == 2:
+        U[..., 3] = rho * v
+    U[..., 1] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    U *= 1.0 + 1e-3 * np.random.default_rng(8).standard_normal(U.shape)
+    return U[0] if nd == 1 else U
+
+
+@pytest.mark.parametrize("nd,bc", [(2, (0, 0)), (2, (1, 2)), (2, (2, 1)), (1, (0,)), (1, (1,))])
+def test_step_vjp_through_flattening_matches_tape(A, E, nd, bc):
+    """Where the flattening factor chi lies strictly between 0 and 1 the
+    adjoint differentiates through it (the factor the minimum selected, its
+    pressure ratio, dp/dU and dp/dgamma of the cells it reads, through ghost
+    cells too): one step's VJP equals the torch tape, whose chi gradient is
+    pinned by finite differences on the CPU."""
+    U = _shock_state(nd)
+    Ut = [torch.tensor(np.ascontiguousarray(np.moveaxis(U, -1, 0)[k])) for k in range(U.shape[-1])]
+    chi = T._chi_min(T.pad(Ut, nd, bc), torch.tensor(1.4, dtype=torch.float64), nd)
+    assert int(((chi > 0) & (chi < 1)).sum()) >= 2     # the case this test is for
+    rng = np.random.default_rng(9)
+    lam = rng.standard_normal(_soa(U).shape)
+    dh = 1.0 / U.shape[-2]
+    adj = A.Adjoint(E.Scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc))), 0.9, dh)
+    lam_g, dg = adj.step_vjp(_dev(U), torch.from_numpy(lam).to("cuda"))
+    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh)
+    got = lam_g.cpu().numpy()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-11 * scale, np.abs(got - ref).max() / scale
+    assert abs(dg - dg_ref) <= 1e-11 * max(abs(dg_ref), scale)
+
+
 def _adjoint_run(A, E, U, target, dh, steps, every, bc=(2, 2)):
     adj = A.Adjoint(E.Scheme(gamma=1.4, bc=tuple(bc) + (0,)), 0.9, dh)
     Ud = _dev(U)
