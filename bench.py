@@ -121,7 +121,8 @@ def main():
     if args.mode == "ghost":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
         print(json.dumps(ghost_fluid_case(torch, N, W, stream, args.steps, args.warmup,
-                                          dtype=torch.float32 if args.dtype == "f32" else None)),
+                                          dtype=torch.float32 if args.dtype == "f32" else None,
+                                          numerics=args.numerics)),
               flush=True)
         return
     n = args.n
@@ -265,7 +266,13 @@ def main():
         variants["hllc_f32"]["vs_f64_exact"] = round(variants["hllc_f32"]["value"] / exact_value, 3)
 
         variants["riemann_1d_f64"] = riemann_1d_case(torch, N, W, stream)
-        variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5)
+        # the two-material step in the headline's FP64 numerics, the other
+        # one beside it
+        other = "exact" if args.numerics == "fast" else "fast"
+        variants["ghost_fluid_f64"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5,
+                                                       numerics=args.numerics)
+        variants[f"ghost_fluid_f64_{other}"] = ghost_fluid_case(torch, N, W, stream, steps=10,
+                                                                warmup=5, numerics=other)
         variants["ghost_fluid_f32"] = ghost_fluid_case(torch, N, W, stream, steps=10, warmup=5,
                                                        dtype=torch.float32)
         variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
@@ -541,7 +548,8 @@ def io_case(torch, N, W, stream, n=4096, reps=20, text_n=512):
     return res
 
 
-def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype=None):
+def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype=None,
+                     numerics="exact"):
     """Two-material ghost-fluid steps on the seeded shock-bubble workload
     (ideal-gas bubble, level sets reinitialised every 5 steps in a 6-cell
     band).  A cell-update is one cell of the domain advanced one full
@@ -550,7 +558,9 @@ def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype
     steps the first reinitialisation falls in the warm-up call, so the timed
     steps are the workload's steady state (level sets clamped outside the
     band, as after every reinitialisation).  dtype float32: FP32 bulk states
-    (FP64 level sets, dt and ghost-fluid arithmetic)."""
+    (FP64 level sets, dt and ghost-fluid arithmetic).  numerics "fast": the
+    FP64 sweeps and the level sets' WENO right-hand side in the fast numerics
+    (within 1e-12 of the exact loop)."""
     from fluidity_amd import euler as E
     from fluidity_amd import levelset as L
     sb = W.shock_bubble(nx, ny, seed=1002)
@@ -563,8 +573,9 @@ def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype
         p0 = torch.from_numpy(sb.phi0).to(dev)
         p1 = torch.from_numpy(sb.phi1).to(dev)
     stream.synchronize()
-    sim.add_material(U0, p0, E.Scheme(gamma=sb.gamma0))
-    sim.add_material(U1, p1, E.Scheme(gamma=sb.gamma1))
+    num = N.FL_NUMERICS_FAST if numerics == "fast" else N.FL_NUMERICS_EXACT
+    sim.add_material(U0, p0, E.Scheme(gamma=sb.gamma0, numerics=num))
+    sim.add_material(U1, p1, E.Scheme(gamma=sb.gamma1, numerics=num))
     kw = dict(band=6, extrap_sweeps=8, reinit_every=5, stream=stream)
     sim.simulate_mm(0.5, sb.dh, max_iters=warmup, **kw)
     torch.cuda.synchronize()
@@ -576,7 +587,7 @@ def ghost_fluid_case(torch, N, W, stream, steps, warmup, nx=2048, ny=1024, dtype
     cells = nx * ny
     return {"value": round(steps * cells / el, 1), "unit": "cell-updates/s",
             "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps,
-            "warmup": warmup,
+            "warmup": warmup, "numerics": numerics,
             "workload": f"shock-bubble {nx}x{ny} {'FP32 bulk' if dtype else 'FP64'}, 2 materials, "
                         f"Mach {sb.mach:.3f}, reinit every 5 steps, band 6, SoA"}
 

@@ -316,30 +316,35 @@ int launch_sweep(const fl_context* ctx, const fl_field* in, const fl_field* out,
 // with scheme s[m].  The two grids share shape, layout and dtype; the flux
 // methods must be equal (the launch instantiates one), else each grid is
 // swept on its own.
-template <typename T, int FLUX, bool VLC>
+template <typename T, int FLUX, bool FAST, bool VLC>
 void launch_flux2_v(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
                     const fl::SweepArgs<T>& a1, hipStream_t stream) {
   if (ndim == 1)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX, VLC>), grid, dim3(B),
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 1, FLUX, FAST, VLC>), grid, dim3(B),
                        fl::x_tile_lds_bytes(B, 3, sizeof(T)), stream, a0, a1);
   else if (dim == 0)
-    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX, VLC>), grid, dim3(B),
+    hipLaunchKernelGGL((fl::sweep_x2_kernel<T, 2, FLUX, FAST, VLC>), grid, dim3(B),
                        fl::x_tile_lds_bytes(B, 4, sizeof(T)), stream, a0, a1);
   else
-    hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX, VLC>), grid, dim3(B),
+    hipLaunchKernelGGL((fl::sweep_y2_kernel<T, FLUX, FAST, VLC>), grid, dim3(B),
                        fl::y_tile_lds_bytes(B, sizeof(T)), stream, a0, a1);
 }
 
+// fast: the fast FP64 numerics (FP64 only; the FP32 kernels have their own)
 template <typename T, int FLUX>
 void launch_flux2(int ndim, int dim, dim3 grid, int B, const fl::SweepArgs<T>& a0,
-                  const fl::SweepArgs<T>& a1, hipStream_t stream) {
-  if constexpr (has_vlc<FLUX>()) {
-    if (is_vlc(a0.limiter, a0.form) && is_vlc(a1.limiter, a1.form)) {
-      launch_flux2_v<T, FLUX, true>(ndim, dim, grid, B, a0, a1, stream);
-      return;
+                  const fl::SweepArgs<T>& a1, bool fast, hipStream_t stream) {
+  const bool vlc = has_vlc<FLUX>() && is_vlc(a0.limiter, a0.form) && is_vlc(a1.limiter, a1.form);
+  if constexpr (std::is_same<T, double>::value) {
+    if (fast) {
+      if constexpr (has_vlc<FLUX>())
+        if (vlc) return launch_flux2_v<T, FLUX, true, true>(ndim, dim, grid, B, a0, a1, stream);
+      return launch_flux2_v<T, FLUX, true, false>(ndim, dim, grid, B, a0, a1, stream);
     }
   }
-  launch_flux2_v<T, FLUX, false>(ndim, dim, grid, B, a0, a1, stream);
+  if constexpr (has_vlc<FLUX>())
+    if (vlc) return launch_flux2_v<T, FLUX, false, true>(ndim, dim, grid, B, a0, a1, stream);
+  launch_flux2_v<T, FLUX, false, false>(ndim, dim, grid, B, a0, a1, stream);
 }
 
 template <typename T>
@@ -370,15 +375,20 @@ int launch_sweep2_t(const fl_context* ctx, const fl_field* const in[2],
                                            f->ndim == 1 ? 1 : rows);
   const fl::SweepArgs<T> a1 = make_args<T>(ctx, in[1], out[1], s[1], dtdh, ctl, nullptr,
                                            f->ndim == 1 ? 1 : rows);
+  const bool fast = fast_numerics(s[0]);
   switch (s[0]->flux) {
-    case FL_FLUX_HLLC: launch_flux2<T, fl::kFluxHllc>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    case FL_FLUX_HLLC:
+      launch_flux2<T, fl::kFluxHllc>(f->ndim, dim, grid, B, a0, a1, fast, stream);
+      break;
     case FL_FLUX_FORCE:
-      launch_flux2<T, fl::kFluxForce>(f->ndim, dim, grid, B, a0, a1, stream);
+      launch_flux2<T, fl::kFluxForce>(f->ndim, dim, grid, B, a0, a1, fast, stream);
       break;
     case FL_FLUX_LAX_FRIEDRICHS:
-      launch_flux2<T, fl::kFluxLaxFriedrichs>(f->ndim, dim, grid, B, a0, a1, stream);
+      launch_flux2<T, fl::kFluxLaxFriedrichs>(f->ndim, dim, grid, B, a0, a1, fast, stream);
       break;
-    default: launch_flux2<T, fl::kFluxRichtmyer>(f->ndim, dim, grid, B, a0, a1, stream); break;
+    default:
+      launch_flux2<T, fl::kFluxRichtmyer>(f->ndim, dim, grid, B, a0, a1, fast, stream);
+      break;
   }
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -387,9 +397,9 @@ int launch_sweep2_t(const fl_context* ctx, const fl_field* const in[2],
 int launch_sweep2(const fl_context* ctx, const fl_field* const in[2], const fl_field* const out[2],
                   int dim, const fl_scheme* const s[2], double dtdh, hipStream_t stream,
                   const fl::StepCtl* ctl) {
-  // the two-grid kernels run the exact FP64 numerics (FP32 its own forms)
-  if (s[0]->flux != s[1]->flux || (in[0]->dtype == FL_F64 && (fast_numerics(s[0]) ||
-                                                              fast_numerics(s[1])))) {
+  // one launch instantiates one flux method and one form of the numerics
+  if (s[0]->flux != s[1]->flux ||
+      (in[0]->dtype == FL_F64 && fast_numerics(s[0]) != fast_numerics(s[1]))) {
     for (int m = 0; m < 2; ++m) {
       const int rc = launch_sweep(ctx, in[m], out[m], dim, s[m], dtdh, ctl, nullptr, stream);
       if (rc) return rc;

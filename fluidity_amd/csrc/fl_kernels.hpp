@@ -84,9 +84,9 @@ constexpr int kMaxWsUnroll = 4;
 namespace fl {
 template <typename T, int ND, int FLUX, bool FAST, bool VLC> __global__ void sweep_x_kernel(SweepArgs<T> a);
 template <typename T, int FLUX, bool FAST, bool VLC> __global__ void sweep_y_kernel(SweepArgs<T> a);
-template <typename T, int ND, int FLUX, bool VLC>
+template <typename T, int ND, int FLUX, bool FAST, bool VLC>
 __global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
-template <typename T, int FLUX, bool VLC>
+template <typename T, int FLUX, bool FAST, bool VLC>
 __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int ND>
 __global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es0, int64_t cs0,

@@ -23,6 +23,7 @@
 // kernel is a deterministic function of its inputs, so results do not depend
 // on launch geometry or scheduling.
 #include "fl_levelset.hpp"
+#include "fl_numerics_fast.hpp"
 
 namespace fl {
 namespace ls {
@@ -225,6 +226,9 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 // HJWeno5::diff (scheme/stencils/weno_hj_5.hpp:97-143), std::pow(x, 2) as x*x.
+// FAST: the fast FP64 numerics (four refined reciprocals instead of the
+// correctly rounded quotients).
+template <bool FAST>
 __device__ __forceinline__ double weno_diff(double d1, double d2, double d3, double d4,
                                             double d5) {
   const double phi_1 = 2.0 * d1 - 7.0 * d2 + 11.0 * d3;
@@ -241,6 +245,15 @@ __device__ __forceinline__ double weno_diff(double d1, double d2, double d3, dou
   const double e =
       10.0e-6 * smax(smax(smax(smax(d1 * d1, d2 * d2), d3 * d3), d4 * d4), d5 * d5) + 10.0e-99;
   const double a1e = alpha_1 + e, a2e = alpha_2 + e, a3e = alpha_3 + e;
+  if (FAST) {
+    // the quotients as products with refined reciprocals; a*e*^2 >= 1e-196
+    // and the weight sum <= 6e196 keep every reciprocal in the normal range
+    alpha_1 = 0.1 * fast::rcp(a1e * a1e);
+    alpha_2 = 0.6 * fast::rcp(a2e * a2e);
+    alpha_3 = 0.3 * fast::rcp(a3e * a3e);
+    const double w_denom = (alpha_1 + alpha_2 + alpha_3) * 6.0;
+    return (alpha_1 * phi_1 + alpha_2 * phi_2 + alpha_3 * phi_3) * fast::rcp(w_denom);
+  }
   alpha_1 = 0.1 / (a1e * a1e);
   alpha_2 = 0.6 / (a2e * a2e);
   alpha_3 = 0.3 / (a3e * a3e);
@@ -260,7 +273,7 @@ __device__ __forceinline__ int mirror32(int i, int n) {
 // g.nd; the 7-cell stencils of all dimensions are loaded before any is used.
 // xin / yin: every cell of the stencil in x / y lies inside the grid (the
 // caller's workgroup-uniform test), so the mirror index math is skipped.
-template <int ND>
+template <int ND, bool FAST>
 __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, int y, double v,
                          double h1, bool xin, bool yin) {
   double result = 0.0;
@@ -300,9 +313,9 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
     const double* f = fs[d];
-    const double bd = weno_diff(h1 * (f[1] - f[0]), h1 * (f[2] - f[1]), h1 * (f[3] - f[2]),
+    const double bd = weno_diff<FAST>(h1 * (f[1] - f[0]), h1 * (f[2] - f[1]), h1 * (f[3] - f[2]),
                                 h1 * (f[4] - f[3]), h1 * (f[5] - f[4]));
-    const double fd = weno_diff(h1 * (f[6] - f[5]), h1 * (f[5] - f[4]), h1 * (f[4] - f[3]),
+    const double fd = weno_diff<FAST>(h1 * (f[6] - f[5]), h1 * (f[5] - f[4]), h1 * (f[4] - f[3]),
                                 h1 * (f[3] - f[2]), h1 * (f[2] - f[1]));
     double back, fwrd;
     if (v <= 0.0) {
@@ -314,7 +327,7 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
     }
     result += smax(back * back, fwrd * fwrd);
   }
-  return v * sqrt(result);
+  return v * (FAST ? fast::sqt(result) : sqrt(result));
 }
 
 // One RK3 stage (scheme/updaters/runge_kutta_3.hpp:117-148), ND = g.nd:
@@ -323,7 +336,7 @@ __device__ double ls_rhs(const double* __restrict__ phi, const Grid& g, int x, i
 //  stage 2: out = (1/3) phi_n + (2 (1/3)) (s2 - dt L(s2))
 // Launched on a (ceil(nx / blockDim.x), min(ny, 65535)) grid; rows stride by
 // gridDim.y.
-template <int ND>
+template <int ND, bool FAST>
 __device__ __forceinline__ void rk3_stage_body(int stage, const double* __restrict__ phi_n,
                                                const double* __restrict__ src,
                                                const double* __restrict__ V, const Grid& g,
@@ -335,7 +348,7 @@ __device__ __forceinline__ void rk3_stage_body(int stage, const double* __restri
   for (int y = blockIdx.y; y < g.ny; y += gridDim.y) {
     const int64_t i = (int64_t)y * g.nx + x;
     const bool yin = y >= 3 && y + 3 < g.ny;
-    const double l = ls_rhs<ND>(src, g, x, y, V[i], h1, xin, yin);
+    const double l = ls_rhs<ND, FAST>(src, g, x, y, V[i], h1, xin, yin);
     if (stage == 0) {
       out[i] = phi_n[i] - dt * l;
     } else if (stage == 1) {
@@ -357,16 +370,16 @@ __global__ void rk3_stage_kernel(int stage, const double* __restrict__ phi_n,
                                  Grid g, double dt, const double* dt_dev, double h1,
                                  double* __restrict__ out) {
   if (dt_dev) dt = *dt_dev;
-  rk3_stage_body<ND>(stage, phi_n, src, V, g, dt, h1, out);
+  rk3_stage_body<ND, false>(stage, phi_n, src, V, g, dt, h1, out);
 }
 
 // The same stage of two level sets' advection in one launch (blockIdx.z
-// selects the job), dt in device memory.
-template <int ND>
+// selects the job), dt in device memory; FAST: the fast FP64 numerics.
+template <int ND, bool FAST>
 __global__ void rk3_stage2_kernel(int stage, Rk3Job j0, Rk3Job j1, Grid g,
                                   const double* dt_dev, double h1) {
   const Rk3Job j = blockIdx.z ? j1 : j0;
-  rk3_stage_body<ND>(stage, j.phi_n, j.src, j.V, g, *dt_dev, h1, j.out);
+  rk3_stage_body<ND, FAST>(stage, j.phi_n, j.src, j.V, g, *dt_dev, h1, j.out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1079,8 +1092,10 @@ __global__ void combine_kernel(const double* __restrict__ phi0, StateView u0, St
                                                StateView, int, Grid, double, int, double*); \
   template __global__ void velocity2_kernel<ND>(const double*, const double*, StateView,     \
                                                 StateView, Grid, double, double*, double*); \
-  template __global__ void rk3_stage2_kernel<ND>(int, Rk3Job, Rk3Job, Grid, const double*,    \
-                                                 double);                                     \
+  template __global__ void rk3_stage2_kernel<ND, false>(int, Rk3Job, Rk3Job, Grid,            \
+                                                        const double*, double);              \
+  template __global__ void rk3_stage2_kernel<ND, true>(int, Rk3Job, Rk3Job, Grid,             \
+                                                       const double*, double);               \
   template __global__ void rk3_stage_kernel<ND>(int, const double*, const double*,            \
                                                 const double*, Grid, double, const double*,  \
                                                 double, double*);                            \

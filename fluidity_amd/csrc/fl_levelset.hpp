@@ -96,7 +96,7 @@ struct Rk3Job {
   const double* V;
   double* out;
 };
-template <int ND>
+template <int ND, bool FAST>
 __global__ void rk3_stage2_kernel(int stage, Rk3Job j0, Rk3Job j1, Grid g, const double* dt_dev,
                                   double h1);
 template <int ND>

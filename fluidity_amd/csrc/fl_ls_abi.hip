@@ -130,15 +130,18 @@ int launch_advect(const double* phi, const double* V, const ls::Grid& g, double 
 
 // RK3 of both level sets, each stage of both in one launch: cur[m] -> out[m]
 // with speeds V[m], dt in device memory; work holds 4 cells of scratch.
+// fast: the fast FP64 numerics in the WENO derivatives.
 int launch_advect2(double* const cur[2], double* const V[2], const ls::Grid& g, double dh,
-                   double* const out[2], double* work, const double* dt_dev, hipStream_t st) {
+                   double* const out[2], double* work, const double* dt_dev, bool fast,
+                   hipStream_t st) {
   const int64_t cells = g.nx * g.ny;
   double* s1[2] = {work, work + cells};
   double* s2[2] = {work + 2 * cells, work + 3 * cells};
   const dim3 grid((unsigned)((g.nx + kBlock - 1) / kBlock),
                   (unsigned)(g.ny < 65535 ? g.ny : 65535), 2);
   const double h1 = 1.0 / dh;
-  auto k = g.nd == 1 ? ls::rk3_stage2_kernel<1> : ls::rk3_stage2_kernel<2>;
+  auto k = g.nd == 1 ? (fast ? ls::rk3_stage2_kernel<1, true> : ls::rk3_stage2_kernel<1, false>)
+                     : (fast ? ls::rk3_stage2_kernel<2, true> : ls::rk3_stage2_kernel<2, false>);
   ls::Rk3Job j[3][2];
   for (int m = 0; m < 2; ++m) {
     j[0][m] = {cur[m], cur[m], V[m], s1[m]};
@@ -460,6 +463,7 @@ int mm_simulate_n(fl_context* c, fl_material* mats, int nmat, const fl_sim_param
   const int S = 2 + g.nd;
   const int64_t esize = mats[0].state.dtype == FL_F64 ? 8 : 4;
   const double dh = p->dh;
+  const bool fast = mats[0].scheme.numerics == FL_NUMERICS_FAST;  // one for every material
   double t = 0.0, dt = 0.0, maxws = 0.0;
   int64_t it = 0;
   int rc;
@@ -559,7 +563,7 @@ int mm_simulate_n(fl_context* c, fl_material* mats, int nmat, const fl_sim_param
         double* const c2[2] = {cur[m], cur[m + 1]};
         double* const v2[2] = {w.V[m], w.V[m + 1]};
         double* const o2[2] = {nxt[m], nxt[m + 1]};
-        if ((rc = launch_advect2(c2, v2, g, dh, o2, w.adv, &ctl->dt, st))) return rc;
+        if ((rc = launch_advect2(c2, v2, g, dh, o2, w.adv, &ctl->dt, fast, st))) return rc;
       } else if ((rc = launch_advect(cur[m], w.V[m], g, 0.0, dh, nxt[m], w.adv, st, &ctl->dt))) {
         return rc;
       }
@@ -760,7 +764,8 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
       return fail(FL_ERR_ARGUMENT, "mats[%d]: every state and work grid needs one dtype", m);
     if (!mats[m].phi) return fail(FL_ERR_ARGUMENT, "mats[%d].phi is NULL", m);
     if ((rc = check_scheme(&mats[m].scheme, g.nd))) return rc;
-    if ((rc = require_exact(&mats[m].scheme, "the two-material step"))) return rc;
+    if (mats[m].scheme.numerics != mats[0].scheme.numerics)
+      return fail(FL_ERR_ARGUMENT, "mats[%d]: every material needs the same numerics", m);
   }
   if (!(p->cfl > 0.0) || !isfinite(p->cfl)) return fail(FL_ERR_ARGUMENT, "cfl must be > 0");
   if ((rc = check_dh(p->dh))) return rc;
@@ -784,6 +789,7 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
   const int S = 2 + g.nd;
   const int64_t esize = mats[0].state.dtype == FL_F64 ? 8 : 4;
   const double dh = p->dh;
+  const bool fast = mats[0].scheme.numerics == FL_NUMERICS_FAST;  // one for every material
   double t = 0.0, dt = 0.0, maxws = 0.0;
   int64_t it = 0;
   // Each level set lives in the caller's phi or in the advection buffer tphi:
@@ -860,7 +866,7 @@ int fl_mm_simulate(fl_context* c, fl_material* mats, int nmat, const fl_sim_para
     // pass), advection with the device dt, projection, reinitialisation every
     // `every` steps
     if ((rc = launch_velocity2(cur, mats[0].state, mats[1].state, g, dh, w.V, st))) return rc;
-    if ((rc = launch_advect2(cur, w.V, g, dh, nxt, w.adv, &ctl->dt, st))) return rc;
+    if ((rc = launch_advect2(cur, w.V, g, dh, nxt, w.adv, &ctl->dt, fast, st))) return rc;
     if ((rc = launch_project(nxt[0], nxt[1], cells, st))) return rc;
     const bool reinit = (it % every) == every - 1;
     if (reinit) {  // both level sets in one solve, written back into cur[m]

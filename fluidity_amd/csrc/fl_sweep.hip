@@ -48,20 +48,20 @@ sweep_y_kernel(SweepArgs<T> a) {
 
 // The same sweeps over two grids of one shape in one launch (the two
 // materials of a two-material step): blockIdx.z selects the grid.
-template <typename T, int ND, int FLUX, bool VLC>
+template <typename T, int ND, int FLUX, bool FAST, bool VLC>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_x_tile<T, ND, FLUX, false, VLC>(blockIdx.z ? a1 : a0, bx, by);
+  sweep_x_tile<T, ND, FLUX, FAST, VLC>(blockIdx.z ? a1 : a0, bx, by);
 }
 
-template <typename T, int FLUX, bool VLC>
+template <typename T, int FLUX, bool FAST, bool VLC>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1) {
   int64_t bx, by;
   tile_of_block(bx, by);
-  sweep_y_tile<T, FLUX, false, VLC>(blockIdx.z ? a1 : a0, bx, by);
+  sweep_y_tile<T, FLUX, FAST, VLC>(blockIdx.z ? a1 : a0, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,11 +380,11 @@ __global__ void __launch_bounds__(kMaxBlock) component_sums_kernel(const T* __re
 // explicit instantiations (launched from fl_abi.cpp)
 // ---------------------------------------------------------------------------
 
-#define FL_INST_SWEEPS2_V(T, FLUX, VLC)                                               \
-  template __global__ void sweep_x2_kernel<T, 1, FLUX, VLC>(SweepArgs<T>, SweepArgs<T>); \
-  template __global__ void sweep_x2_kernel<T, 2, FLUX, VLC>(SweepArgs<T>, SweepArgs<T>); \
-  template __global__ void sweep_y2_kernel<T, FLUX, VLC>(SweepArgs<T>, SweepArgs<T>);
-#define FL_INST_SWEEPS2(T, FLUX) FL_INST_SWEEPS2_V(T, FLUX, false)
+#define FL_INST_SWEEPS2_V(T, FLUX, FAST, VLC)                                               \
+  template __global__ void sweep_x2_kernel<T, 1, FLUX, FAST, VLC>(SweepArgs<T>, SweepArgs<T>); \
+  template __global__ void sweep_x2_kernel<T, 2, FLUX, FAST, VLC>(SweepArgs<T>, SweepArgs<T>); \
+  template __global__ void sweep_y2_kernel<T, FLUX, FAST, VLC>(SweepArgs<T>, SweepArgs<T>);
+#define FL_INST_SWEEPS2(T, FLUX) FL_INST_SWEEPS2_V(T, FLUX, false, false)
 
 FL_INST_SWEEPS2(double, kFluxHllc)
 FL_INST_SWEEPS2(double, kFluxForce)
@@ -394,10 +394,17 @@ FL_INST_SWEEPS2(float, kFluxHllc)
 FL_INST_SWEEPS2(float, kFluxForce)
 FL_INST_SWEEPS2(float, kFluxLaxFriedrichs)
 FL_INST_SWEEPS2(float, kFluxRichtmyer)
-FL_INST_SWEEPS2_V(double, kFluxHllc, true)
-FL_INST_SWEEPS2_V(double, kFluxForce, true)
-FL_INST_SWEEPS2_V(float, kFluxHllc, true)
-FL_INST_SWEEPS2_V(float, kFluxForce, true)
+FL_INST_SWEEPS2_V(double, kFluxHllc, false, true)
+FL_INST_SWEEPS2_V(double, kFluxForce, false, true)
+FL_INST_SWEEPS2_V(float, kFluxHllc, false, true)
+FL_INST_SWEEPS2_V(float, kFluxForce, false, true)
+// the fast FP64 numerics
+FL_INST_SWEEPS2_V(double, kFluxHllc, true, false)
+FL_INST_SWEEPS2_V(double, kFluxForce, true, false)
+FL_INST_SWEEPS2_V(double, kFluxLaxFriedrichs, true, false)
+FL_INST_SWEEPS2_V(double, kFluxRichtmyer, true, false)
+FL_INST_SWEEPS2_V(double, kFluxHllc, true, true)
+FL_INST_SWEEPS2_V(double, kFluxForce, true, true)
 
 #define FL_INST_MAXWS2(T)                                                                 \
   template __global__ void max_wavespeed2_kernel<T, 1>(const T*, const T*, int64_t, int64_t, \

@@ -4,6 +4,8 @@ for bit.  Inputs are the seeded shock-bubble workload at reduced and full
 size, and random fields for the single kernels."""
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 import pytest
 
@@ -198,8 +200,9 @@ def test_combine(L, oracle):
 
 # ---- the two-material step ---------------------------------------------------------
 
-def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8, every=1, dtype=None):
+def _run_mm(L, E, sb, steps, layout=0, band=6, sweeps=8, every=1, dtype=None, numerics=0):
     es0, es1 = _schemes_e(E, sb)
+    es0.numerics = es1.numerics = numerics
     sim = L.MultimaterialSimulator(layout=layout)
     U0, U1 = _dev(sb.U0, layout), _dev(sb.U1, layout)
     if dtype is not None:
@@ -419,6 +422,44 @@ def test_mm_simulate_fp32_bulk(L, E, nx, ny, steps):
     assert np.isfinite(got[1]).all() and np.isfinite(got[2]).all()
     assert abs(got[0].run_time - ref[0].run_time) <= 1e-5 * ref[0].run_time
     assert worst <= 1e-5, worst
+
+
+@pytest.mark.parametrize("nx,ny,steps,stiff", [(256, 128, 12, False), (256, 128, 12, True),
+                                               (2048, 1024, 20, False)])
+def test_mm_simulate_fast_numerics(L, E, nx, ny, steps, stiff):
+    """FL_NUMERICS_FAST on the two-material loop (both materials' sweeps in
+    their fast forms, refined reciprocals and square root in the level sets'
+    WENO-HJ5 / Godunov right-hand side; star states, extrapolation, dt and the
+    eikonal solve keep the exact numerics) against the exact loop, which
+    equals the oracle bit for bit (test_mm_simulate_bitwise,
+    test_mm_simulate_full_size): every conserved field of both materials and
+    both level sets within 1e-12 relative L-inf, the same step count, the run
+    time within 1e-12 and no level-set sign change.  Relative L-inf is
+    max |diff| / max |exact| over both materials' grids, for the momentum
+    components max |diff| / the largest momentum magnitude: the bubble is at
+    rest until the shock reaches it and the planar shock has no y momentum,
+    so those fields hold only rounding-level values of either sign.
+    Reinitialisation every 5 steps, as the shock-bubble workload."""
+    sb = _bubble(nx, ny, stiff)
+    ref = _run_mm(L, E, sb, steps, layout=1, every=5)
+    got = _run_mm(L, E, sb, steps, layout=1, every=5, numerics=1)
+    assert got[0].iters == ref[0].iters == steps
+    assert abs(got[0].run_time - ref[0].run_time) <= 1e-12 * ref[0].run_time
+    rs = [np.asarray(ref[1]), np.asarray(ref[2])]
+    mom = max(float(np.hypot(r[..., 2], r[..., 3]).max()) for r in rs)
+    scale = [max(float(np.abs(r[..., 0]).max()) for r in rs),
+             max(float(np.abs(r[..., 1]).max()) for r in rs), mom, mom]
+    errs = {}
+    for m in (1, 2):
+        g, r = np.asarray(got[m]), rs[m - 1]
+        for k, name in enumerate(["rho", "E", "m_x", "m_y"]):
+            errs[f"U{m - 1} {name}"] = float(np.abs(g[..., k] - r[..., k]).max() / scale[k])
+    for m in (3, 4):
+        g, r = np.asarray(got[m]), np.asarray(ref[m])
+        errs[f"phi{m - 3}"] = float(np.abs(g - r).max() / np.abs(r).max())
+        assert (np.sign(g) == np.sign(r)).all(), f"phi{m - 3} changes sign"
+    print(f"fast {nx}x{ny} stiff={stiff}: rel Linf {errs}", file=sys.stderr)
+    assert max(errs.values()) <= 1e-12, errs
 
 
 @pytest.mark.parametrize("layout,every", [(0, 1), (1, 5)])
