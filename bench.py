@@ -112,7 +112,8 @@ def main():
         return
     if args.mode == "batch":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
-        print(json.dumps(batched_case(torch, N, W, stream, args.steps)), flush=True)
+        print(json.dumps(batched_case(torch, N, W, stream, args.steps, numerics=args.numerics)),
+              flush=True)
         return
     if args.mode == "io":
         stream = torch.cuda.Stream(device=torch.device("cuda:0"))
@@ -277,7 +278,7 @@ def main():
                                                        dtype=torch.float32)
         variants["adjoint_f64"] = adjoint_case(torch, N, W, stream)
         variants["io_f64"] = io_case(torch, N, W, stream)
-        bf = batched_case(torch, N, W, stream, steps=20)
+        bf = batched_case(torch, N, W, stream, steps=20, numerics=args.numerics)
         for k, v in bf.items():
             # against the headline FP64 step and against the bit-exact one
             v["vs_f64_hllc"] = round(v["value"] / value, 3)
@@ -444,19 +445,26 @@ def adjoint_case(torch, N, W, stream, steps=200, every=20, n=1024):
                         "HLLC + van Leer, CFL 0.9, FP64"}
 
 
-def batched_case(torch, N, W, stream, steps):
+def batched_case(torch, N, W, stream, steps, numerics="exact"):
     """The 512-problem batch (seed 1003: 384 1D problems of 256..65536 cells,
-    128 2D problems of 64^2..1024^2, per-problem gamma and dt), FP64 and FP32
+    128 2D problems of 64^2..1024^2, per-problem gamma and dt), FP64 in the
+    given numerics (batched_f64), FP64 in the other one beside it and FP32
     bulk, tile shapes from the autotuner, steps replayed from a hipGraph.  A
-    cell-update is one cell of one problem advanced one step.  The FP32 line
-    carries the per-problem relative L1 density error against the FP64 batch,
-    which equals the oracle run on each problem alone bit for bit."""
+    cell-update is one cell of one problem advanced one step.  The FP32 and
+    fast FP64 lines carry their per-problem density error against the exact
+    FP64 batch, which equals the oracle run on each problem alone bit for
+    bit."""
     from fluidity_amd import batch as BT
     specs = W.batched(seed=1003)
     cells = sum(int(np.prod(sp.n)) for sp in specs)
+    # the exact FP64 batch first: the reference of the others
+    runs = [("batched_f64" if num == numerics else f"batched_f64_{num}", torch.float64, num)
+            for num in ("exact", "fast")] + [("batched_f32", torch.float32, "exact")]
     out, ref = {}, None
-    for name, dt in (("batched_f64", torch.float64), ("batched_f32", torch.float32)):
-        probs = BT.problems_from_workload(specs, dt, N.FL_LAYOUT_SOA, max_iters=steps + 2)
+    for name, dt, num in runs:
+        probs = BT.problems_from_workload(
+            specs, dt, N.FL_LAYOUT_SOA, max_iters=steps + 2,
+            numerics=N.FL_NUMERICS_FAST if num == "fast" else N.FL_NUMERICS_EXACT)
         b = BT.Batch(probs, stream=stream)
         t = b.autotune(reps=2, stream=stream)
         b.step(2, graph=True, stream=stream)          # warm-up (captures the graph)
@@ -474,14 +482,18 @@ def batched_case(torch, N, W, stream, steps):
                 "ms_per_step": round(el * 1e3 / steps, 4), "problems": len(specs),
                 "failed_problems": ok.count(False), "cells": cells,
                 "tuning": [t.x_block, t.x_rows, t.y_block, t.y_rows]}
+        if dt == torch.float64:
+            line["numerics"] = num
         if ref is None:
             ref = [(b.state(i)[0].clone(), ok[i]) for i in range(len(specs))]
         else:
-            errs = [float((b.state(i)[0].double() - ref[i][0]).abs().sum() /
-                          ref[i][0].abs().sum())
-                    for i in range(len(specs)) if ref[i][1] and ok[i]]
+            pairs = [(b.state(i)[0].double(), ref[i][0])
+                     for i in range(len(specs)) if ref[i][1] and ok[i]]
+            errs = [float((d - r).abs().sum() / r.abs().sum()) for d, r in pairs]
             line["density_rel_l1_max"] = float("%.3g" % max(errs))
             line["density_rel_l1_median"] = float("%.3g" % float(np.median(errs)))
+            line["density_rel_linf_max"] = float("%.3g" % max(
+                float((d - r).abs().max() / r.abs().max()) for d, r in pairs))
             line["error_problems"] = len(errs)
         out[name] = line
         del b, probs

@@ -97,14 +97,16 @@ class Batch:
 
 def problems_from_workload(specs, dtype=torch.float64, layout: int = N.FL_LAYOUT_SOA,
                            cfl: float = 0.9, max_iters: int = 1 << 62, flux: int = N.FL_FLUX_HLLC,
-                           device: str = "cuda"):
-    """Device problems from workloads.batched() specs (AoS float64 arrays)."""
+                           device: str = "cuda", numerics: int = N.FL_NUMERICS_EXACT):
+    """Device problems from workloads.batched() specs (AoS float64 arrays);
+    numerics: the FP64 numerics of every problem (one per FP64 batch)."""
     out = []
     for sp in specs:
         U = sp.U
         if layout == N.FL_LAYOUT_SOA:
             U = np.ascontiguousarray(np.moveaxis(U, -1, 0))
         T = torch.from_numpy(U).to(device=device, dtype=dtype)
-        out.append({"U": T, "scheme": Scheme(flux=flux, gamma=sp.gamma), "cfl": cfl,
+        out.append({"U": T, "scheme": Scheme(flux=flux, gamma=sp.gamma, numerics=numerics),
+                    "cfl": cfl,
                     "dh": sp.dh, "max_iters": max_iters})
     return out

@@ -66,9 +66,10 @@ __global__ void reset_counters_kernel(Counters* cnt) {
 // Persistent sweep phase.  timing != 0 runs every tile state -> work with a
 // fixed dt/dh and no wavespeed output (launch-shape measurement; the states
 // are not modified).
-// VLC: every problem limits with van Leer in conservative form (fixed at
-// compile time, as in the single-grid sweeps).
-template <typename T, int FLUX, int PHASE, bool VLC>
+// FAST: the fast FP64 numerics (FP32 has its own forms); VLC: every problem
+// limits with van Leer in conservative form (fixed at compile time, as in
+// the single-grid sweeps).
+template <typename T, int FLUX, int PHASE, bool FAST, bool VLC>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(SweepWaves<T>::value)))
 phase_kernel(const Prob<T>* __restrict__ probs, const Tile* __restrict__ tiles, int ntiles,
              Counters* __restrict__ cnt, int timing, T timing_dtdh) {
@@ -97,10 +98,10 @@ phase_kernel(const Prob<T>* __restrict__ probs, const Tile* __restrict__ tiles, 
           }
           a.ws_out = &P.ctl->ws_bits[it & 1];
         }
-        sweep_x_tile<T, 1, FLUX, false, VLC>(a, td.bx, 0);
+        sweep_x_tile<T, 1, FLUX, FAST, VLC>(a, td.bx, 0);
       } else {
         a.ws_out = nullptr;
-        sweep_x_tile<T, 2, FLUX, false, VLC>(a, td.bx, td.by);
+        sweep_x_tile<T, 2, FLUX, FAST, VLC>(a, td.bx, td.by);
       }
     } else {
       a.rows_per_block = P.rows_y;
@@ -109,7 +110,7 @@ phase_kernel(const Prob<T>* __restrict__ probs, const Tile* __restrict__ tiles, 
         a.out = const_cast<T*>(P.x.in);
         a.ws_out = &P.ctl->ws_bits[P.ctl->iters & 1];
       }
-      sweep_y_tile<T, FLUX, false, VLC>(a, td.bx, td.by);
+      sweep_y_tile<T, FLUX, FAST, VLC>(a, td.bx, td.by);
     }
     __syncthreads();  // the next tile reuses the LDS
   }
@@ -162,23 +163,26 @@ __global__ void __launch_bounds__(256) sums_kernel(const Prob<T>* __restrict__ p
   }
 }
 
-#define FL_BATCH_PHASES(T, VLC)                                                                 \
-  template __global__ void phase_kernel<T, kFluxHllc, 0, VLC>(const Prob<T>*, const Tile*, int,  \
-                                                              Counters*, int, T);                \
-  template __global__ void phase_kernel<T, kFluxHllc, 1, VLC>(const Prob<T>*, const Tile*, int,  \
-                                                              Counters*, int, T);                \
-  template __global__ void phase_kernel<T, kFluxForce, 0, VLC>(const Prob<T>*, const Tile*, int, \
-                                                               Counters*, int, T);               \
-  template __global__ void phase_kernel<T, kFluxForce, 1, VLC>(const Prob<T>*, const Tile*, int, \
-                                                               Counters*, int, T);
+#define FL_BATCH_PHASES(T, FAST, VLC)                                                    \
+  template __global__ void phase_kernel<T, kFluxHllc, 0, FAST, VLC>(                        \
+      const Prob<T>*, const Tile*, int, Counters*, int, T);                                  \
+  template __global__ void phase_kernel<T, kFluxHllc, 1, FAST, VLC>(                        \
+      const Prob<T>*, const Tile*, int, Counters*, int, T);                                  \
+  template __global__ void phase_kernel<T, kFluxForce, 0, FAST, VLC>(                       \
+      const Prob<T>*, const Tile*, int, Counters*, int, T);                                  \
+  template __global__ void phase_kernel<T, kFluxForce, 1, FAST, VLC>(                       \
+      const Prob<T>*, const Tile*, int, Counters*, int, T);
 #define FL_BATCH_INST(T)                                                                     \
-  FL_BATCH_PHASES(T, false)                                                                  \
-  FL_BATCH_PHASES(T, true)                                                                   \
+  FL_BATCH_PHASES(T, false, false)                                                           \
+  FL_BATCH_PHASES(T, false, true)                                                            \
   template __global__ void finish_kernel<T>(const Prob<T>*, int, int);                        \
   template __global__ void sums_kernel<T>(const Prob<T>*, int, double*);
 
 FL_BATCH_INST(double)
 FL_BATCH_INST(float)
+// the fast FP64 numerics
+FL_BATCH_PHASES(double, true, false)
+FL_BATCH_PHASES(double, true, true)
 
 }  // namespace batch
 }  // namespace fl

@@ -46,7 +46,7 @@ struct Counters {
 __global__ void init_kernel(StepCtl* ctls, int nprob);
 __global__ void control_kernel(StepCtl* ctls, const Params* params, int nprob, Counters* cnt);
 __global__ void reset_counters_kernel(Counters* cnt);
-template <typename T, int FLUX, int PHASE, bool VLC>
+template <typename T, int FLUX, int PHASE, bool FAST, bool VLC>
 __global__ void phase_kernel(const Prob<T>* probs, const Tile* tiles, int ntiles, Counters* cnt,
                              int timing, T timing_dtdh);
 template <typename T>

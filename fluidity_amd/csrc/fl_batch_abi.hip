@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "fl_abi_internal.hpp"
@@ -38,6 +39,7 @@ struct fl_batch {
   hipGraphExec_t exec;
   bool graph_ok;
   bool vlc;                  // every problem: van Leer, conservative form
+  bool fast;                 // FP64 batch in the fast numerics
 };
 
 namespace {
@@ -127,24 +129,28 @@ std::vector<bt::Prob<T>> make_probs(const fl_batch* b, const fl_tuning& t) {
   return v;
 }
 
-template <typename T, int FLUX, bool VLC>
+template <typename T, int FLUX, bool FAST, bool VLC>
 const void* phase_fn(int phase) {
-  return phase ? reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 1, VLC>)
-               : reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 0, VLC>);
+  return phase ? reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 1, FAST, VLC>)
+               : reinterpret_cast<const void*>(&bt::phase_kernel<T, FLUX, 0, FAST, VLC>);
 }
 
-template <typename T, bool VLC>
+template <typename T, bool FAST, bool VLC>
 const void* phase_fn_f(int flux, int phase) {
-  return flux == FL_FLUX_HLLC ? phase_fn<T, fl::kFluxHllc, VLC>(phase)
-                              : phase_fn<T, fl::kFluxForce, VLC>(phase);
+  return flux == FL_FLUX_HLLC ? phase_fn<T, fl::kFluxHllc, FAST, VLC>(phase)
+                              : phase_fn<T, fl::kFluxForce, FAST, VLC>(phase);
 }
 
 const void* phase_kernel_ptr(const fl_batch* b, int phase) {
-  if (b->dtype == FL_F64)
-    return b->vlc ? phase_fn_f<double, true>(b->flux, phase)
-                  : phase_fn_f<double, false>(b->flux, phase);
-  return b->vlc ? phase_fn_f<float, true>(b->flux, phase)
-                : phase_fn_f<float, false>(b->flux, phase);
+  if (b->dtype == FL_F64) {
+    if (b->fast)
+      return b->vlc ? phase_fn_f<double, true, true>(b->flux, phase)
+                    : phase_fn_f<double, true, false>(b->flux, phase);
+    return b->vlc ? phase_fn_f<double, false, true>(b->flux, phase)
+                  : phase_fn_f<double, false, false>(b->flux, phase);
+  }
+  return b->vlc ? phase_fn_f<float, false, true>(b->flux, phase)
+                : phase_fn_f<float, false, false>(b->flux, phase);
 }
 
 // Uploads the problem table and tile lists of tuning t; sizes the persistent
@@ -186,17 +192,17 @@ int apply_tuning(fl_batch* b, const fl_tuning& t) {
   return FL_OK;
 }
 
-template <typename T, int FLUX, bool VLC>
+template <typename T, int FLUX, bool FAST, bool VLC>
 void launch_phases_v(const fl_batch* b, hipStream_t st, int timing, double timing_dtdh,
                      int only_phase) {
   const bt::Prob<T>* probs = (const bt::Prob<T>*)b->probs;
   if (only_phase != 1 && b->ntiles[0] > 0)
-    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 0, VLC>), dim3(b->grid[0]),
+    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 0, FAST, VLC>), dim3(b->grid[0]),
                        dim3(b->tuning.x_block), fl::x_tile_lds_bytes(b->tuning.x_block, 4, sizeof(T)),
                        st, probs, (const bt::Tile*)b->tiles[0],
                        b->ntiles[0], b->cnt, timing, (T)timing_dtdh);
   if (only_phase != 0 && b->ntiles[1] > 0)
-    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1, VLC>), dim3(b->grid[1]),
+    hipLaunchKernelGGL((bt::phase_kernel<T, FLUX, 1, FAST, VLC>), dim3(b->grid[1]),
                        dim3(b->tuning.y_block), fl::y_tile_lds_bytes(b->tuning.y_block, sizeof(T)),
                        st, probs, (const bt::Tile*)b->tiles[1],
                        b->ntiles[1], b->cnt, timing, (T)timing_dtdh);
@@ -205,10 +211,19 @@ void launch_phases_v(const fl_batch* b, hipStream_t st, int timing, double timin
 template <typename T, int FLUX>
 void launch_phases_t(const fl_batch* b, hipStream_t st, int timing, double timing_dtdh,
                      int only_phase) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (b->fast) {
+      if (b->vlc)
+        launch_phases_v<T, FLUX, true, true>(b, st, timing, timing_dtdh, only_phase);
+      else
+        launch_phases_v<T, FLUX, true, false>(b, st, timing, timing_dtdh, only_phase);
+      return;
+    }
+  }
   if (b->vlc)
-    launch_phases_v<T, FLUX, true>(b, st, timing, timing_dtdh, only_phase);
+    launch_phases_v<T, FLUX, false, true>(b, st, timing, timing_dtdh, only_phase);
   else
-    launch_phases_v<T, FLUX, false>(b, st, timing, timing_dtdh, only_phase);
+    launch_phases_v<T, FLUX, false, false>(b, st, timing, timing_dtdh, only_phase);
 }
 
 // only_phase: -1 both phases, else just that one.
@@ -290,8 +305,9 @@ int fl_batch_create(fl_context* c, const fl_batch_problem* problems, int nprob, 
     int rc = check_field(&q.state, "problem state");
     if (!rc) rc = check_field(&q.work, "problem work");
     if (!rc) rc = check_scheme(&q.scheme, q.state.ndim);
-    if (!rc && q.state.dtype == FL_F64) rc = require_exact(&q.scheme, "an FP64 batch");
     if (rc) return fail(rc, "problem %d: %s", p, fl_last_error());
+    if (q.state.dtype == FL_F64 && q.scheme.numerics != problems[0].scheme.numerics)
+      return fail(FL_ERR_UNSUPPORTED, "problem %d: one FP64 numerics per batch", p);
     if (q.work.dtype != q.state.dtype || q.work.layout != q.state.layout ||
         q.work.ndim != q.state.ndim || q.work.n[0] != q.state.n[0] ||
         (q.state.ndim > 1 && q.work.n[1] != q.state.n[1]) || q.work.data == q.state.data)
@@ -313,6 +329,7 @@ int fl_batch_create(fl_context* c, const fl_batch_problem* problems, int nprob, 
   b->nprob = nprob;
   b->dtype = dtype;
   b->flux = flux;
+  b->fast = dtype == FL_F64 && problems[0].scheme.numerics == FL_NUMERICS_FAST;
   b->vlc = FL_VLC_KERNELS != 0;
   for (int p = 0; p < nprob; ++p)
     b->vlc = b->vlc && problems[p].scheme.limiter == FL_LIMITER_VAN_LEER &&
