@@ -142,6 +142,14 @@ def _load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m fluidity_amd.build` "
             "(there is no fallback implementation)")
+    # torch ships its own HIP and HSA runtimes (SONAMEs libamdhip64.so.7 and
+    # libhsa-runtime64.so.1, requested by torch's libraries under the
+    # unversioned names).  With torch loaded first, this library's
+    # libamdhip64.so.7 dependency binds to that runtime and the process holds
+    # one.  Loaded first, this library would bring in /opt/rocm's runtime and
+    # torch a second one; whichever of the two initialises later then finds
+    # no device.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     fp = C.POINTER(Field)

@@ -40,6 +40,29 @@ def test_library_exports_every_declared_symbol(N):
     assert set(N.EXPORTED) == set(names)
 
 
+_MAPPED_RUNTIMES = """
+import sys
+sys.path.insert(0, {root!r})
+import fluidity_amd.native
+paths = set()
+for line in open('/proc/self/maps'):
+    parts = line.split()
+    if len(parts) >= 6 and ('libamdhip64' in parts[-1] or 'libhsa-runtime64' in parts[-1]):
+        paths.add(parts[-1])
+print(len([p for p in paths if 'libamdhip64' in p]), len([p for p in paths if 'libhsa' in p]))
+"""
+
+
+def test_one_hip_runtime_per_process():
+    """Importing the binding before torch leaves one HIP and one HSA runtime
+    in the process (torch's, which the library's libamdhip64.so.7 dependency
+    binds to).  Loading the library first used to map /opt/rocm's runtime
+    beside torch's, and the runtime initialised second found no device."""
+    out = subprocess.check_output(["python", "-c", _MAPPED_RUNTIMES.format(root=ROOT)],
+                                  text=True, cwd=ROOT)
+    assert out.split() == ["1", "1"], out
+
+
 def test_status_strings(N):
     assert N.lib.fl_status_string(0) == b"ok"
     assert N.lib.fl_status_string(N.FL_ERR_ARGUMENT) == b"invalid argument"
