@@ -30,9 +30,9 @@ int check_adj_scheme(const fl_scheme* s, int ndim) {
   if ((rc = require_exact(s, "the adjoint"))) return rc;
   if (s->eos == FL_EOS_STIFF_CORRECTED)
     return fail(FL_ERR_UNSUPPORTED, "the adjoint covers the reference's equations of state");
-  if (s->flux != FL_FLUX_HLLC || s->form != FL_FORM_CONSERVATIVE)
+  if ((s->flux != FL_FLUX_HLLC && s->flux != FL_FLUX_FORCE) || s->form != FL_FORM_CONSERVATIVE)
     return fail(FL_ERR_UNSUPPORTED,
-                "the adjoint covers HLLC with conservative-form limiting only");
+                "the adjoint covers HLLC and FORCE with conservative-form limiting only");
   return FL_OK;
 }
 
@@ -206,9 +206,10 @@ void sweep_adj(const fl_field* f, const fl_scheme* s, const double* Uin, const d
   hipLaunchKernelGGL(aj::chi_pad_kernel<ND>, dim3((unsigned)ceil_div(cpad, 256)), dim3(256), 0,
                      st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1, L.chi,
                      L.any);
-  hipLaunchKernelGGL((aj::face_adj_kernel<ND, D>), dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0,
-                     st, g, lam_out, m, s->limiter, rec, (const double*)L.chi, L.contrib,
-                     L.face_g, L.chi_g);
+  auto face = s->flux == FL_FLUX_FORCE ? aj::face_adj_kernel<ND, D, fl::kFluxForce>
+                                       : aj::face_adj_kernel<ND, D, fl::kFluxHllc>;
+  hipLaunchKernelGGL(face, dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0, st, g, lam_out, m,
+                     s->limiter, rec, (const double*)L.chi, L.contrib, L.face_g, L.chi_g);
   hipLaunchKernelGGL(aj::partial_sum_kernel, dim3(aj::kPartialBlocks), dim3(256), 0, st,
                      (const double*)L.face_g, nf, partials);
   const int64_t cells = g.nx * g.ny;

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __r
   chi[q] = c;
 }
 
-template <int ND, int D>
+template <int ND, int D, int FLUX>
 __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __restrict__ lam,
                                                        Material<double> m, int limiter,
                                                        const StepRec* __restrict__ rec,
@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
   double R[S], L[S], tmp[S], G[S];
   evolve_faces<double, ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, kFormCons, m, R, tmp);
   evolve_faces<double, ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, kFormCons, m, tmp, L);
-  hllc_flux<double, ND, D>(R, L, m, G);
+  if constexpr (FLUX == kFluxHllc) hllc_flux<double, ND, D>(R, L, m, G);
+  else force_flux<double, ND, D>(R, L, dtdh, 1.0 / dtdh, m, G);
   double la[S], lb[S];
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -183,7 +184,8 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
     gL[k] = 0.0;
     zero[k] = 0.0;
   }
-  hllc_adj<ND, D>(R, L, m, mu, gR, gL, gm);
+  if constexpr (FLUX == kFluxHllc) hllc_adj<ND, D>(R, L, m, mu, gR, gL, gm);
+  else force_adj<ND, D>(R, L, dtdh, m, mu, gR, gL, g_dtdh, gm);
   double c[4][S];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
@@ -578,9 +580,12 @@ __global__ void acc_clear_kernel(double* acc) {
 }
 
 #define FL_ADJ_INST(ND, D)                                                                  \
-  template __global__ void face_adj_kernel<ND, D>(AGrid, const double*, Material<double>,    \
-                                                  int, const StepRec*, const double*,        \
-                                                  double*, double*, double*);                \
+  template __global__ void face_adj_kernel<ND, D, kFluxHllc>(                                \
+      AGrid, const double*, Material<double>, int, const StepRec*, const double*, double*,     \
+      double*, double*);                                                                     \
+  template __global__ void face_adj_kernel<ND, D, kFluxForce>(                               \
+      AGrid, const double*, Material<double>, int, const StepRec*, const double*, double*,     \
+      double*, double*);                                                                     \
   template __global__ void gather_adj_kernel<ND, D>(AGrid, const double*, double*,           \
                                                     const double*);                          \
   template __global__ void chi_adj_kernel<ND, D>(AGrid, const double*, const double*,         \

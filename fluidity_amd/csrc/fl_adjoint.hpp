@@ -1,7 +1,7 @@
 // fl_adjoint.hpp -- hand-written reverse-mode (adjoint) versions of the FP64
 // per-cell numerics of fl_numerics.hpp: primitive conversion, sound speed,
 // Euler flux, van Leer / Superbee limiting, the MUSCL-Hancock evolved face
-// states (conservative-form limiting) and the HLLC flux.
+// states (conservative-form limiting), the HLLC flux and the FORCE flux.
 //
 // Each *_adj function recomputes nothing that the caller already holds: it
 // takes the primal inputs (and, where cheaper, the primal intermediates),
@@ -9,8 +9,8 @@
 // inputs into the g_* arguments, given the adjoint of its outputs.  The
 // adjoint of the material enters through d/dgamma (gm1 = gamma - 1 and
 // q = (gamma+1)/(2 gamma) depend on it; p_inf is a constant).
-// The flattening factor chi is held constant (its derivative is zero
-// wherever it is 1 or clamped, which covers smooth flow).
+// The evolved face states also return the adjoint of their flattening
+// factor chi; the caller carries it to the pressures chi was formed from.
 //
 // Primal quantities recomputed here (which decide branches) use the same
 // correctly rounded divisions as the forward step.  Derivative terms that
@@ -389,6 +389,49 @@ __device__ __forceinline__ void hllc_adj(const double* ul, const double* ur,
 }
 
 // Adjoint of the cell wavespeed max_d |u_d| + a (state/state_impl.hpp:167-178).
+// FORCE flux (force_flux, flux_method/flux_force.hpp:73-84):
+// F = (LF + RI) / 2 with LF = (fl + fr + (ul - ur) dhdt) / 2, RI = flux(uri),
+// uri = (ul + ur + (fl - fr) dtdh) / 2 and dhdt = 1 / dtdh.  Adds the adjoints
+// of ul, ur, dtdh (directly and through dhdt) and gamma, given gF.
+template <int ND, int D>
+__device__ __forceinline__ void force_adj(const double* ul, const double* ur, double dtdh,
+                                          const Material<double>& m, const double* gF,
+                                          double* gL, double* gR, double& g_dtdh, GMat& gm) {
+  constexpr int S = 2 + ND;
+  const double dhdt = 1.0 / dtdh;
+  const Prim<double, ND> wl = prim_of<double, ND>(ul, m);
+  const Prim<double, ND> wr = prim_of<double, ND>(ur, m);
+  double fl[S], fr[S], uri[S];
+  euler_flux<double, ND, D>(ul, wl, fl);
+  euler_flux<double, ND, D>(ur, wr, fr);
+#pragma unroll
+  for (int k = 0; k < S; ++k) uri[k] = (ul[k] + ur[k] + (fl[k] - fr[k]) * dtdh) * 0.5;
+  const Prim<double, ND> wri = prim_of<double, ND>(uri, m);
+  // adjoints of LF and of RI (each half of F), then of uri through RI's flux
+  double g_half[S], g_uri[S], g_fl[S], g_fr[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    g_half[k] = 0.5 * gF[k];
+    g_uri[k] = 0.0;
+  }
+  flux_full_adj<ND, D>(uri, wri, m, g_half, g_uri, gm);
+  double g_dhdt = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const double gl = 0.5 * g_half[k];  // LF's terms
+    const double gu = 0.5 * g_uri[k];   // uri's terms
+    gL[k] += gu + gl * dhdt;
+    gR[k] += gu - gl * dhdt;
+    g_fl[k] = gl + gu * dtdh;
+    g_fr[k] = gl - gu * dtdh;
+    g_dtdh += gu * (fl[k] - fr[k]);
+    g_dhdt += gl * (ul[k] - ur[k]);
+  }
+  flux_full_adj<ND, D>(ul, wl, m, g_fl, gL, gm);
+  flux_full_adj<ND, D>(ur, wr, m, g_fr, gR, gm);
+  g_dtdh -= g_dhdt * dhdt * dhdt;  // d(1/x)/dx = -1/x^2
+}
+
 template <int ND>
 __device__ __forceinline__ void wavespeed_adj(const double* U, const Material<double>& m,
                                               double g, double* gU, GMat& gm) {

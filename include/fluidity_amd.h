@@ -61,10 +61,13 @@ typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1, FL_EOS_STIFF_CORRECTED
  * the van Leer slope (minmod) and contracts multiply-adds; it is held to
  * 1e-12 relative L-inf of the reference over a 100-step run.  FP32 sweeps
  * use the fast forms whatever this says.  The single-grid entry points
- * (fl_split_sweep/solve/step, fl_simulate, fl_ctx_autotune) honour it; the
- * batched, two-material and adjoint entry points run FP64 exact and reject
- * FAST with FL_ERR_UNSUPPORTED.  An extension: the reference has one
- * arithmetic. */
+ * (fl_split_sweep/solve/step, fl_simulate, fl_ctx_autotune) honour it, and so
+ * do batches (every FP64 problem of a batch must ask for the same numerics,
+ * else FL_ERR_UNSUPPORTED) and the two-material loop (every material the
+ * same, else FL_ERR_ARGUMENT; FAST there also takes the level sets' WENO
+ * right-hand side in its fast form); the adjoint entry points run FP64 exact
+ * and reject FAST with FL_ERR_UNSUPPORTED.  An extension: the reference has
+ * one arithmetic. */
 typedef enum fl_numerics { FL_NUMERICS_EXACT = 0, FL_NUMERICS_FAST = 1 } fl_numerics;
 typedef enum fl_boundary {
   FL_BC_TRANSMISSIVE = 0,  /* solver/boundary_loader.hpp:29-32 */
@@ -324,11 +327,11 @@ int fl_batch_finish(fl_batch* batch, fl_sim_result* results, void* stream);
 int fl_batch_conservation_sums(fl_batch* batch, double* sums, void* stream);
 
 /* ---- adjoint of the split step ------------------------------------------------
- * Hand-written reverse mode of the FP64 split MUSCL-Hancock + HLLC step
- * (conservative-form limiting, any limiter and boundary), including the
- * dependence of dt = cfl dh / max wavespeed on the state and on gamma.
- * Branches follow the primal (the derivative of the branch taken), the
- * flattening factor chi is held constant.  The reference has no adjoint; these
+ * Hand-written reverse mode of the FP64 split MUSCL-Hancock step with the
+ * HLLC or FORCE flux (conservative-form limiting, any limiter and boundary),
+ * including the dependence of dt = cfl dh / max wavespeed and of the
+ * flattening factor chi on the state and on gamma.  Branches follow the
+ * primal (the derivative of the branch taken).  The reference has no adjoint; these
  * entry points differentiate its update call
  * (solver/split_solver.hpp:105-162 inside simulator/simulator_impl.hpp:222-258).
  * Adjoint fields use the state's layout. */

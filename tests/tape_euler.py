@@ -1,10 +1,11 @@
-"""FP64 tape reference of the split MUSCL-Hancock + HLLC step in torch (CPU).
+"""FP64 tape reference of the split MUSCL-Hancock + HLLC / FORCE step in torch (CPU).
 
 TEST INFRASTRUCTURE ONLY: the reference gradient ("tape") the hand-written
 adjoint is checked against.  The forward restates the oracle's step (the
 reference's solver/split_solver.hpp:138-162 with
 reconstruction/muscl_reconstructor.hpp:59-172, limiting/van_leer_limiter.hpp,
-flux_method/flux_hllc.hpp:56-245, dt from simulator/parameters.hpp:60-63)
+flux_method/flux_hllc.hpp:56-245 or flux_method/flux_force.hpp:73-84, dt from
+simulator/parameters.hpp:60-63)
 with vectorised torch operations in the same operation order, so that its
 values equal the oracle's; torch.autograd then differentiates it.  Branches
 are torch.where selections (the derivative of the branch taken, with the
@@ -199,8 +200,25 @@ def hllc(ul, ur, gamma, D, nd):
     return [torch.where(m_l, Fl[k], torch.where(m_r, Fr[k], Fs[k])) for k in range(S)]
 
 
-def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0):
-    """One split sweep in dimension D (van Leer, conservative-form limiting)."""
+def force(ul, ur, gamma, D, nd, dtdh):
+    """FORCE flux (flux_method/flux_force.hpp:73-84): the mean of the
+    Lax-Friedrichs flux (flux_lax_friedrichs.hpp:62-78) and the Richtmyer flux
+    (flux_richtmyer.hpp:78-88), dh/dt = 1 / (dt/dh)."""
+    dhdt = 1.0 / dtdh
+    S = 2 + nd
+    wl_v, pl = prim(ul, gamma, nd)
+    wr_v, pr = prim(ur, gamma, nd)
+    fl = flux(ul, wl_v, pl, D, nd)
+    fr = flux(ur, wr_v, pr, D, nd)
+    uri = [(ul[k] + ur[k] + (fl[k] - fr[k]) * dtdh) * 0.5 for k in range(S)]
+    wi_v, pi = prim(uri, gamma, nd)
+    fri = flux(uri, wi_v, pi, D, nd)
+    return [((fl[k] + fr[k] + (ul[k] - ur[k]) * dhdt) * 0.5 + fri[k]) * 0.5 for k in range(S)]
+
+
+def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0, flux_method="hllc"):
+    """One split sweep in dimension D (van Leer, conservative-form limiting;
+    flux_method "hllc" or "force")."""
     assert limiter == 0
     P = pad(U, nd, bcs)
     chi = _chi_min(P, gamma, nd)                     # grid padded by 1 everywhere
@@ -254,7 +272,10 @@ def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0):
     n = U[0].shape[axis]
     Rf = [r.narrow(axis, 0, n + 1) for r in R]       # R_i, i = -1..n-1
     Lf = [l.narrow(axis, 1, n + 1) for l in L]       # L_{i+1}
-    G = hllc(Rf, Lf, gamma, D, nd)                   # faces -1/2 .. n-1/2
+    if flux_method == "force":                       # faces -1/2 .. n-1/2
+        G = force(Rf, Lf, gamma, D, nd, dtdh)
+    else:
+        G = hllc(Rf, Lf, gamma, D, nd)
     out = []
     for k in range(S):
         gm = G[k].narrow(axis, 0, n)
@@ -272,14 +293,14 @@ def max_wavespeed(U, gamma, nd):
     return (mv + sound(p, U[0], gamma)).max()
 
 
-def step(U, gamma, cfl, dh, nd, bcs):
+def step(U, gamma, cfl, dh, nd, bcs, flux_method="hllc"):
     maxws = max_wavespeed(U, gamma, nd)
     dt = cfl * dh / maxws
     dtdh = dt / dh
-    X = sweep(U, gamma, dtdh, 0, nd, bcs)
+    X = sweep(U, gamma, dtdh, 0, nd, bcs, flux_method=flux_method)
     if nd == 1:
         return X
-    return sweep(X, gamma, dtdh, 1, nd, bcs)
+    return sweep(X, gamma, dtdh, 1, nd, bcs, flux_method=flux_method)
 
 
 def split(Usoa):

@@ -1,10 +1,10 @@
 """The hand-written adjoint on the GPU.
 
 * One step's vector-Jacobian product (fl_split_step_vjp) against the torch
-  FP64 tape of the same step, 1D and 2D, each boundary kind.
+  FP64 tape of the same step, 1D and 2D, each boundary kind, HLLC and FORCE.
 * The checkpointed multi-step adjoint (fl_adj_forward / fl_adj_backward)
-  against the tape over 25 steps, and bit-for-bit invariance under the
-  checkpoint interval.
+  against the tape over 25 steps (HLLC and FORCE), and bit-for-bit
+  invariance under the checkpoint interval.
 * The forward run equals fl_simulate step for step.
 * Central finite differences on 64 smooth random directions of
   (rho, rho u, rho v, E, gamma) on the 1024^2, 200-step workload with a
@@ -19,6 +19,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from fluidity_amd import native as N
 from fluidity_amd import workloads as W
 from tests import tape_euler as T
 
@@ -61,27 +62,50 @@ def _generic_state(nd, ny=20, nx=24, seed=3):
     return U * (1 + 0.01 * rng.random(U.shape))
 
 
-def _tape_vjp(U, lam, nd, bc, cfl, dh, steps=1):
+def _tape_vjp(U, lam, nd, bc, cfl, dh, steps=1, flux="hllc"):
     Ut = [torch.tensor(np.ascontiguousarray(np.moveaxis(U, -1, 0)[k]), requires_grad=True)
           for k in range(U.shape[-1])]
     g = torch.tensor(1.4, dtype=torch.float64, requires_grad=True)
     X = Ut
     for _ in range(steps):
-        X = T.step(X, g, cfl, dh, nd, bc)
+        X = T.step(X, g, cfl, dh, nd, bc, flux_method=flux)
     L = sum((x * torch.tensor(np.ascontiguousarray(lam[k]))).sum() for k, x in enumerate(X))
     grads = torch.autograd.grad(L, Ut + [g])
     return np.stack([gr.numpy() for gr in grads[:-1]]), float(grads[-1])
 
 
+def test_binding_imported_before_torch_initialises(A):
+    """The binding imported first, then torch's first GPU call, then a
+    context: the library's runtime is torch's, so it finds the device (it
+    found none while the library brought a second runtime)."""
+    import subprocess
+    import sys
+    code = ("import fluidity_amd.native as N\n"
+            "import torch\n"
+            "torch.zeros(1, device='cuda')\n"
+            "ctx = N.Context(0)\n"
+            "print('context ok')\n")
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__file__))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=root,
+                         timeout=120)
+    assert out.returncode == 0 and "context ok" in out.stdout, out.stderr[-2000:]
+
+
+def _scheme(E, bc, flux):
+    return E.Scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc)),
+                    flux=N.FL_FLUX_FORCE if flux == "force" else N.FL_FLUX_HLLC)
+
+
+@pytest.mark.parametrize("flux", ["hllc", "force"])
 @pytest.mark.parametrize("nd,bc", [(2, (2, 2)), (2, (0, 1)), (2, (1, 2)), (1, (2,)), (1, (1,))])
-def test_step_vjp_matches_tape(A, E, nd, bc):
+def test_step_vjp_matches_tape(A, E, nd, bc, flux):
     U = _generic_state(nd)
     rng = np.random.default_rng(4)
     lam = rng.standard_normal(_soa(U).shape)
     dh = 1.0 / U.shape[-2]
-    adj = A.Adjoint(E.Scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc))), 0.9, dh)
+    adj = A.Adjoint(_scheme(E, bc, flux), 0.9, dh)
     lam_g, dg = adj.step_vjp(_dev(U), torch.from_numpy(lam).to("cuda"))
-    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh)
+    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh, flux=flux)
     got = lam_g.cpu().numpy()
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-11 * scale
@@ -114,8 +138,11 @@ def _shock_state(nd):
     return U[0] if nd == 1 else U
 
 
-@pytest.mark.parametrize("nd,bc", [(2, (0, 0)), (2, (1, 2)), (2, (2, 1)), (1, (0,)), (1, (1,))])
-def test_step_vjp_through_flattening_matches_tape(A, E, nd, bc):
+@pytest.mark.parametrize("nd,bc,flux", [(2, (0, 0), "hllc"), (2, (1, 2), "hllc"),
+                                        (2, (2, 1), "hllc"), (1, (0,), "hllc"),
+                                        (1, (1,), "hllc"), (2, (1, 2), "force"),
+                                        (1, (0,), "force")])
+def test_step_vjp_through_flattening_matches_tape(A, E, nd, bc, flux):
     """Where the flattening factor chi lies strictly between 0 and 1 the
     adjoint differentiates through it (the factor the minimum selected, its
     pressure ratio, dp/dU and dp/dgamma of the cells it reads, through ghost
@@ -128,17 +155,17 @@ def test_step_vjp_through_flattening_matches_tape(A, E, nd, bc):
     rng = np.random.default_rng(9)
     lam = rng.standard_normal(_soa(U).shape)
     dh = 1.0 / U.shape[-2]
-    adj = A.Adjoint(E.Scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc))), 0.9, dh)
+    adj = A.Adjoint(_scheme(E, bc, flux), 0.9, dh)
     lam_g, dg = adj.step_vjp(_dev(U), torch.from_numpy(lam).to("cuda"))
-    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh)
+    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh, flux=flux)
     got = lam_g.cpu().numpy()
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-11 * scale, np.abs(got - ref).max() / scale
     assert abs(dg - dg_ref) <= 1e-11 * max(abs(dg_ref), scale)
 
 
-def _adjoint_run(A, E, U, target, dh, steps, every, bc=(2, 2)):
-    adj = A.Adjoint(E.Scheme(gamma=1.4, bc=tuple(bc) + (0,)), 0.9, dh)
+def _adjoint_run(A, E, U, target, dh, steps, every, bc=(2, 2), flux="hllc"):
+    adj = A.Adjoint(_scheme(E, bc, flux), 0.9, dh)
     Ud = _dev(U)
     run = adj.forward(Ud, steps, every)
     tgt = torch.from_numpy(target).to("cuda")
@@ -149,13 +176,14 @@ def _adjoint_run(A, E, U, target, dh, steps, every, bc=(2, 2)):
     return loss, grad.cpu().numpy(), dg, Ud.cpu().numpy(), run
 
 
-def test_multistep_adjoint_matches_tape(A, E):
+@pytest.mark.parametrize("flux", ["hllc", "force"])
+def test_multistep_adjoint_matches_tape(A, E, flux):
     U = _generic_state(2)
     ny, nx = U.shape[:2]
     dh = 1.0 / nx
     target = 1.0 + 0.05 * np.random.default_rng(5).random((ny, nx))
     steps = 25
-    loss, g, dg, _, _ = _adjoint_run(A, E, U, target, dh, steps, 7)
+    loss, g, dg, _, _ = _adjoint_run(A, E, U, target, dh, steps, 7, flux=flux)
     lam = np.zeros((4, ny, nx))
     # tape: L = dh^2 sum (rho_N - target)^2
     Ut = [torch.tensor(np.ascontiguousarray(np.moveaxis(U, -1, 0)[k]), requires_grad=True)
@@ -163,7 +191,7 @@ def test_multistep_adjoint_matches_tape(A, E):
     gam = torch.tensor(1.4, dtype=torch.float64, requires_grad=True)
     X = Ut
     for _ in range(steps):
-        X = T.step(X, gam, 0.9, dh, 2, (2, 2))
+        X = T.step(X, gam, 0.9, dh, 2, (2, 2), flux_method=flux)
     L = (dh * dh * (X[0] - torch.tensor(target)) ** 2).sum()
     grads = torch.autograd.grad(L, Ut + [gam])
     ref = np.stack([gr.numpy() for gr in grads[:-1]])
