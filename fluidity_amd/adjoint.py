@@ -4,7 +4,7 @@ with a thin torch.autograd.Function binding used to check the gradients.
 The reference has no adjoint; this differentiates its update loop
 (``SimulatorImpl.simulate``, simulator/simulator_impl.hpp:222-258, with
 ``SplitSolver.solve``, solver/split_solver.hpp:105-162): FP64, HLLC or FORCE,
-conservative-form limiting, dt = cfl dh / max wavespeed recomputed every step
+either limiter and limiting form, dt = cfl dh / max wavespeed recomputed every step
 and the flattening factor (their dependence on the state and on gamma
 included).  Grids are FP64 device
 tensors as in ``euler.grid_field``.

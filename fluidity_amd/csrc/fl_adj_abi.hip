@@ -30,9 +30,8 @@ int check_adj_scheme(const fl_scheme* s, int ndim) {
   if ((rc = require_exact(s, "the adjoint"))) return rc;
   if (s->eos == FL_EOS_STIFF_CORRECTED)
     return fail(FL_ERR_UNSUPPORTED, "the adjoint covers the reference's equations of state");
-  if ((s->flux != FL_FLUX_HLLC && s->flux != FL_FLUX_FORCE) || s->form != FL_FORM_CONSERVATIVE)
-    return fail(FL_ERR_UNSUPPORTED,
-                "the adjoint covers HLLC and FORCE with conservative-form limiting only");
+  if (s->flux != FL_FLUX_HLLC && s->flux != FL_FLUX_FORCE)
+    return fail(FL_ERR_UNSUPPORTED, "the adjoint covers the HLLC and FORCE fluxes");
   return FL_OK;
 }
 
@@ -206,8 +205,11 @@ void sweep_adj(const fl_field* f, const fl_scheme* s, const double* Uin, const d
   hipLaunchKernelGGL(aj::chi_pad_kernel<ND>, dim3((unsigned)ceil_div(cpad, 256)), dim3(256), 0,
                      st, g, (const double*)L.P, (const double*)L.V0, (const double*)L.V1, L.chi,
                      L.any);
-  auto face = s->flux == FL_FLUX_FORCE ? aj::face_adj_kernel<ND, D, fl::kFluxForce>
-                                       : aj::face_adj_kernel<ND, D, fl::kFluxHllc>;
+  const bool force = s->flux == FL_FLUX_FORCE, prim = s->form == FL_FORM_PRIMITIVE;
+  auto face = force ? (prim ? aj::face_adj_kernel<ND, D, fl::kFluxForce, true>
+                            : aj::face_adj_kernel<ND, D, fl::kFluxForce, false>)
+                    : (prim ? aj::face_adj_kernel<ND, D, fl::kFluxHllc, true>
+                            : aj::face_adj_kernel<ND, D, fl::kFluxHllc, false>);
   hipLaunchKernelGGL(face, dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0, st, g, lam_out, m,
                      s->limiter, rec, (const double*)L.chi, L.contrib, L.face_g, L.chi_g);
   hipLaunchKernelGGL(aj::partial_sum_kernel, dim3(aj::kPartialBlocks), dim3(256), 0, st,

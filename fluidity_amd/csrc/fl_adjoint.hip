@@ -104,7 +104,9 @@ __global__ void __launch_bounds__(256) chi_pad_kernel(AGrid g, const double* __r
   chi[q] = c;
 }
 
-template <int ND, int D, int FLUX>
+// PRIM: primitive-form limiting (a template parameter, so that the
+// conservative-form kernel carries none of its code).
+template <int ND, int D, int FLUX, bool PRIM>
 __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __restrict__ lam,
                                                        Material<double> m, int limiter,
                                                        const StepRec* __restrict__ rec,
@@ -113,6 +115,7 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
                                                        double* __restrict__ face_g,
                                                        double* __restrict__ chi_g) {
   constexpr int S = 2 + ND;
+  constexpr int form = PRIM ? kFormPrim : kFormCons;
   const int64_t n = D == 0 ? g.nx : g.ny;          // cells along the sweep
   const int64_t lines = D == 0 ? g.ny : g.nx;
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,8 +152,8 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
   const double chi_a = chi_pad[(ND > 1 ? (ya + 1) * cxw : 0) + xa + 1];
   const double chi_b = chi_pad[(ND > 1 ? (yb + 1) * cxw : 0) + xb + 1];
   double R[S], L[S], tmp[S], G[S];
-  evolve_faces<double, ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, kFormCons, m, R, tmp);
-  evolve_faces<double, ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, kFormCons, m, tmp, L);
+  evolve_faces<double, ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, form, m, R, tmp);
+  evolve_faces<double, ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, form, m, tmp, L);
   if constexpr (FLUX == kFluxHllc) hllc_flux<double, ND, D>(R, L, m, G);
   else force_flux<double, ND, D>(R, L, dtdh, 1.0 / dtdh, m, G);
   double la[S], lb[S];
@@ -192,10 +195,10 @@ __global__ void __launch_bounds__(256) face_adj_kernel(AGrid g, const double* __
 #pragma unroll
     for (int k = 0; k < S; ++k) c[s][k] = 0.0;
   double g_half = 0.0, g_chi_a = 0.0, g_chi_b = 0.0;
-  evolve_adj<ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, m, gR, zero, c[0], c[1], c[2],
-                    g_half, g_chi_a, gm);
-  evolve_adj<ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, m, zero, gL, c[1], c[2], c[3],
-                    g_half, g_chi_b, gm);
+  evolve_adj<ND, D>(u[0], u[1], u[2], chi_a, half_dtdh, limiter, form, m, gR, zero, c[0], c[1],
+                    c[2], g_half, g_chi_a, gm);
+  evolve_adj<ND, D>(u[1], u[2], u[3], chi_b, half_dtdh, limiter, form, m, zero, gL, c[1], c[2],
+                    c[3], g_half, g_chi_b, gm);
   chi_g[2 * f] = g_chi_a;      // of cell i (its right face state)
   chi_g[2 * f + 1] = g_chi_b;  // of cell i + 1 (its left face state)
   // contributions as planes (slot, component) x faces: coalesced both here and
@@ -579,13 +582,15 @@ __global__ void acc_clear_kernel(double* acc) {
   acc[1] = 0.0;
 }
 
+#define FL_FACE_INST(ND, D, FLUX, PRIM)                                                     \
+  template __global__ void face_adj_kernel<ND, D, FLUX, PRIM>(                               \
+      AGrid, const double*, Material<double>, int, const StepRec*, const double*, double*,     \
+      double*, double*);
 #define FL_ADJ_INST(ND, D)                                                                  \
-  template __global__ void face_adj_kernel<ND, D, kFluxHllc>(                                \
-      AGrid, const double*, Material<double>, int, const StepRec*, const double*, double*,     \
-      double*, double*);                                                                     \
-  template __global__ void face_adj_kernel<ND, D, kFluxForce>(                               \
-      AGrid, const double*, Material<double>, int, const StepRec*, const double*, double*,     \
-      double*, double*);                                                                     \
+  FL_FACE_INST(ND, D, kFluxHllc, false)                                                      \
+  FL_FACE_INST(ND, D, kFluxForce, false)                                                     \
+  FL_FACE_INST(ND, D, kFluxHllc, true)                                                       \
+  FL_FACE_INST(ND, D, kFluxForce, true)                                                      \
   template __global__ void gather_adj_kernel<ND, D>(AGrid, const double*, double*,           \
                                                     const double*);                          \
   template __global__ void chi_adj_kernel<ND, D>(AGrid, const double*, const double*,         \

@@ -145,20 +145,39 @@ __device__ __forceinline__ void limit_adj(int limiter, double L, double R, doubl
 // gup), of half_dtdh, of the flattening factor chi and of gamma.
 template <int ND, int D>
 __device__ __forceinline__ void evolve_adj(const double* um, const double* u0, const double* up,
-                                           double chi, double half_dtdh, int limiter,
+                                           double chi, double half_dtdh, int limiter, int form,
                                            const Material<double>& m, const double* gR,
                                            const double* gL, double* gum, double* gu0,
                                            double* gup, double& g_half_dtdh, double& g_chi,
                                            GMat& gm) {
   constexpr int S = 2 + ND;
   const double cscale = chi * 0.5;
+  const bool pf = form == kFormPrim;
   double plus[S], minus[S], back[S], fwrd[S], lim0[S], phi[S];
+  // primitive form: the limiter reads differences of (rho, p, vel)
+  // (limiting/limiting_impl.hpp:77-81, 142-146), as evolve_faces does
+  Prim<double, ND> wm{}, w0{}, wp{};
+  double bd[S], fd[S];
+  if (pf) {
+    wm = prim_of<double, ND>(um, m);
+    w0 = prim_of<double, ND>(u0, m);
+    wp = prim_of<double, ND>(up, m);
+    bd[0] = u0[0] - um[0];
+    fd[0] = up[0] - u0[0];
+    bd[1] = w0.p - wm.p;
+    fd[1] = wp.p - w0.p;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      bd[2 + i] = w0.vel[i] - wm.vel[i];
+      fd[2 + i] = wp.vel[i] - w0.vel[i];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     back[k] = u0[k] - um[k];
     fwrd[k] = up[k] - u0[k];
     lim0[k] = (back[k] + fwrd[k]) * 0.5;
-    phi[k] = limit_single(limiter, back[k], fwrd[k]);
+    phi[k] = pf ? limit_single(limiter, bd[k], fd[k]) : limit_single(limiter, back[k], fwrd[k]);
     const double delta = (lim0[k] * phi[k]) * cscale;
     plus[k] = u0[k] + delta;
     minus[k] = u0[k] - delta;
@@ -180,6 +199,7 @@ __device__ __forceinline__ void evolve_adj(const double* um, const double* u0, c
   }
   flux_full_adj<ND, D>(minus, wmi, m, g_fm, g_minus, gm);
   flux_full_adj<ND, D>(plus, wpl, m, g_fp, g_plus, gm);
+  double g_bd[S], g_fd[S];
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     gu0[k] += g_plus[k] + g_minus[k];
@@ -190,10 +210,29 @@ __device__ __forceinline__ void evolve_adj(const double* um, const double* u0, c
     const double g_lim0 = g_lim * phi[k];
     const double g_phi = g_lim * lim0[k];
     double g_back = 0.5 * g_lim0, g_fwrd = 0.5 * g_lim0;
-    limit_adj(limiter, back[k], fwrd[k], g_phi, g_back, g_fwrd);
+    g_bd[k] = 0.0;
+    g_fd[k] = 0.0;
+    if (pf) limit_adj(limiter, bd[k], fd[k], g_phi, g_bd[k], g_fd[k]);
+    else limit_adj(limiter, back[k], fwrd[k], g_phi, g_back, g_fwrd);
     gu0[k] += g_back - g_fwrd;
     gum[k] -= g_back;
     gup[k] += g_fwrd;
+  }
+  if (pf) {
+    // density differences are conservative ones; p and vel through prim()
+    gu0[0] += g_bd[0] - g_fd[0];
+    gum[0] -= g_bd[0];
+    gup[0] += g_fd[0];
+    double gvm[ND], gv0[ND], gvp[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      gv0[i] = g_bd[2 + i] - g_fd[2 + i];
+      gvm[i] = -g_bd[2 + i];
+      gvp[i] = g_fd[2 + i];
+    }
+    prim_adj<ND>(um, wm, m, gvm, -g_bd[1], gum, gm);
+    prim_adj<ND>(u0, w0, m, gv0, g_bd[1] - g_fd[1], gu0, gm);
+    prim_adj<ND>(up, wp, m, gvp, g_fd[1], gup, gm);
   }
 }
 

@@ -30,7 +30,7 @@ __global__ void prim_pad_kernel(AGrid g, Material<double> m, double* P, double* 
 template <int ND>
 __global__ void chi_pad_kernel(AGrid g, const double* P, const double* V0, const double* V1,
                                double* chi, int* any);
-template <int ND, int D, int FLUX>
+template <int ND, int D, int FLUX, bool PRIM>
 __global__ void face_adj_kernel(AGrid g, const double* lam, Material<double> m, int limiter,
                                 const StepRec* rec, const double* chi_pad, double* contrib,
                                 double* face_g, double* chi_g);

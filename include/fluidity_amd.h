@@ -328,7 +328,7 @@ int fl_batch_conservation_sums(fl_batch* batch, double* sums, void* stream);
 
 /* ---- adjoint of the split step ------------------------------------------------
  * Hand-written reverse mode of the FP64 split MUSCL-Hancock step with the
- * HLLC or FORCE flux (conservative-form limiting, any limiter and boundary),
+ * HLLC or FORCE flux (either limiter, either limiting form, any boundary),
  * including the dependence of dt = cfl dh / max wavespeed and of the
  * flattening factor chi on the state and on gamma.  Branches follow the
  * primal (the derivative of the branch taken).  The reference has no adjoint; these

@@ -216,8 +216,10 @@ def force(ul, ur, gamma, D, nd, dtdh):
     return [((fl[k] + fr[k] + (ul[k] - ur[k]) * dhdt) * 0.5 + fri[k]) * 0.5 for k in range(S)]
 
 
-def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0, flux_method="hllc"):
-    """One split sweep in dimension D (van Leer, conservative-form limiting;
+def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0, flux_method="hllc", form="cons"):
+    """One split sweep in dimension D (van Leer; form "cons" limits on the
+    conservative differences, "prim" takes the limiter's ratio from the
+    differences of (rho, p, vel), limiting/limiting_impl.hpp:77-81, 142-146;
     flux_method "hllc" or "force")."""
     assert limiter == 0
     P = pad(U, nd, bcs)
@@ -251,12 +253,21 @@ def sweep(U, gamma, dtdh, D, nd, bcs, limiter=0, flux_method="hllc"):
     up = [cells(c, 1) for c in P]
     cscale = chi_cells() * 0.5
     half = 0.5 * dtdh
+    if form == "prim":
+        vm_, pm_ = prim(um, gamma, nd)
+        v0_, p0_ = prim(u0, gamma, nd)
+        vp_, pp_ = prim(up, gamma, nd)
+        bd = [u0[0] - um[0], p0_ - pm_] + [v0_[i] - vm_[i] for i in range(nd)]
+        fd = [up[0] - u0[0], pp_ - p0_] + [vp_[i] - v0_[i] for i in range(nd)]
     plus, minus = [], []
     for k in range(S):
         back = u0[k] - um[k]
         fwrd = up[k] - u0[k]
         lim = (back + fwrd) * 0.5
-        lim = lim * van_leer(back, fwrd)
+        if form == "prim":
+            lim = lim * van_leer(bd[k], fd[k])
+        else:
+            lim = lim * van_leer(back, fwrd)
         delta = lim * cscale
         plus.append(u0[k] + delta)
         minus.append(u0[k] - delta)
@@ -293,14 +304,14 @@ def max_wavespeed(U, gamma, nd):
     return (mv + sound(p, U[0], gamma)).max()
 
 
-def step(U, gamma, cfl, dh, nd, bcs, flux_method="hllc"):
+def step(U, gamma, cfl, dh, nd, bcs, flux_method="hllc", form="cons"):
     maxws = max_wavespeed(U, gamma, nd)
     dt = cfl * dh / maxws
     dtdh = dt / dh
-    X = sweep(U, gamma, dtdh, 0, nd, bcs, flux_method=flux_method)
+    X = sweep(U, gamma, dtdh, 0, nd, bcs, flux_method=flux_method, form=form)
     if nd == 1:
         return X
-    return sweep(X, gamma, dtdh, 1, nd, bcs, flux_method=flux_method)
+    return sweep(X, gamma, dtdh, 1, nd, bcs, flux_method=flux_method, form=form)
 
 
 def split(Usoa):

@@ -1,7 +1,8 @@
 """The hand-written adjoint on the GPU.
 
 * One step's vector-Jacobian product (fl_split_step_vjp) against the torch
-  FP64 tape of the same step, 1D and 2D, each boundary kind, HLLC and FORCE.
+  FP64 tape of the same step, 1D and 2D, each boundary kind, HLLC and FORCE,
+  conservative- and primitive-form limiting.
 * The checkpointed multi-step adjoint (fl_adj_forward / fl_adj_backward)
   against the tape over 25 steps (HLLC and FORCE), and bit-for-bit
   invariance under the checkpoint interval.
@@ -62,13 +63,13 @@ def _generic_state(nd, ny=20, nx=24, seed=3):
     return U * (1 + 0.01 * rng.random(U.shape))
 
 
-def _tape_vjp(U, lam, nd, bc, cfl, dh, steps=1, flux="hllc"):
+def _tape_vjp(U, lam, nd, bc, cfl, dh, steps=1, flux="hllc", form="cons"):
     Ut = [torch.tensor(np.ascontiguousarray(np.moveaxis(U, -1, 0)[k]), requires_grad=True)
           for k in range(U.shape[-1])]
     g = torch.tensor(1.4, dtype=torch.float64, requires_grad=True)
     X = Ut
     for _ in range(steps):
-        X = T.step(X, g, cfl, dh, nd, bc, flux_method=flux)
+        X = T.step(X, g, cfl, dh, nd, bc, flux_method=flux, form=form)
     L = sum((x * torch.tensor(np.ascontiguousarray(lam[k]))).sum() for k, x in enumerate(X))
     grads = torch.autograd.grad(L, Ut + [g])
     return np.stack([gr.numpy() for gr in grads[:-1]]), float(grads[-1])
@@ -91,9 +92,32 @@ def test_binding_imported_before_torch_initialises(A):
     assert out.returncode == 0 and "context ok" in out.stdout, out.stderr[-2000:]
 
 
-def _scheme(E, bc, flux):
+def _scheme(E, bc, flux, form="cons"):
     return E.Scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc)),
-                    flux=N.FL_FLUX_FORCE if flux == "force" else N.FL_FLUX_HLLC)
+                    flux=N.FL_FLUX_FORCE if flux == "force" else N.FL_FLUX_HLLC,
+                    form=N.FL_FORM_PRIMITIVE if form == "prim" else N.FL_FORM_CONSERVATIVE)
+
+
+@pytest.mark.parametrize("nd,bc,flux,state", [(2, (2, 2), "hllc", "generic"),
+                                              (2, (0, 1), "force", "generic"),
+                                              (1, (1,), "hllc", "generic"),
+                                              (2, (1, 2), "hllc", "shock"),
+                                              (1, (0,), "force", "shock")])
+def test_step_vjp_primitive_form_matches_tape(A, E, nd, bc, flux, state):
+    """Primitive-form limiting (the limiter's ratio from differences of
+    rho, p and vel): one step's VJP equals the tape's, on a generic state and
+    on a shock state where the flattening factor lies inside (0, 1)."""
+    U = _generic_state(nd) if state == "generic" else _shock_state(nd)
+    rng = np.random.default_rng(10)
+    lam = rng.standard_normal(_soa(U).shape)
+    dh = 1.0 / U.shape[-2]
+    adj = A.Adjoint(_scheme(E, bc, flux, "prim"), 0.9, dh)
+    lam_g, dg = adj.step_vjp(_dev(U), torch.from_numpy(lam).to("cuda"))
+    ref, dg_ref = _tape_vjp(U, lam, nd, bc, 0.9, dh, flux=flux, form="prim")
+    got = lam_g.cpu().numpy()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-11 * scale, np.abs(got - ref).max() / scale
+    assert abs(dg - dg_ref) <= 1e-11 * max(abs(dg_ref), scale)
 
 
 @pytest.mark.parametrize("flux", ["hllc", "force"])

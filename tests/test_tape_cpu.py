@@ -37,25 +37,27 @@ def _to_np(Ut):
     return np.moveaxis(np.stack([u.detach().numpy() for u in Ut]), 0, -1)
 
 
-@pytest.mark.parametrize("flux", ["hllc", "force"])
+@pytest.mark.parametrize("flux,form", [("hllc", "cons"), ("force", "cons"), ("hllc", "prim"),
+                                       ("force", "prim")])
 @pytest.mark.parametrize("nd,bc", [(1, (2,)), (1, (1,)), (1, (0,)), (2, (2, 2)), (2, (0, 1)),
                                    (2, (1, 2))])
-def test_tape_forward_matches_oracle(oracle, nd, bc, flux):
+def test_tape_forward_matches_oracle(oracle, nd, bc, flux, form):
     rng = np.random.default_rng(1)
     U = _state(nd, rng)
     s = oracle.scheme(gamma=1.4, bc=tuple(bc) + (0,) * (3 - len(bc)),
-                      flux=oracle.FLUX_FORCE if flux == "force" else oracle.FLUX_HLLC)
+                      flux=oracle.FLUX_FORCE if flux == "force" else oracle.FLUX_HLLC,
+                      form=oracle.FORM_PRIM if form == "prim" else oracle.FORM_CONS)
     dh = 1.0 / U.shape[-2]
     Uo, _, it, _ = oracle.simulate(U, s, 0.9, dh, max_iters=3)
     Ut = _to_t(U)
     g = torch.tensor(1.4, dtype=torch.float64)
     for _ in range(3):
-        Ut = T.step(Ut, g, 0.9, dh, nd, bc, flux_method=flux)
+        Ut = T.step(Ut, g, 0.9, dh, nd, bc, flux_method=flux, form=form)
     np.testing.assert_allclose(_to_np(Ut), Uo, rtol=1e-13, atol=1e-15)
 
 
-@pytest.mark.parametrize("flux", ["hllc", "force"])
-def test_tape_gradient_matches_finite_differences(flux):
+@pytest.mark.parametrize("flux,form", [("hllc", "cons"), ("force", "cons"), ("hllc", "prim")])
+def test_tape_gradient_matches_finite_differences(flux, form):
     rng = np.random.default_rng(2)
     U = _state(2, rng)
     dh = 1.0 / 14
@@ -63,7 +65,7 @@ def test_tape_gradient_matches_finite_differences(flux):
 
     def loss(Ut, g):
         for _ in range(2):
-            Ut = T.step(Ut, g, 0.9, dh, 2, (2, 2), flux_method=flux)
+            Ut = T.step(Ut, g, 0.9, dh, 2, (2, 2), flux_method=flux, form=form)
         return sum((u * torch.tensor(np.ascontiguousarray(lam[..., k]))).sum()
                    for k, u in enumerate(Ut))
     Ut = [u.requires_grad_() for u in _to_t(U)]
