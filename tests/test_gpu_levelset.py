@@ -484,3 +484,37 @@ def test_mm_simulate_three_materials_bitwise(L, E, oracle, layout, every):
     for m in range(3):
         _same(_host(dev_U[m], layout), Us[m])
         _same(_host(dev_p[m]), phis[m])
+
+
+def test_mm_simulate_three_materials_fast_numerics(L, E, oracle):
+    """The n-material loop in FL_NUMERICS_FAST (three materials, seed 1007)
+    against the oracle's exact n-material loop: every field of every
+    material and every level set within 1e-12 relative L-inf (momentum
+    against the largest momentum magnitude, as in the two-material test), the
+    same step count, run time within 1e-12, no level-set sign change."""
+    sbs = W.shock_two_bubbles(192, 96, seed=1007)
+    ss = [oracle.scheme(gamma=g) for g in sbs.gammas]
+    steps = 10
+    Us, phis, rt, it, _ = oracle.mm_simulate_n(sbs.Us, sbs.phis, ss, 0.5, sbs.dh,
+                                               max_iters=steps, reinit_every=5)
+    sim = L.MultimaterialSimulator(layout=1)
+    dev_U = [_dev(U, 1) for U in sbs.Us]
+    dev_p = [_dev(p) for p in sbs.phis]
+    for U, p, g in zip(dev_U, dev_p, sbs.gammas):
+        sim.add_material(U, p, E.Scheme(gamma=g, numerics=1))
+    r = sim.simulate_mm(0.5, sbs.dh, max_iters=steps, reinit_every=5)
+    assert r.iters == it == steps
+    assert abs(r.run_time - rt) <= 1e-12 * rt
+    mom = max(float(np.hypot(U[..., 2], U[..., 3]).max()) for U in Us)
+    scale = [max(float(np.abs(U[..., 0]).max()) for U in Us),
+             max(float(np.abs(U[..., 1]).max()) for U in Us), mom, mom]
+    worst = 0.0
+    for m in range(3):
+        g = _host(dev_U[m], 1)
+        for k in range(4):
+            worst = max(worst, float(np.abs(g[..., k] - Us[m][..., k]).max()) / scale[k])
+        gp = _host(dev_p[m])
+        worst = max(worst, float(np.abs(gp - phis[m]).max() / np.abs(phis[m]).max()))
+        assert (np.sign(gp) == np.sign(phis[m])).all()
+    print(f"fast three materials: worst rel Linf {worst:.3e}", file=sys.stderr)
+    assert worst <= 1e-12, worst
