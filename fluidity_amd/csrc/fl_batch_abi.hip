@@ -57,7 +57,10 @@ bool valid_width(int b) { return b >= 64 && b <= fl::kMaxBlock && (b % 64) == 0;
 // kMinRowTiles tiles per column band.  Tall problems pay the warm-up rows of
 // a tile (6 in x, 8 in y) less often; short ones keep short tiles, which end
 // the phase evenly.
-constexpr int kMinRowTiles = 8;
+#ifndef FL_BATCH_MIN_ROW_TILES
+#define FL_BATCH_MIN_ROW_TILES 8
+#endif
+constexpr int kMinRowTiles = FL_BATCH_MIN_ROW_TILES;
 int64_t tile_rows(int64_t ny, int base) {
   int64_t r = base;
   for (int k = 0; k < 3 && ny >= 2 * r * kMinRowTiles; ++k) r *= 2;
