@@ -34,10 +34,10 @@ def _host(T, nd, layout=1):
     return np.ascontiguousarray(a)
 
 
-def _oracle_runs(oracle, specs, steps, sim_times=None):
+def _oracle_runs(oracle, specs, steps, sim_times=None, flux=0):
     out = []
     for i, sp in enumerate(specs):
-        s = oracle.scheme(gamma=sp.gamma, bc=(0, 0))
+        s = oracle.scheme(gamma=sp.gamma, bc=(0, 0), flux=flux)
         st = float("inf") if sim_times is None else sim_times[i]
         U, rt, it, _ = oracle.simulate(sp.U, s, 0.9, sp.dh, sim_time=st, max_iters=steps)
         out.append((U, rt, it))
@@ -56,16 +56,21 @@ def _same_float(x, y):
     return (np.isnan(x) and np.isnan(y)) or x == y
 
 
-@pytest.mark.parametrize("graph", [True, False])
-@pytest.mark.parametrize("layout", [0, 1])
-def test_batch_f64_bitwise(B, oracle, graph, layout):
+@pytest.mark.parametrize("graph,layout,flux", [(True, 0, 0), (True, 1, 0), (False, 0, 0),
+                                               (False, 1, 0), (True, 1, 1), (False, 0, 1)],
+                         ids=["graph-aos-hllc", "graph-soa-hllc", "launch-aos-hllc",
+                              "launch-soa-hllc", "graph-soa-force", "launch-aos-force"])
+def test_batch_f64_bitwise(B, oracle, graph, layout, flux):
+    """Every problem of a batch (HLLC or FORCE for the whole batch) bit for
+    bit against the oracle run on that problem alone, with the per-problem
+    step count and run time."""
     specs = _specs()
     steps = 17
-    probs = B.problems_from_workload(specs, torch.float64, layout, max_iters=steps)
+    probs = B.problems_from_workload(specs, torch.float64, layout, max_iters=steps, flux=flux)
     bt = B.Batch(probs, layout=layout)
     bt.step(steps, graph=graph)
     res = bt.finish(allow_failed=True)
-    ref = _oracle_runs(oracle, specs, steps)
+    ref = _oracle_runs(oracle, specs, steps, flux=flux)
     failed = 0
     for i, sp in enumerate(specs):
         assert res[i].iters == ref[i][2], i
@@ -151,16 +156,17 @@ def test_batch_sums(B):
         np.testing.assert_allclose(sums[i, : U.shape[-1]], ref, rtol=1e-12)
 
 
-def test_batch_f32_error(B, oracle):
+@pytest.mark.parametrize("flux", [0, 1], ids=["hllc", "force"])
+def test_batch_f32_error(B, oracle, flux):
     """FP32 bulk with FP64 dt: per-problem relative L1 density error against
-    the FP64 oracle after 20 steps, bound 1e-5 per problem."""
+    the FP64 oracle after 20 steps, bound 1e-5 per problem (HLLC and FORCE)."""
     specs = _specs()
     steps = 20
-    probs = B.problems_from_workload(specs, torch.float32, 1, max_iters=steps)
+    probs = B.problems_from_workload(specs, torch.float32, 1, max_iters=steps, flux=flux)
     bt = B.Batch(probs)
     bt.step(steps)
     bt.finish(allow_failed=True)
-    ref = _oracle_runs(oracle, specs, steps)
+    ref = _oracle_runs(oracle, specs, steps, flux=flux)
     errs = []
     for i, sp in enumerate(specs):
         U = _host(bt.state(i), sp.nd)
