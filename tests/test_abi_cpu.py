@@ -176,3 +176,59 @@ def test_level_set_grid_size_limits(N):
         assert fn(2, ok) > 0
     long1d = (C.c_int64 * 3)((1 << 30) + 1, 1, 1)  # a side above 2^30
     assert N.lib.fl_ls_reinit_work_bytes(1, long1d) == -1
+
+
+def _batch_problems(N, numerics):
+    probs = (N.BatchProblem * len(numerics))()
+    for i, num in enumerate(numerics):
+        probs[i].state = N.field(0x100000 * (2 * i + 1), 2, (16, 16))
+        probs[i].work = N.field(0x100000 * (2 * i + 2), 2, (16, 16))
+        probs[i].scheme = N.scheme(numerics=num)
+        probs[i].params = N.SimParams(0.9, 1.0 / 16, 1.0, 10)
+    return probs
+
+
+def test_batch_rejects_mixed_fp64_numerics(N):
+    """One FP64 numerics per batch: a batch mixing exact and fast problems is
+    refused before any device work."""
+    probs = _batch_problems(N, [N.FL_NUMERICS_EXACT, N.FL_NUMERICS_FAST])
+    out = C.c_void_p()
+    rc = N.lib.fl_batch_create(_fake_ctx(N), probs, 2, C.byref(out), None)
+    assert rc == N.FL_ERR_UNSUPPORTED
+    assert b"numerics" in N.lib.fl_last_error()
+
+
+def test_mm_simulate_rejects_mixed_numerics(N):
+    """Every material of a two-material step takes the same numerics."""
+    mats = (N.Material * 2)()
+    for m in range(2):
+        mats[m].state = N.field(0x100000 * (3 * m + 1), 2, (16, 16))
+        mats[m].work = N.field(0x100000 * (3 * m + 2), 2, (16, 16))
+        mats[m].phi = 0x100000 * (3 * m + 3)
+        mats[m].scheme = N.scheme(numerics=N.FL_NUMERICS_FAST if m else N.FL_NUMERICS_EXACT)
+    p = N.SimParams(0.5, 1.0 / 16, 1.0, 10)
+    r = N.SimResult()
+    rc = N.lib.fl_mm_simulate(_fake_ctx(N), mats, 2, C.byref(p), None, C.c_void_p(0x7000000),
+                              1 << 30, C.byref(r), None)
+    assert rc == N.FL_ERR_ARGUMENT
+    assert b"same numerics" in N.lib.fl_last_error()
+
+
+@pytest.mark.parametrize("mutate,status,needle", [
+    (lambda s: setattr(s, "flux", 2), 2, b"HLLC and FORCE"),
+    (lambda s: setattr(s, "flux", 3), 2, b"HLLC and FORCE"),
+    (lambda s: setattr(s, "numerics", 1), 2, b"exact"),
+])
+def test_adjoint_scheme_limits(N, mutate, status, needle):
+    """The adjoint runs HLLC and FORCE (either limiter and form) in the exact
+    numerics; the standalone Lax-Friedrichs and Richtmyer fluxes and the fast
+    numerics are refused before any device work."""
+    f = N.field(0x1000, 2, (16, 16))
+    s = N.scheme()
+    mutate(s)
+    dg = C.c_double()
+    rc = N.lib.fl_split_step_vjp(_fake_ctx(N), C.byref(f), C.byref(f), C.byref(f), C.byref(s),
+                                 0.9, 1.0 / 16, C.byref(dg), C.c_void_p(0x9000000), 1 << 30, None)
+    assert rc == status
+    assert needle in N.lib.fl_last_error()
+
