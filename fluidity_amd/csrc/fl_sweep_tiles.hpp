@@ -339,6 +339,32 @@ __device__ __forceinline__ double cell_ws(const T* u, const SweepArgs<T>& a) {
   }
 }
 
+// Folds an updated cell's wavespeed into the lane's running maximum (bit
+// pattern).  Fast FP64 forms: with c^2 = gamma p / rho, a cell whose
+// c^2 <= (max - |v|max)^2 cannot raise the maximum by more than the rounding
+// of that test (a few ulps, well inside the fast numerics' bound), so a wave
+// whose every lane holds such a cell skips the square roots.  A NaN or
+// infinite value fails the test and is folded in, so it still surfaces.
+template <bool F, typename T, int ND>
+__device__ __forceinline__ void ws_fold(const T* u, const SweepArgs<T>& a,
+                                        unsigned long long& wsmax) {
+  if constexpr (F && std::is_same<T, double>::value) {
+    const fast::PrimR<double, ND> w = fast::prim_of<double, ND>(u, a.mat);
+    double mv = fabs(w.vel[0]);
+#pragma unroll
+    for (int i = 1; i < ND; ++i) mv = fast::mx(mv, fabs(w.vel[i]));
+    const double c2 = a.mat.stiff ? a.mat.gamma * (w.p + a.mat.p_inf) * w.rinv
+                                  : a.mat.gamma * w.p * w.rinv;
+    const double t = __longlong_as_double((long long)wsmax) - mv;
+    if (__all(t > 0.0 && c2 <= t * t)) return;
+    const unsigned long long b = ws_bits(mv + fast::sqt(c2));
+    wsmax = b > wsmax ? b : wsmax;
+  } else {
+    const unsigned long long b = ws_bits(cell_ws<F, T, ND>(u, a));
+    wsmax = b > wsmax ? b : wsmax;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ bool sweep_dtdh(const SweepArgs<T>& a, T& dtdh) {
   if (a.ctl) {
@@ -554,8 +580,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, int64_t bx, 
       if constexpr (ND == 2) store_row<T, S>(a, jo, out_off, o);
       else store_cell<T, S>(a.out, jo * a.nx + cr, a.es, a.cs, o);
       if (a.ws_out) {
-        const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
-        wsmax = b > wsmax ? b : wsmax;
+        ws_fold<F, T, ND>(o, a, wsmax);
       }
     }
   }
@@ -723,8 +748,7 @@ __device__ __forceinline__ void sweep_y_tile(const SweepArgs<T>& a, int64_t bx, 
           for (int k = 0; k < S; ++k) o[k] = um[k] + (Gprev[k] - G[k]) * dtdh;
           store_row<T, S>(a, jo, out_off, o);
           if (a.ws_out) {
-            const unsigned long long b = ws_bits(cell_ws<F, T, ND>(o, a));
-            wsmax = b > wsmax ? b : wsmax;
+            ws_fold<F, T, ND>(o, a, wsmax);
           }
         }
 #pragma unroll
