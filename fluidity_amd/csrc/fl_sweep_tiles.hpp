@@ -340,11 +340,12 @@ __device__ __forceinline__ double cell_ws(const T* u, const SweepArgs<T>& a) {
 }
 
 // Folds an updated cell's wavespeed into the lane's running maximum (bit
-// pattern).  Fast FP64 forms: with c^2 = gamma p / rho, a cell whose
-// c^2 <= (max - |v|max)^2 cannot raise the maximum by more than the rounding
-// of that test (a few ulps, well inside the fast numerics' bound), so a wave
-// whose every lane holds such a cell skips the square roots.  A NaN or
-// infinite value fails the test and is folded in, so it still surfaces.
+// pattern).  Fast FP64 forms: with c^2 = gamma p / rho and t = max - |v|max,
+// a cell whose c^2 <= t^2 (1 + 1e-14) cannot raise the maximum by more than
+// about 5e-15 of it (well inside the fast numerics' bound), so a wave whose
+// every lane holds such a cell skips the square roots.  The slack admits the
+// cells of a uniform region, whose t rounds to just below their own c.  A
+// NaN or infinite value fails the test and is folded in, so it surfaces.
 template <bool F, typename T, int ND>
 __device__ __forceinline__ void ws_fold(const T* u, const SweepArgs<T>& a,
                                         unsigned long long& wsmax) {
@@ -356,7 +357,7 @@ __device__ __forceinline__ void ws_fold(const T* u, const SweepArgs<T>& a,
     const double c2 = a.mat.stiff ? a.mat.gamma * (w.p + a.mat.p_inf) * w.rinv
                                   : a.mat.gamma * w.p * w.rinv;
     const double t = __longlong_as_double((long long)wsmax) - mv;
-    if (__all(t > 0.0 && c2 <= t * t)) return;
+    if (__all(t > 0.0 && c2 <= t * t * (1.0 + 1e-14))) return;
     const unsigned long long b = ws_bits(mv + fast::sqt(c2));
     wsmax = b > wsmax ? b : wsmax;
   } else {
