@@ -424,21 +424,21 @@ int launch_max_ws2(const fl_field* f0, const fl_field* f1, const fl_scheme* s0,
   const fl::Material<double> m0 = make_material<double>(s0), m1 = make_material<double>(s1);
   const dim3 grid((unsigned)blocks);
   if (f0->dtype == FL_F64) {
-    if (f0->ndim == 1)
-      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 1>), grid, dim3(B), 0, stream,
-                         (const double*)f0->data, (const double*)f1->data, es0, cs0, es1, cs1,
-                         n, m0, m1, dst);
-    else
-      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<double, 2>), grid, dim3(B), 0, stream,
-                         (const double*)f0->data, (const double*)f1->data, es0, cs0, es1, cs1,
-                         n, m0, m1, dst);
+    // the fast forms when both materials ask for the fast numerics
+    const bool fast = fast_numerics(s0) && fast_numerics(s1);
+    auto k = f0->ndim == 1 ? (fast ? fl::max_wavespeed2_kernel<double, 1, true>
+                                   : fl::max_wavespeed2_kernel<double, 1, false>)
+                           : (fast ? fl::max_wavespeed2_kernel<double, 2, true>
+                                   : fl::max_wavespeed2_kernel<double, 2, false>);
+    hipLaunchKernelGGL(k, grid, dim3(B), 0, stream, (const double*)f0->data,
+                       (const double*)f1->data, es0, cs0, es1, cs1, n, m0, m1, dst);
   } else {
     if (f0->ndim == 1)
-      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 1>), grid, dim3(B), 0, stream,
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 1, false>), grid, dim3(B), 0, stream,
                          (const float*)f0->data, (const float*)f1->data, es0, cs0, es1, cs1,
                          n, m0, m1, dst);
     else
-      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 2>), grid, dim3(B), 0, stream,
+      hipLaunchKernelGGL((fl::max_wavespeed2_kernel<float, 2, false>), grid, dim3(B), 0, stream,
                          (const float*)f0->data, (const float*)f1->data, es0, cs0, es1, cs1,
                          n, m0, m1, dst);
   }

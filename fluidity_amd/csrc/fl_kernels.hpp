@@ -88,7 +88,7 @@ template <typename T, int ND, int FLUX, bool FAST, bool VLC>
 __global__ void sweep_x2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
 template <typename T, int FLUX, bool FAST, bool VLC>
 __global__ void sweep_y2_kernel(SweepArgs<T> a0, SweepArgs<T> a1);
-template <typename T, int ND>
+template <typename T, int ND, bool FAST>
 __global__ void max_wavespeed2_kernel(const T* in0, const T* in1, int64_t es0, int64_t cs0,
                                       int64_t es1, int64_t cs1, int64_t ncells,
                                       Material<double> m0, Material<double> m1,

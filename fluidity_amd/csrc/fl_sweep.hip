@@ -104,7 +104,16 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed_kernel(const T* __res
 // The max over two grids of one shape (the two materials of a two-material
 // step; each grid with its own element and component strides): each thread
 // takes its cells of both.
-template <typename T, int ND>
+template <typename T, int ND, bool FAST>
+__device__ __forceinline__ double ws2_cell(const T* u, const Material<double>& m) {
+  if constexpr (FAST && std::is_same<T, double>::value)
+    return fast::max_wavespeed<double, ND>(fast::prim_of<double, ND>(u, m), m);
+  else
+    return cell_wavespeed64<T, ND>(u, m);
+}
+
+// FAST (FP64): the fast forms of the wavespeed (fast numerics requested).
+template <typename T, int ND, bool FAST>
 __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
     const T* __restrict__ in0, const T* __restrict__ in1, int64_t es0, int64_t cs0, int64_t es1,
     int64_t cs1, int64_t ncells, Material<double> m0, Material<double> m1,
@@ -122,7 +131,7 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], (k & 1) ? m1 : m0));
+      const unsigned long long b = ws_bits(ws2_cell<T, ND, FAST>(u[k], (k & 1) ? m1 : m0));
       v = b > v ? b : v;
     }
   }
@@ -132,7 +141,7 @@ __global__ void __launch_bounds__(kMaxBlock) max_wavespeed2_kernel(
     load_cell<T, 2 + ND>(in1, i, es1, cs1, u[1]);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const unsigned long long b = ws_bits(cell_wavespeed64<T, ND>(u[k], k ? m1 : m0));
+      const unsigned long long b = ws_bits(ws2_cell<T, ND, FAST>(u[k], k ? m1 : m0));
       v = b > v ? b : v;
     }
   }
@@ -406,17 +415,18 @@ FL_INST_SWEEPS2_V(double, kFluxRichtmyer, true, false)
 FL_INST_SWEEPS2_V(double, kFluxHllc, true, true)
 FL_INST_SWEEPS2_V(double, kFluxForce, true, true)
 
-#define FL_INST_MAXWS2(T)                                                                 \
-  template __global__ void max_wavespeed2_kernel<T, 1>(const T*, const T*, int64_t, int64_t, \
-                                                       int64_t, int64_t, int64_t,            \
-                                                       Material<double>, Material<double>,   \
-                                                       unsigned long long*);                 \
-  template __global__ void max_wavespeed2_kernel<T, 2>(const T*, const T*, int64_t, int64_t, \
-                                                       int64_t, int64_t, int64_t,            \
-                                                       Material<double>, Material<double>,   \
-                                                       unsigned long long*);
+#define FL_INST_MAXWS2_F(T, FAST)                                                          \
+  template __global__ void max_wavespeed2_kernel<T, 1, FAST>(                                 \
+      const T*, const T*, int64_t, int64_t, int64_t, int64_t, int64_t, Material<double>,      \
+      Material<double>, unsigned long long*);                                                \
+  template __global__ void max_wavespeed2_kernel<T, 2, FAST>(                                 \
+      const T*, const T*, int64_t, int64_t, int64_t, int64_t, int64_t, Material<double>,      \
+      Material<double>, unsigned long long*);
+#define FL_INST_MAXWS2(T) FL_INST_MAXWS2_F(T, false)
+
 FL_INST_MAXWS2(double)
 FL_INST_MAXWS2(float)
+FL_INST_MAXWS2_F(double, true)
 
 #define FL_INST_PERSIST(T)                                                                   \
   template __global__ void persist1d_kernel<T, kFluxHllc>(SweepArgs<T>, SweepArgs<T>,        \

@@ -65,7 +65,8 @@ typedef enum fl_eos { FL_EOS_IDEAL = 0, FL_EOS_STIFF = 1, FL_EOS_STIFF_CORRECTED
  * do batches (every FP64 problem of a batch must ask for the same numerics,
  * else FL_ERR_UNSUPPORTED) and the two-material loop (every material the
  * same, else FL_ERR_ARGUMENT; FAST there also takes the level sets' WENO
- * right-hand side in its fast form); the adjoint entry points run FP64 exact
+ * right-hand side and, with two materials, the wavespeed maximum that sets
+ * dt in their fast forms); the adjoint entry points run FP64 exact
  * and reject FAST with FL_ERR_UNSUPPORTED.  An extension: the reference has
  * one arithmetic. */
 typedef enum fl_numerics { FL_NUMERICS_EXACT = 0, FL_NUMERICS_FAST = 1 } fl_numerics;
